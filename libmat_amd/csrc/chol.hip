@@ -1,0 +1,107 @@
+// Lower Cholesky factorization A = L L^T (mat_chol, mat.h:6062-6095).
+//
+// Reference semantics (x86 path mat__chol_scalar_generic_, mat.h:6031-6060):
+// only the lower triangle of A is read; column j's pivot is
+// d = a(j,j) - sum_k l(j,k)^2, the factorization fails (-1) at the first
+// d <= 0, l(j,j) = sqrt(d) and the column below is multiplied by 1/l(j,j).
+//
+// MI355X design: recursive blocking.  A 64-column diagonal block is
+// factored by one workgroup in LDS (right-looking: scale the column, rank-1
+// update of the trailing lower triangle); the panel below it is solved by
+// the triangular-solve kernels (A21 := A21 L11^-T), and the trailing lower
+// triangle is updated by one MFMA GEMM restricted to tiles on or below the
+// diagonal (the reference's SYRK, mat.h:5471-5487).  The first failing
+// column is recorded in a device flag that later leaves check, so work stops
+// at the same column the reference stops at.
+#include "ops.hpp"
+
+namespace mg {
+
+constexpr int CHOL_NB = 64;
+
+template <typename T>
+__global__ __launch_bounds__(256) void chol_leaf_kernel(int n, T *A, long lda, int j0, int *info) {
+  constexpr int NB = CHOL_NB;
+  __shared__ T S[NB * (NB + 1)];  // S[c*(NB+1) + r] = A(r, c)
+  __shared__ T s_inv;
+  __shared__ int s_fail;
+  if (*(volatile int *)info != 0) return;  // an earlier column already failed
+  const int t = threadIdx.x;
+  for (int idx = t; idx < NB * NB; idx += 256) {
+    const int r = idx % NB, c = idx / NB;
+    S[c * (NB + 1) + r] = (r < n && c < n && r >= c) ? A[r + (long)c * lda] : T(0);
+  }
+  if (t == 0) s_fail = 0;
+  __syncthreads();
+  for (int j = 0; j < n; ++j) {
+    if (t == 0) {
+      const T d = S[j * (NB + 1) + j];
+      if (d <= T(0)) {
+        s_fail = 1;
+        *info = j0 + j + 1;
+      } else {
+        const T l = sqrt(d);
+        S[j * (NB + 1) + j] = l;
+        s_inv = T(1) / l;
+      }
+    }
+    __syncthreads();
+    if (s_fail) return;
+    const T inv = s_inv;
+    for (int i = j + 1 + t; i < n; i += 256) S[j * (NB + 1) + i] *= inv;
+    __syncthreads();
+    // trailing lower triangle: a(i,c) -= l(i,j) l(c,j), j < c <= i
+    const int m = n - j - 1;
+    for (int idx = t; idx < m * m; idx += 256) {
+      const int i = j + 1 + idx % m, c = j + 1 + idx / m;
+      if (i >= c) S[c * (NB + 1) + i] = fma(-S[j * (NB + 1) + i], S[j * (NB + 1) + c], S[c * (NB + 1) + i]);
+    }
+    __syncthreads();
+  }
+  for (int idx = t; idx < NB * NB; idx += 256) {
+    const int r = idx % NB, c = idx / NB;
+    if (r < n && c < n && r >= c) A[r + (long)c * lda] = S[c * (NB + 1) + r];
+  }
+}
+
+template <typename T>
+static void potrf_rec(hipStream_t s, int n, T *A, long lda, int j0, int *info, Mul mul) {
+  if (n <= CHOL_NB) {
+    hipLaunchKernelGGL((chol_leaf_kernel<T>), dim3(1), dim3(256), 0, s, n, A, lda, j0, info);
+    MG_HIP(hipGetLastError());
+    return;
+  }
+  int h = ((n / 2 + CHOL_NB - 1) / CHOL_NB) * CHOL_NB;
+  if (h >= n) h = n - CHOL_NB;
+  const int m = n - h;
+  potrf_rec<T>(s, h, A, lda, j0, info, mul);
+  T *A21 = A + h;
+  T *A22 = A + h + (long)h * lda;
+  // A21 := A21 L11^-T  <=>  L11 * A21^T = A21^T (A21^T is row-major in A21's storage)
+  trsm_left<T>(s, true, false, h, m, View<const T>{A, 1, lda}, View<T>{A21, lda, 1}, mul);
+  // A22 -= A21 A21^T on the lower triangle
+  View<const T> a21{A21, 1, lda};
+  if constexpr (std::is_same<T, float>::value) {
+    if (mul == Mul::Fp8) {
+      gemm_fp8(s, m, m, h, -1.0f, a21, a21.t(), View<float>{A22, 1, lda}, true);
+    } else if (mul != Mul::Native) {
+      gemm_bf16x(s, m, m, h, -1.0f, a21, a21.t(), View<float>{A22, 1, lda}, (int)mul);
+    } else {
+      gemm<T>(s, m, m, h, T(-1), a21, a21.t(), T(1), View<T>{A22, 1, lda}, true);
+    }
+  } else {
+    gemm<T>(s, m, m, h, T(-1), a21, a21.t(), T(1), View<T>{A22, 1, lda}, true);
+  }
+  potrf_rec<T>(s, m, A22, lda, j0 + h, info, mul);
+}
+
+template <typename T>
+void potrf_lower(hipStream_t s, int n, T *A, long lda, int *info, Mul mul) {
+  fill_int(s, info, 1, 0);
+  potrf_rec<T>(s, n, A, lda, 0, info, mul);
+}
+
+template void potrf_lower<float>(hipStream_t, int, float *, long, int *, Mul);
+template void potrf_lower<double>(hipStream_t, int, double *, long, int *, Mul);
+
+}  // namespace mg

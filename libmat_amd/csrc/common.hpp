@@ -1,0 +1,103 @@
+// Shared definitions for the gfx950 HIP implementation of libmat's dense
+// GEMM + factor-and-solve path.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+
+namespace mg {
+
+// ---------------------------------------------------------------- errors --
+// Every HIP failure inside the library is turned into a C++ exception at
+// the call site; the C-ABI layer converts it into a status code (extension
+// entry points) or an assert-like abort (the drop-in libmat symbols, whose
+// reference signatures have no error channel, mat.h:328-357).
+struct Error {
+  int code;
+  std::string msg;
+};
+
+void set_last_error(const std::string &m);
+const char *last_error();
+
+#define MG_HIP(expr)                                                          \
+  do {                                                                        \
+    hipError_t e_ = (expr);                                                   \
+    if (e_ != hipSuccess)                                                     \
+      throw ::mg::Error{-1, std::string(#expr) + ": " + hipGetErrorString(e_) \
+                                + " (" __FILE__ ":" + std::to_string(__LINE__) + ")"}; \
+  } while (0)
+
+#define MG_CHECK(cond, msg)                                                   \
+  do {                                                                        \
+    if (!(cond)) throw ::mg::Error{-2, std::string(msg)};                     \
+  } while (0)
+
+// ------------------------------------------------------------ workspace --
+// Device scratch owned by the library, one set per (device, stream):
+// named slots that grow on demand and are reused by later calls on the
+// same stream (stream order makes reuse safe).  Freed at process exit.
+struct Workspace {
+  std::unordered_map<std::string, std::pair<void *, size_t>> slots;
+  void *get(const std::string &name, size_t bytes);
+  ~Workspace();
+};
+Workspace &workspace(hipStream_t s);
+
+// Library-owned default stream for the drop-in (synchronous) entry points.
+hipStream_t default_stream();
+
+// Number of compute units of the current device.
+int num_cus();
+
+// ------------------------------------------------------------ strided views
+// A matrix operand addressed as p[i*rs + j*cs]; exactly one of rs/cs is 1.
+template <typename T>
+struct View {
+  T *p;
+  long rs, cs;
+  __host__ __device__ T &operator()(long i, long j) const { return p[i * rs + j * cs]; }
+  View sub(long i, long j) const { return View{p + i * rs + j * cs, rs, cs}; }
+  View t() const { return View{p, cs, rs}; }
+};
+template <typename T>
+View<T> colmajor(T *p, long ld) { return View<T>{p, 1, ld}; }
+template <typename T>
+View<T> rowmajor(T *p, long ld) { return View<T>{p, ld, 1}; }
+template <typename T>
+View<const T> cview(const View<T> &v) { return View<const T>{v.p, v.rs, v.cs}; }
+
+inline unsigned ceil_div(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
+
+// ------------------------------------------------------------ GEMM entry --
+// C = alpha * A * B + beta * C on arbitrary strided views (M x K, K x N,
+// M x N).  beta == 0 never reads C.  If lower_only, only elements with
+// row >= col of C are written (tiles strictly above the diagonal skipped).
+template <typename T>
+void gemm(hipStream_t s, long M, long N, long K, T alpha, View<const T> A,
+          View<const T> B, T beta, View<T> C, bool lower_only = false);
+
+// Low-precision GEMM used by the mixed solvers: C(fp32) += alpha * A * B with
+// A, B given in fp32 and multiplied on BF16 MFMA after an exact split into
+// `passes` (1, 3 or 6) bf16 terms; accumulation in fp32.
+void gemm_bf16x(hipStream_t s, long M, long N, long K, float alpha,
+                View<const float> A, View<const float> B, View<float> C,
+                int passes);
+
+// FP8 e4m3 GEMM with one scale per 128x128 tile of each operand (computed
+// on the fly), fp32 accumulation: C += alpha * A * B.
+void gemm_fp8(hipStream_t s, long M, long N, long K, float alpha,
+              View<const float> A, View<const float> B, View<float> C,
+              bool lower_only);
+
+// ---------------------------------------------------- precision policy --
+// How the trailing updates of a factorization are multiplied.
+enum class Mul { Native = 0, Bf16x1 = 1, Bf16x3 = 3, Bf16x6 = 6, Fp8 = 8 };
+
+}  // namespace mg

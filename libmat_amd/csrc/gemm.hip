@@ -1,0 +1,302 @@
+// FP32 / FP64 MFMA GEMM for gfx950.
+//
+// Serves mat_mul / mat_gemm (mat.h:1842-1850, 2787-2796) and every trailing
+// update of the LU / Cholesky factorizations and triangular solves.
+//
+// Shape of the kernel:
+//  * 256 threads = 4 wavefronts in a 2x2 grid; a workgroup owns a BM x BN
+//    tile of C, each wavefront a (BM/2) x (BN/2) sub-tile held in MFMA
+//    accumulators (fp32: v_mfma_f32_32x32x2f32, fp64: v_mfma_f64_16x16x4f64).
+//  * K is walked in BK-deep slabs, double-buffered in LDS: the next slab's
+//    global loads are issued before the current slab's MFMAs, then parked in
+//    the other LDS buffer; one barrier per slab.
+//  * Both operands live in LDS as [k][mn] (mn contiguous) so every MFMA
+//    fragment read is a conflict-free 32-lane contiguous ds_read.  Global
+//    tiles that are k-contiguous are transposed on the way in.
+//  * Inside a slab, lane group g = lane / FM feeds k = g*(BK/G) + s at step
+//    s; A and B use the same assignment, so the product is exact for any
+//    assignment and each lane reads one scalar per fragment.
+//  * Tiles are mapped XCD-aware: consecutive workgroup ids land on different
+//    XCDs, so ids are regrouped so that each XCD's L2 sees a compact band of
+//    C tiles (grouped along M) that shares A rows and B columns.
+#include "common.hpp"
+
+namespace mg {
+
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+template <typename T>
+struct Mf;
+template <>
+struct Mf<float> {
+  static constexpr int FM = 32;
+  static constexpr int NACC = 16;
+  using acc_t = f32x16;
+  __device__ static acc_t mma(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c, 0, 0, 0);
+  }
+  // accumulator register r of lane -> row inside the 32x32 fragment
+  __device__ static int row(int r, int lane) { return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3); }
+};
+template <>
+struct Mf<double> {
+  static constexpr int FM = 16;
+  static constexpr int NACC = 4;
+  using acc_t = f64x4;
+  __device__ static acc_t mma(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  // v_mfma_f64_16x16x4f64 interleaves rows across the four lane groups:
+  // register r of lane l holds row (l/16) + 4r, column l%16
+  __device__ static int row(int r, int lane) { return (lane >> 4) + 4 * r; }
+};
+
+template <typename T>
+struct GemmP {
+  const T *A;
+  const T *B;
+  T *C;
+  long lda, ldb, ldc;
+  int M, N, K;
+  T alpha, beta;
+  int tiles_m, tiles_n, tiles;
+  int lower;
+};
+
+// Loads one BK x MNT operand slab from global memory into registers.
+// MN=true: element (mn, k) at p[mn + k*ld]; MN=false: at p[k + mn*ld].
+template <typename T, int MNT, int BK, bool MN, int VW>
+struct SlabLoader {
+  static constexpr int NT = 256;
+  static constexpr int NV = MNT * BK / (NT * VW);
+  static_assert(NV * NT * VW == MNT * BK, "slab must split evenly");
+  T r[NV][VW];
+
+  __device__ void load(const T *p, long ld, int mn0, int k0, int MNdim, int Kdim) {
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      const int idx = t + NT * v;
+      int mm, kk;
+      if (MN) {
+        kk = idx / (MNT / VW);
+        mm = (idx % (MNT / VW)) * VW;
+      } else {
+        mm = idx / (BK / VW);
+        kk = (idx % (BK / VW)) * VW;
+      }
+      const int gmn = mn0 + mm, gk = k0 + kk;
+      const T *src = MN ? p + (long)gmn + (long)gk * ld : p + (long)gk + (long)gmn * ld;
+      bool full = MN ? (gmn + VW <= MNdim && gk < Kdim) : (gmn < MNdim && gk + VW <= Kdim);
+      if (VW > 1 && full) {
+        if constexpr (VW == 4 && sizeof(T) == 4) {
+          float4 x = *reinterpret_cast<const float4 *>(src);
+          r[v][0] = x.x; r[v][1] = x.y; r[v][2] = x.z; r[v][3] = x.w;
+        } else if constexpr (VW == 2 && sizeof(T) == 8) {
+          double2 x = *reinterpret_cast<const double2 *>(src);
+          r[v][0] = x.x; r[v][1] = x.y;
+        }
+      } else {
+#pragma unroll
+        for (int c = 0; c < VW; ++c) {
+          bool ok = MN ? (gmn + c < MNdim && gk < Kdim) : (gmn < MNdim && gk + c < Kdim);
+          r[v][c] = ok ? src[c] : T(0);
+        }
+      }
+    }
+  }
+
+  // Store into LDS laid out as S[k][mn] with row stride LDS_LD.
+  template <int LDS_LD>
+  __device__ void store(T *S) const {
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      const int idx = t + NT * v;
+      if (MN) {
+        const int kk = idx / (MNT / VW);
+        const int mm = (idx % (MNT / VW)) * VW;
+#pragma unroll
+        for (int c = 0; c < VW; ++c) S[kk * LDS_LD + mm + c] = r[v][c];
+      } else {
+        const int mm = idx / (BK / VW);
+        const int kk = (idx % (BK / VW)) * VW;
+#pragma unroll
+        for (int c = 0; c < VW; ++c) S[(kk + c) * LDS_LD + mm] = r[v][c];
+      }
+    }
+  }
+};
+
+// Workgroup id -> (tile_m, tile_n) with XCD regrouping and GROUP_M bands.
+__device__ inline bool map_tile(int bid, int grid, int tiles, int tiles_m, int tiles_n,
+                                int &tm, int &tn) {
+  constexpr int XCD = 8, GROUP_M = 8;
+  int id = bid;
+  if (grid % XCD == 0) id = (bid % XCD) * (grid / XCD) + bid / XCD;
+  if (id >= tiles) return false;
+  const int per_group = GROUP_M * tiles_n;
+  const int g = id / per_group;
+  const int first_m = g * GROUP_M;
+  const int gsz = min(tiles_m - first_m, GROUP_M);
+  const int in = id % per_group;
+  tm = first_m + in % gsz;
+  tn = in / gsz;
+  return true;
+}
+
+template <typename T, int BM, int BN, int BK, bool A_MN, bool B_MN, bool VEC>
+__global__ __launch_bounds__(256) void gemm_kernel(GemmP<T> p) {
+  using F = Mf<T>;
+  constexpr int FM = F::FM;
+  constexpr int G = 64 / FM;
+  constexpr int SK = BK / G;
+  constexpr int WTM = BM / 2, WTN = BN / 2;
+  constexpr int TM = WTM / FM, TN = WTN / FM;
+  constexpr int VW = VEC ? 16 / (int)sizeof(T) : 1;
+  constexpr int PAD = 16 / (int)sizeof(T);
+  constexpr int LDA_S = BM + PAD, LDB_S = BN + PAD;
+
+  __shared__ T As[2][BK * LDA_S];
+  __shared__ T Bs[2][BK * LDB_S];
+
+  int tm, tn;
+  if (!map_tile(blockIdx.x, gridDim.x, p.tiles, p.tiles_m, p.tiles_n, tm, tn)) return;
+  const int bm0 = tm * BM, bn0 = tn * BN;
+  if (p.lower && bm0 + BM - 1 < bn0) return;  // tile strictly above the diagonal
+
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const int g = lane / FM, li = lane % FM;
+
+  typename F::acc_t acc[TM][TN];
+#pragma unroll
+  for (int a = 0; a < TM; ++a)
+#pragma unroll
+    for (int b = 0; b < TN; ++b)
+#pragma unroll
+      for (int r = 0; r < F::NACC; ++r) acc[a][b][r] = T(0);
+
+  SlabLoader<T, BM, BK, A_MN, VW> la;
+  SlabLoader<T, BN, BK, B_MN, VW> lb;
+  const int nk = (p.K + BK - 1) / BK;
+  if (nk > 0) {
+    la.load(p.A, p.lda, bm0, 0, p.M, p.K);
+    lb.load(p.B, p.ldb, bn0, 0, p.N, p.K);
+    la.template store<LDA_S>(As[0]);
+    lb.template store<LDB_S>(Bs[0]);
+  }
+  __syncthreads();
+
+  for (int kt = 0; kt < nk; ++kt) {
+    const int cur = kt & 1;
+    const bool more = kt + 1 < nk;
+    if (more) {
+      la.load(p.A, p.lda, bm0, (kt + 1) * BK, p.M, p.K);
+      lb.load(p.B, p.ldb, bn0, (kt + 1) * BK, p.N, p.K);
+    }
+    const T *as = As[cur];
+    const T *bs = Bs[cur];
+#pragma unroll
+    for (int s = 0; s < SK; ++s) {
+      const int k = g * SK + s;
+      T af[TM], bf[TN];
+#pragma unroll
+      for (int a = 0; a < TM; ++a) af[a] = as[k * LDA_S + wm * WTM + a * FM + li];
+#pragma unroll
+      for (int b = 0; b < TN; ++b) bf[b] = bs[k * LDB_S + wn * WTN + b * FM + li];
+#pragma unroll
+      for (int a = 0; a < TM; ++a)
+#pragma unroll
+        for (int b = 0; b < TN; ++b) acc[a][b] = F::mma(af[a], bf[b], acc[a][b]);
+    }
+    if (more) {
+      la.template store<LDA_S>(As[cur ^ 1]);
+      lb.template store<LDB_S>(Bs[cur ^ 1]);
+    }
+    __syncthreads();
+  }
+
+  // Epilogue: C = alpha*acc + beta*C (C not read when beta == 0).
+#pragma unroll
+  for (int a = 0; a < TM; ++a)
+#pragma unroll
+    for (int b = 0; b < TN; ++b) {
+      const int j = bn0 + wn * WTN + b * FM + li;
+      if (j >= p.N) continue;
+#pragma unroll
+      for (int r = 0; r < F::NACC; ++r) {
+        const int i = bm0 + wm * WTM + a * FM + F::row(r, lane);
+        if (i >= p.M || (p.lower && i < j)) continue;
+        T *c = p.C + (long)i + (long)j * p.ldc;
+        T v = p.alpha * acc[a][b][r];
+        if (p.beta != T(0)) v += p.beta * *c;
+        *c = v;
+      }
+    }
+}
+
+template <typename T, bool A_MN, bool B_MN>
+static void launch_mode(hipStream_t s, GemmP<T> &p, bool vec) {
+  constexpr int BM = 128, BN = 128, BK = 16;
+  p.tiles_m = ceil_div(p.M, BM);
+  p.tiles_n = ceil_div(p.N, BN);
+  p.tiles = p.tiles_m * p.tiles_n;
+  int grid = p.tiles;
+  if (grid >= 8) grid = (grid + 7) / 8 * 8;
+  if (vec)
+    hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, A_MN, B_MN, true>), dim3(grid), dim3(256), 0, s, p);
+  else
+    hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, A_MN, B_MN, false>), dim3(grid), dim3(256), 0, s, p);
+  MG_HIP(hipGetLastError());
+}
+
+template <typename T>
+void gemm(hipStream_t s, long M, long N, long K, T alpha, View<const T> A, View<const T> B,
+          T beta, View<T> C, bool lower_only) {
+  if (M <= 0 || N <= 0) return;
+  if (C.rs != 1) {
+    // Row-major C: compute C^T = B^T A^T into a column-major view.
+    MG_CHECK(C.cs == 1 && !lower_only, "gemm: unsupported C layout");
+    gemm<T>(s, N, M, K, alpha, B.t(), A.t(), beta, View<T>{C.p, 1, C.rs}, false);
+    return;
+  }
+  MG_CHECK(M < (1L << 31) && N < (1L << 31) && K < (1L << 31), "gemm: dimension too large");
+  GemmP<T> p{};
+  p.C = C.p;
+  p.ldc = C.cs;
+  p.M = (int)M;
+  p.N = (int)N;
+  p.K = (int)K;
+  p.alpha = alpha;
+  p.beta = beta;
+  p.lower = lower_only ? 1 : 0;
+  // A: element (i,k).  MN-contiguous when rs == 1.
+  const bool a_mn = (A.rs == 1);
+  p.A = A.p;
+  p.lda = a_mn ? A.cs : A.rs;
+  MG_CHECK(a_mn || A.cs == 1, "gemm: A needs a unit stride");
+  // B: element (k,j).  K-contiguous when rs == 1, else contiguous along j.
+  const bool b_mn = (B.rs != 1);
+  p.B = B.p;
+  p.ldb = b_mn ? B.rs : B.cs;
+  MG_CHECK(!b_mn || B.cs == 1, "gemm: B needs a unit stride");
+  constexpr int VW = 16 / sizeof(T);
+  auto aligned = [&](const void *ptr, long ld) {
+    return ((uintptr_t)ptr % 16 == 0) && (ld % VW == 0);
+  };
+  const bool vec = aligned(p.A, p.lda) && aligned(p.B, p.ldb);
+  if (a_mn && !b_mn) launch_mode<T, true, false>(s, p, vec);
+  else if (a_mn && b_mn) launch_mode<T, true, true>(s, p, vec);
+  else if (!a_mn && !b_mn) launch_mode<T, false, false>(s, p, vec);
+  else launch_mode<T, false, true>(s, p, vec);
+}
+
+template void gemm<float>(hipStream_t, long, long, long, float, View<const float>,
+                          View<const float>, float, View<float>, bool);
+template void gemm<double>(hipStream_t, long, long, long, double, View<const double>,
+                           View<const double>, double, View<double>, bool);
+
+}  // namespace mg

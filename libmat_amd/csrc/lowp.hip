@@ -1,0 +1,492 @@
+// Low-precision MFMA GEMMs for the mixed-precision solvers:
+//   C(fp32) += alpha * A * B, A and B given in fp32.
+//
+// BF16 path (LU trailing updates): every fp32 operand is split exactly into
+// bf16 terms x = x0 + x1 + x2 (x0 = bf16(x), x1 = bf16(x - x0), ...).
+// passes = 1 multiplies x0*y0 only (bf16 accuracy, ~2^-8); passes = 3 adds
+// x0*y1 + x1*y0 (~2^-16); passes = 6 adds x0*y2 + x1*y1 + x2*y0 (~fp32).
+// Each bf16 x bf16 product is exact in fp32 and the MFMA accumulates in
+// fp32, so the extra passes buy accuracy at 1/3 or 1/6 of the bf16 rate
+// (still 5x / 2.6x the fp32 MFMA rate).
+//
+// FP8 path (Cholesky trailing updates): each operand is quantized to
+// e4m3 with one power-of-two scale per 128 x 128 tile (128 rows/columns by
+// 128 of K); the MFMA sums one 128-deep K block in fp32 and the epilogue of
+// that block multiplies by the two tile scales before adding into the
+// running fp32 accumulator.
+//
+// Both run on pre-packed operands: a pack kernel writes each operand as
+// [mn][k] planes (k contiguous, zero padded to whole tiles), so the GEMM
+// kernel moves 16-byte rows global -> LDS -> MFMA fragments with no
+// bounds checks and no transposes.
+#include <hip/hip_bf16.h>
+
+#include "ops.hpp"
+
+namespace mg {
+
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef int i32x8 __attribute__((ext_vector_type(8)));
+
+constexpr int LP_BM = 128, LP_BN = 128;
+
+__device__ inline unsigned short f32_to_bf16_rne(float x) {
+  unsigned u = __float_as_uint(x);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (unsigned short)((u >> 16) | ((u & 0xffff) ? 0x40 : 0));
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+__device__ inline float bf16_to_f32(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
+
+// ---------------------------------------------------------------- pack --
+// Src element (mn, k) = src[mn*rs + k*cs] (a strided view with one unit
+// stride).  Dst plane p: dst[p*plane + mn*Kp + k], bf16 bits.
+__global__ __launch_bounds__(256) void pack_bf16_kernel(const float *src, long rs, long cs, int MN,
+                                                        int K, int Kp, int nplanes, long plane,
+                                                        unsigned short *dst) {
+  __shared__ float tile[32][33];
+  const int m0 = blockIdx.y * 32, k0 = blockIdx.x * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  // read coalesced along the source's unit stride
+  const bool mn_unit = (rs == 1);
+  for (int yy = ty; yy < 32; yy += 8) {
+    int mm, kk;
+    if (mn_unit) {
+      mm = m0 + tx;
+      kk = k0 + yy;
+    } else {
+      kk = k0 + tx;
+      mm = m0 + yy;
+    }
+    float v = 0.f;
+    if (mm < MN && kk < K) v = src[(long)mm * rs + (long)kk * cs];
+    if (mn_unit) tile[tx][yy] = v;  // tile[mn][k]
+    else tile[yy][tx] = v;
+  }
+  __syncthreads();
+  for (int yy = ty; yy < 32; yy += 8) {
+    const int mm = m0 + yy, kk = k0 + tx;
+    float x = tile[yy][tx];
+    const long o = (long)mm * Kp + kk;
+    for (int p = 0; p < nplanes; ++p) {
+      const unsigned short h = f32_to_bf16_rne(x);
+      dst[p * plane + o] = h;
+      x = x - bf16_to_f32(h);
+    }
+  }
+}
+
+// FP8 e4m3 tile scales: one power of two per (128-row tile, 128-deep K
+// block) so that max|x| / scale <= 448.
+__global__ __launch_bounds__(256) void fp8_scale_kernel(const float *src, long rs, long cs, int MN,
+                                                        int K, int ktiles, float *scale) {
+  const int tm = blockIdx.y, tk = blockIdx.x;
+  float mx = 0.f;
+  for (int idx = threadIdx.x; idx < 128 * 128; idx += 256) {
+    int mm, kk;
+    if (rs == 1) {
+      mm = tm * 128 + idx % 128;
+      kk = tk * 128 + idx / 128;
+    } else {
+      kk = tk * 128 + idx % 128;
+      mm = tm * 128 + idx / 128;
+    }
+    if (mm < MN && kk < K) mx = fmaxf(mx, fabsf(src[(long)mm * rs + (long)kk * cs]));
+  }
+  __shared__ float red[256];
+  red[threadIdx.x] = mx;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (threadIdx.x < off) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + off]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    // smallest power of two s with max|x| / s <= 448
+    const float m = red[0];
+    float s = 1.f;
+    if (m > 0.f && m < 3.0e38f) {
+      int e;
+      const float f = frexpf(m / 448.f, &e);  // m/448 = f * 2^e, f in [0.5, 1)
+      s = ldexpf(1.f, f == 0.5f ? e - 1 : e);
+    }
+    scale[tm * ktiles + tk] = s;
+  }
+}
+
+__device__ inline unsigned char f32_to_e4m3(float x) {
+  // round-to-nearest-even into OCP e4m3 (bias 7, max 448, no infinities)
+  const unsigned u = __float_as_uint(x);
+  const unsigned sign = (u >> 24) & 0x80u;
+  float a = fabsf(x);
+  if (!(a == a)) return (unsigned char)(sign | 0x7f);
+  if (a >= 448.f) return (unsigned char)(sign | 0x7e);
+  if (a < 0.015625f) {  // below 2^-6: subnormals, step 2^-9 (q == 8 encodes 2^-6)
+    const float q = rintf(a * 512.f);
+    return (unsigned char)(sign | (unsigned)q);
+  }
+  int e;
+  float m = frexpf(a, &e);  // a = m * 2^e, m in [0.5, 1)
+  int E = e - 1 + 7;        // biased exponent of the 1.mmm form, >= 1 here
+  float frac = m * 2.f - 1.f;  // [0,1)
+  float q = rintf(frac * 8.f);
+  if (q >= 8.f) {
+    q = 0.f;
+    E += 1;
+  }
+  if (E > 15 || (E == 15 && q > 6.f)) return (unsigned char)(sign | 0x7e);
+  return (unsigned char)(sign | (E << 3) | (unsigned)q);
+}
+
+__global__ __launch_bounds__(256) void pack_fp8_kernel(const float *src, long rs, long cs, int MN,
+                                                       int K, int Kp, int ktiles,
+                                                       const float *scale, unsigned char *dst) {
+  __shared__ float tile[32][33];
+  const int m0 = blockIdx.y * 32, k0 = blockIdx.x * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const bool mn_unit = (rs == 1);
+  for (int yy = ty; yy < 32; yy += 8) {
+    int mm, kk;
+    if (mn_unit) {
+      mm = m0 + tx;
+      kk = k0 + yy;
+    } else {
+      kk = k0 + tx;
+      mm = m0 + yy;
+    }
+    float v = 0.f;
+    if (mm < MN && kk < K) v = src[(long)mm * rs + (long)kk * cs];
+    if (mn_unit) tile[tx][yy] = v;
+    else tile[yy][tx] = v;
+  }
+  __syncthreads();
+  for (int yy = ty; yy < 32; yy += 8) {
+    const int mm = m0 + yy, kk = k0 + tx;
+    const float s = scale[(mm / 128) * ktiles + kk / 128];
+    dst[(long)mm * Kp + kk] = f32_to_e4m3(tile[yy][tx] / s);
+  }
+}
+
+// ---------------------------------------------------------------- GEMM --
+struct LpP {
+  const unsigned short *A;  // [planes][Mp][Kp]
+  const unsigned short *B;  // [planes][Np][Kp]
+  long planeA, planeB;
+  float *C;
+  long ldc;  // C column-major (rs = 1)
+  int M, N, Kp;
+  float alpha;
+  int tiles_m, tiles_n, tiles;
+  int lower;
+};
+
+__device__ inline bool lp_map_tile(int bid, int grid, int tiles, int tiles_m, int tiles_n, int &tm,
+                                   int &tn) {
+  constexpr int XCD = 8, GROUP_M = 8;
+  int id = bid;
+  if (grid % XCD == 0) id = (bid % XCD) * (grid / XCD) + bid / XCD;
+  if (id >= tiles) return false;
+  const int per_group = GROUP_M * tiles_n;
+  const int g = id / per_group;
+  const int first_m = g * GROUP_M;
+  const int gsz = min(tiles_m - first_m, GROUP_M);
+  const int in = id % per_group;
+  tm = first_m + in % gsz;
+  tn = in / gsz;
+  return true;
+}
+
+// 256 threads, 2x2 wavefronts, 64x64 per wavefront (2x2 MFMA 32x32x16).
+// NP planes per operand; PASSES products per K step (see file comment).
+template <int NP, int PASSES>
+__global__ __launch_bounds__(256) void gemm_bf16x_kernel(LpP p) {
+  constexpr int BM = LP_BM, BN = LP_BN, BK = 32, LDK = BK + 8;
+  __shared__ unsigned short As[2][NP][BM * LDK];
+  __shared__ unsigned short Bs[2][NP][BN * LDK];
+  int tm, tn;
+  if (!lp_map_tile(blockIdx.x, gridDim.x, p.tiles, p.tiles_m, p.tiles_n, tm, tn)) return;
+  const int bm0 = tm * BM, bn0 = tn * BN;
+  if (p.lower && bm0 + BM - 1 < bn0) return;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const int g = lane >> 5, li = lane & 31;
+
+  f32x16 acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.f;
+
+  // global -> register staging: each thread moves two 16-byte row chunks
+  // per plane per operand
+  uint4 ra[NP][2], rb[NP][2];
+  const int srow = t >> 2, sch = (t & 3) * 8;
+  const int nk = p.Kp / BK;
+  const unsigned short *Ag = p.A + (long)(bm0 + srow) * p.Kp + sch;
+  const unsigned short *Bg = p.B + (long)(bn0 + srow) * p.Kp + sch;
+  const long rstride = 64L * p.Kp;
+#pragma unroll
+  for (int pl = 0; pl < NP; ++pl) {
+    ra[pl][0] = *reinterpret_cast<const uint4 *>(Ag + pl * p.planeA);
+    ra[pl][1] = *reinterpret_cast<const uint4 *>(Ag + pl * p.planeA + rstride);
+    rb[pl][0] = *reinterpret_cast<const uint4 *>(Bg + pl * p.planeB);
+    rb[pl][1] = *reinterpret_cast<const uint4 *>(Bg + pl * p.planeB + rstride);
+  }
+#pragma unroll
+  for (int pl = 0; pl < NP; ++pl) {
+    *reinterpret_cast<uint4 *>(&As[0][pl][srow * LDK + sch]) = ra[pl][0];
+    *reinterpret_cast<uint4 *>(&As[0][pl][(srow + 64) * LDK + sch]) = ra[pl][1];
+    *reinterpret_cast<uint4 *>(&Bs[0][pl][srow * LDK + sch]) = rb[pl][0];
+    *reinterpret_cast<uint4 *>(&Bs[0][pl][(srow + 64) * LDK + sch]) = rb[pl][1];
+  }
+  __syncthreads();
+  for (int kt = 0; kt < nk; ++kt) {
+    const int cur = kt & 1;
+    const bool more = kt + 1 < nk;
+    const int k1 = more ? (kt + 1) * BK : 0;
+    // prefetch of the next slab (the last iteration re-reads slab 0, unused)
+#pragma unroll
+    for (int pl = 0; pl < NP; ++pl) {
+      ra[pl][0] = *reinterpret_cast<const uint4 *>(Ag + pl * p.planeA + k1);
+      ra[pl][1] = *reinterpret_cast<const uint4 *>(Ag + pl * p.planeA + rstride + k1);
+      rb[pl][0] = *reinterpret_cast<const uint4 *>(Bg + pl * p.planeB + k1);
+      rb[pl][1] = *reinterpret_cast<const uint4 *>(Bg + pl * p.planeB + rstride + k1);
+    }
+#pragma unroll
+    for (int ks = 0; ks < BK / 16; ++ks) {
+      bf16x8 fa[NP][2], fb[NP][2];
+#pragma unroll
+      for (int pl = 0; pl < NP; ++pl)
+#pragma unroll
+        for (int x = 0; x < 2; ++x) {
+          fa[pl][x] = *reinterpret_cast<const bf16x8 *>(&As[cur][pl][(wm * 64 + x * 32 + li) * LDK + ks * 16 + g * 8]);
+          fb[pl][x] = *reinterpret_cast<const bf16x8 *>(&Bs[cur][pl][(wn * 64 + x * 32 + li) * LDK + ks * 16 + g * 8]);
+        }
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+          // small terms first, leading term last
+          if constexpr (PASSES >= 6) {
+            acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[0][a], fb[2 % NP][b], acc[a][b], 0, 0, 0);
+            acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[1 % NP][a], fb[1 % NP][b], acc[a][b], 0, 0, 0);
+            acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[2 % NP][a], fb[0][b], acc[a][b], 0, 0, 0);
+          }
+          if constexpr (PASSES >= 3) {
+            acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[0][a], fb[1 % NP][b], acc[a][b], 0, 0, 0);
+            acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[1 % NP][a], fb[0][b], acc[a][b], 0, 0, 0);
+          }
+          acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[0][a], fb[0][b], acc[a][b], 0, 0, 0);
+        }
+    }
+    if (more) {
+#pragma unroll
+      for (int pl = 0; pl < NP; ++pl) {
+        *reinterpret_cast<uint4 *>(&As[cur ^ 1][pl][srow * LDK + sch]) = ra[pl][0];
+        *reinterpret_cast<uint4 *>(&As[cur ^ 1][pl][(srow + 64) * LDK + sch]) = ra[pl][1];
+        *reinterpret_cast<uint4 *>(&Bs[cur ^ 1][pl][srow * LDK + sch]) = rb[pl][0];
+        *reinterpret_cast<uint4 *>(&Bs[cur ^ 1][pl][(srow + 64) * LDK + sch]) = rb[pl][1];
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      const int j = bn0 + wn * 64 + b * 32 + li;
+      if (j >= p.N) continue;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int i = bm0 + wm * 64 + a * 32 + 8 * (r >> 2) + 4 * g + (r & 3);
+        if (i >= p.M || (p.lower && i < j)) continue;
+        float *c = p.C + (long)i + (long)j * p.ldc;
+        *c += p.alpha * acc[a][b][r];
+      }
+    }
+}
+
+// FP8: A, B packed e4m3 [mn][Kp]; scales per (128-tile, 128-K block).
+struct F8P {
+  const unsigned char *A, *B;
+  const float *sA, *sB;  // [tiles][ktiles]
+  float *C;
+  long ldc;
+  int M, N, Kp, ktiles;
+  float alpha;
+  int tiles_m, tiles_n, tiles;
+  int lower;
+};
+
+__global__ __launch_bounds__(256) void gemm_fp8_kernel(F8P p) {
+  constexpr int BM = 128, BN = 128, BK = 128, LDK = BK + 16;
+  __shared__ unsigned char As[BM * LDK];
+  __shared__ unsigned char Bs[BN * LDK];
+  int tm, tn;
+  if (!lp_map_tile(blockIdx.x, gridDim.x, p.tiles, p.tiles_m, p.tiles_n, tm, tn)) return;
+  const int bm0 = tm * BM, bn0 = tn * BN;
+  if (p.lower && bm0 + BM - 1 < bn0) return;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const int g = lane >> 5, li = lane & 31;
+  f32x16 acc[2][2], part[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.f;
+  const int nk = p.Kp / BK;
+  for (int kt = 0; kt < nk; ++kt) {
+    // 128 rows x 128 bytes per operand: 1024 16-byte chunks, 4 per thread
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int idx = t + 256 * r, row = idx >> 3, ch = (idx & 7) * 16;
+      *reinterpret_cast<uint4 *>(&As[row * LDK + ch]) =
+          *reinterpret_cast<const uint4 *>(p.A + (long)(bm0 + row) * p.Kp + kt * BK + ch);
+      *reinterpret_cast<uint4 *>(&Bs[row * LDK + ch]) =
+          *reinterpret_cast<const uint4 *>(p.B + (long)(bn0 + row) * p.Kp + kt * BK + ch);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) part[a][b][r] = 0.f;
+#pragma unroll
+    for (int ks = 0; ks < BK / 64; ++ks) {
+      i32x8 fa[2], fb[2];
+#pragma unroll
+      for (int x = 0; x < 2; ++x) {
+        fa[x] = *reinterpret_cast<const i32x8 *>(&As[(wm * 64 + x * 32 + li) * LDK + ks * 64 + g * 32]);
+        fb[x] = *reinterpret_cast<const i32x8 *>(&Bs[(wn * 64 + x * 32 + li) * LDK + ks * 64 + g * 32]);
+      }
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+          part[a][b] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(fa[a], fb[b], part[a][b], 0, 0,
+                                                                       0, 127, 0, 127);
+    }
+    const float sc = p.sA[tm * p.ktiles + kt] * p.sB[tn * p.ktiles + kt];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[a][b][r] = fmaf(part[a][b][r], sc, acc[a][b][r]);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      const int j = bn0 + wn * 64 + b * 32 + li;
+      if (j >= p.N) continue;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int i = bm0 + wm * 64 + a * 32 + 8 * (r >> 2) + 4 * g + (r & 3);
+        if (i >= p.M || (p.lower && i < j)) continue;
+        float *c = p.C + (long)i + (long)j * p.ldc;
+        *c += p.alpha * acc[a][b][r];
+      }
+    }
+}
+
+static long roundup(long x, long m) { return (x + m - 1) / m * m; }
+
+// Normalizes the problem so that C is column-major; returns false if C is
+// row-major and the caller must transpose the problem.
+void gemm_bf16x(hipStream_t s, long M, long N, long K, float alpha, View<const float> A,
+                View<const float> B, View<float> C, int passes) {
+  if (M <= 0 || N <= 0 || K <= 0) return;
+  if (C.rs != 1) {
+    MG_CHECK(C.cs == 1, "gemm_bf16x: unsupported C layout");
+    gemm_bf16x(s, N, M, K, alpha, B.t(), A.t(), View<float>{C.p, 1, C.rs}, passes);
+    return;
+  }
+  MG_CHECK(passes == 1 || passes == 3 || passes == 6, "gemm_bf16x: passes must be 1, 3 or 6");
+  const int np = passes == 1 ? 1 : (passes == 3 ? 2 : 3);
+  const long Mp = roundup(M, LP_BM), Np = roundup(N, LP_BN), Kp = roundup(K, 32);
+  Workspace &ws = workspace(s);
+  unsigned short *Ap = (unsigned short *)ws.get("lp.A", (size_t)np * Mp * Kp * 2);
+  unsigned short *Bp = (unsigned short *)ws.get("lp.B", (size_t)np * Np * Kp * 2);
+  // A(i,k) = A.p[i*rs + k*cs] -> [mn=i][k];  B(k,j) = B.p[k*rs + j*cs] -> [mn=j][k]
+  hipLaunchKernelGGL(pack_bf16_kernel, dim3(Kp / 32, Mp / 32), dim3(256), 0, s, A.p, A.rs, A.cs,
+                     (int)M, (int)K, (int)Kp, np, Mp * Kp, Ap);
+  hipLaunchKernelGGL(pack_bf16_kernel, dim3(Kp / 32, Np / 32), dim3(256), 0, s, B.p, B.cs, B.rs,
+                     (int)N, (int)K, (int)Kp, np, Np * Kp, Bp);
+  MG_HIP(hipGetLastError());
+  LpP p{};
+  p.A = Ap;
+  p.B = Bp;
+  p.planeA = Mp * Kp;
+  p.planeB = Np * Kp;
+  p.C = C.p;
+  p.ldc = C.cs;
+  p.M = (int)M;
+  p.N = (int)N;
+  p.Kp = (int)Kp;
+  p.alpha = alpha;
+  p.tiles_m = (int)(Mp / LP_BM);
+  p.tiles_n = (int)(Np / LP_BN);
+  p.tiles = p.tiles_m * p.tiles_n;
+  p.lower = 0;
+  int grid = p.tiles;
+  if (grid >= 8) grid = (grid + 7) / 8 * 8;
+  if (passes == 1) hipLaunchKernelGGL((gemm_bf16x_kernel<1, 1>), dim3(grid), dim3(256), 0, s, p);
+  else if (passes == 3) hipLaunchKernelGGL((gemm_bf16x_kernel<2, 3>), dim3(grid), dim3(256), 0, s, p);
+  else hipLaunchKernelGGL((gemm_bf16x_kernel<3, 6>), dim3(grid), dim3(256), 0, s, p);
+  MG_HIP(hipGetLastError());
+}
+
+void gemm_fp8(hipStream_t s, long M, long N, long K, float alpha, View<const float> A,
+              View<const float> B, View<float> C, bool lower_only) {
+  if (M <= 0 || N <= 0 || K <= 0) return;
+  if (C.rs != 1) {
+    MG_CHECK(C.cs == 1 && !lower_only, "gemm_fp8: unsupported C layout");
+    gemm_fp8(s, N, M, K, alpha, B.t(), A.t(), View<float>{C.p, 1, C.rs}, false);
+    return;
+  }
+  const long Mp = roundup(M, 128), Np = roundup(N, 128), Kp = roundup(K, 128);
+  const int ktiles = (int)(Kp / 128);
+  Workspace &ws = workspace(s);
+  unsigned char *Ap = (unsigned char *)ws.get("f8.A", (size_t)Mp * Kp);
+  unsigned char *Bp = (unsigned char *)ws.get("f8.B", (size_t)Np * Kp);
+  float *sA = (float *)ws.get("f8.sA", (size_t)(Mp / 128) * ktiles * 4);
+  float *sB = (float *)ws.get("f8.sB", (size_t)(Np / 128) * ktiles * 4);
+  hipLaunchKernelGGL(fp8_scale_kernel, dim3(ktiles, Mp / 128), dim3(256), 0, s, A.p, A.rs, A.cs,
+                     (int)M, (int)K, ktiles, sA);
+  hipLaunchKernelGGL(fp8_scale_kernel, dim3(ktiles, Np / 128), dim3(256), 0, s, B.p, B.cs, B.rs,
+                     (int)N, (int)K, ktiles, sB);
+  hipLaunchKernelGGL(pack_fp8_kernel, dim3(Kp / 32, Mp / 32), dim3(256), 0, s, A.p, A.rs, A.cs,
+                     (int)M, (int)K, (int)Kp, ktiles, sA, Ap);
+  hipLaunchKernelGGL(pack_fp8_kernel, dim3(Kp / 32, Np / 32), dim3(256), 0, s, B.p, B.cs, B.rs,
+                     (int)N, (int)K, (int)Kp, ktiles, sB, Bp);
+  MG_HIP(hipGetLastError());
+  F8P p{};
+  p.A = Ap;
+  p.B = Bp;
+  p.sA = sA;
+  p.sB = sB;
+  p.C = C.p;
+  p.ldc = C.cs;
+  p.M = (int)M;
+  p.N = (int)N;
+  p.Kp = (int)Kp;
+  p.ktiles = ktiles;
+  p.alpha = alpha;
+  p.tiles_m = (int)(Mp / 128);
+  p.tiles_n = (int)(Np / 128);
+  p.tiles = p.tiles_m * p.tiles_n;
+  p.lower = lower_only ? 1 : 0;
+  int grid = p.tiles;
+  if (grid >= 8) grid = (grid + 7) / 8 * 8;
+  hipLaunchKernelGGL(gemm_fp8_kernel, dim3(grid), dim3(256), 0, s, p);
+  MG_HIP(hipGetLastError());
+}
+
+}  // namespace mg

@@ -1,0 +1,80 @@
+// Internal entry points shared by the HIP sources and the C-ABI layer.
+#pragma once
+#include "common.hpp"
+
+namespace mg {
+
+// ----------------------------------------------------------------- LU ----
+// Row-major working layout: element (i,j) of the n x n matrix at W[i*ld+j].
+// Partial pivoting with libmat's rules: the pivot of column k is the first
+// row (lowest index) holding max|a(i,k)| for i >= k (mat.h:1247-1262); the
+// last column (k = n-1) is never pivoted (mat.h:4214); a pivot with
+// |a(k,k)| < eps is swapped in but neither scales its column nor updates the
+// rest of its leaf block (mat.h:4234-4235).  ipiv[k] (device, int32) gets the
+// global row swapped with row k.  `leaf` is the width of the column blocks
+// factored by the pivoting kernel; leaf = 24 reproduces the reference's
+// block boundaries for the eps-skip rule (MAT_PLU_BLOCK_SIZE, mat.h:4196).
+struct LuOpts {
+  int leaf = 32;
+  Mul mul = Mul::Native;  // precision of the trailing updates (float only)
+};
+template <typename T>
+void getrf_rowmajor(hipStream_t s, int n, T *W, long ld, int *ipiv, T eps, const LuOpts &o);
+// Same, also maintaining perm (device, length n): perm[i] = original row now
+// at row i (libmat's Perm, mat.h:382-385).
+template <typename T>
+void getrf_rowmajor_perm(hipStream_t s, int n, T *W, long ld, int *ipiv, int *perm, T eps,
+                         const LuOpts &o);
+
+// Number of k < n-1 with ipiv[k] != k, and libmat's Perm (p[i] = original
+// row now at row i) from the sequential swaps (host arrays).
+int ipiv_to_perm(int n, const int *ipiv, size_t *perm);
+
+// Full-pivoting LU PAQ = LU on a column-major n x n matrix in place
+// (mat_lu, x86 branch mat.h:4502-4557).  rec[3k..3k+2] = {pivot row,
+// pivot col, skipped}; device arrays.
+template <typename T>
+void getrf_full(hipStream_t s, int n, T *A, long lda, int *rec, T eps);
+
+// -------------------------------------------------------------- Cholesky --
+// In-place lower Cholesky of a column-major matrix (only the lower triangle
+// is read or written).  info (device int) gets j+1 for the first column j
+// whose pivot is <= 0, else stays 0.
+template <typename T>
+void potrf_lower(hipStream_t s, int n, T *A, long lda, int *info, Mul mul = Mul::Native);
+
+// ------------------------------------------------------------ triangular --
+// B := op(T)^-1 B for a left-side triangular T (n x n, strided view) and
+// B (n x nrhs, strided view).  lower/upper refers to op(T) as addressed by
+// the view; unit ignores the diagonal.
+template <typename T>
+void trsm_left(hipStream_t s, bool lower, bool unit, int n, int nrhs, View<const T> Tm,
+               View<T> B, Mul mul = Mul::Native);
+
+// ------------------------------------------------------------ utilities --
+template <typename T>
+void transpose(hipStream_t s, int rows, int cols, const T *src, long lds, T *dst, long ldd);
+// Packed row-major LU -> column-major unit-lower L and upper U (full n x n).
+template <typename T>
+void split_lu_rowmajor(hipStream_t s, int n, const T *W, long ld, T *L, T *U);
+// Packed column-major LU -> column-major L and U.
+template <typename T>
+void split_lu_colmajor(hipStream_t s, int n, const T *A, long ld, T *L, T *U);
+// Copy the lower triangle of column-major A into L and zero L's upper part.
+template <typename T>
+void copy_lower(hipStream_t s, int n, const T *A, long lda, T *L, long ldl);
+// Zero the strict upper triangle of a column-major matrix.
+template <typename T>
+void zero_upper(hipStream_t s, int n, T *A, long lda);
+// dst[i + c*ldd] = src[perm[i] + c*lds] (gather rows through a device perm).
+template <typename T>
+void gather_rows(hipStream_t s, int n, int nrhs, const T *src, long lds, const int *perm,
+                 T *dst, long ldd);
+template <typename Ti, typename To>
+void convert(hipStream_t s, long count, const Ti *src, To *dst);
+template <typename Ti, typename To>
+void convert2d(hipStream_t s, int rows, int cols, const Ti *src, long lds, To *dst, long ldd);
+void fill_int(hipStream_t s, int *p, long count, int value);
+void fill_u32(hipStream_t s, unsigned *p, long count, unsigned value);
+
+}  // namespace mg

@@ -1,0 +1,423 @@
+// Solves on top of the factorizations, and the mixed-precision solvers.
+//
+// getrs: mat_solve's second half (mat.h:4825-4830) for many right-hand
+// sides at once: pb = b[p], unit-lower then upper substitution, here as
+// blocked triangular solves with MFMA GEMM updates.
+// potrs: mat_solve_spd's second half (mat.h:4862-4865): L y = b, L^T x = y.
+//
+// Mixed precision (extension, no counterpart in the reference):
+//   gesv_mixed: fp32 working LU whose trailing updates run on BF16 MFMA
+//   (split into 1, 3 or 6 bf16 products, fp32 accumulate), then iterative
+//   refinement in which the residual r = b - A x (A in fp64) and the update
+//   x += d are the only fp64 operations; d solves the fp32 factors.
+//   posv_mixed: same with an fp32 Cholesky whose trailing updates run on
+//   FP8 e4m3 MFMA with per-128x128-tile scales.
+//   Both stop at backward error max_c ||r_c|| / (||A|| ||x_c|| + ||b_c||)
+//   <= tol (infinity norms) and fall back to the plain fp64 factorization
+//   when refinement stalls (two steps without halving the error, growth,
+//   NaN, or the iteration cap).
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "mat_gpu.h"
+#include "ops.hpp"
+
+namespace mg {
+
+template <typename T>
+void getrs_rowmajor(hipStream_t s, int n, int nrhs, const T *W, long ld, const int *perm,
+                    const T *B, long ldb, T *X, long ldx) {
+  gather_rows<T>(s, n, nrhs, B, ldb, perm, X, ldx);
+  View<const T> LU{W, ld, 1};
+  trsm_left<T>(s, true, true, n, nrhs, LU, View<T>{X, 1, ldx});
+  trsm_left<T>(s, false, false, n, nrhs, LU, View<T>{X, 1, ldx});
+}
+
+template <typename T>
+void potrs_lower(hipStream_t s, int n, int nrhs, const T *L, long ld, T *X, long ldx) {
+  View<const T> Lv{L, 1, ld};
+  trsm_left<T>(s, true, false, n, nrhs, Lv, View<T>{X, 1, ldx});
+  trsm_left<T>(s, false, false, n, nrhs, Lv.t(), View<T>{X, 1, ldx});
+}
+
+template void getrs_rowmajor<float>(hipStream_t, int, int, const float *, long, const int *,
+                                    const float *, long, float *, long);
+template void getrs_rowmajor<double>(hipStream_t, int, int, const double *, long, const int *,
+                                     const double *, long, double *, long);
+template void potrs_lower<float>(hipStream_t, int, int, const float *, long, float *, long);
+template void potrs_lower<double>(hipStream_t, int, int, const double *, long, double *, long);
+
+// ------------------------------------------------------ fp64 residual ----
+// R = B - A X, A n x n column-major fp64, up to 16 right-hand sides per
+// pass.  Split-K: workgroup (rb, kc) owns 256 rows and one K chunk; each
+// thread keeps one row's 16 partial sums; X(k, :) is wave-uniform (scalar
+// loads).  Partials are reduced in a fixed order (deterministic).
+constexpr int RES_NR = 16;
+constexpr int RES_KC = 512;
+
+__global__ __launch_bounds__(256) void residual_partial_kernel(int n, int nrhs, const double *A,
+                                                               long lda, const double *X, long ldx,
+                                                               double *part) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const int k0 = blockIdx.y * RES_KC;
+  const int k1 = min(n, k0 + RES_KC);
+  double acc[RES_NR];
+#pragma unroll
+  for (int c = 0; c < RES_NR; ++c) acc[c] = 0.0;
+  if (i < n) {
+    const double *a = A + i;
+    for (int k = k0; k < k1; ++k) {
+      const double v = a[(long)k * lda];
+#pragma unroll
+      for (int c = 0; c < RES_NR; ++c)
+        if (c < nrhs) acc[c] = fma(v, X[k + (long)c * ldx], acc[c]);
+    }
+    double *pp = part + ((long)blockIdx.y * RES_NR) * n + i;
+#pragma unroll
+    for (int c = 0; c < RES_NR; ++c)
+      if (c < nrhs) pp[(long)c * n] = acc[c];
+  }
+}
+
+__global__ void residual_reduce_kernel(int n, int nrhs, int nchunks, const double *part,
+                                       const double *B, long ldb, double *R, long ldr) {
+  const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= (long)n * nrhs) return;
+  const int i = (int)(idx % n), c = (int)(idx / n);
+  double s = 0.0;
+  for (int q = 0; q < nchunks; ++q) s += part[((long)q * RES_NR + c) * n + i];
+  R[i + (long)c * ldr] = B[i + (long)c * ldb] - s;
+}
+
+void residual_f64(hipStream_t s, int n, int nrhs, const double *A, long lda, const double *X,
+                  long ldx, const double *B, long ldb, double *R, long ldr) {
+  const int nchunks = ceil_div(n, RES_KC);
+  double *part = (double *)workspace(s).get("res.part", (size_t)nchunks * RES_NR * n * 8);
+  for (int c0 = 0; c0 < nrhs; c0 += RES_NR) {
+    const int nr = std::min(RES_NR, nrhs - c0);
+    hipLaunchKernelGGL(residual_partial_kernel, dim3(ceil_div(n, 256), nchunks), dim3(256), 0, s, n,
+                       nr, A, lda, X + (long)c0 * ldx, ldx, part);
+    hipLaunchKernelGGL(residual_reduce_kernel, dim3(ceil_div((size_t)n * nr, 256)), dim3(256), 0, s,
+                       n, nr, nchunks, part, B + (long)c0 * ldb, ldb, R + (long)c0 * ldr, ldr);
+  }
+  MG_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------- norms -----
+// out[c] = max_i |X(i,c)| (non-negative doubles order like their bits).
+__global__ void col_amax_kernel(int n, const double *X, long ldx, unsigned long long *out) {
+  const int c = blockIdx.y;
+  double m = 0.0;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256)
+    m = fmax(m, fabs(X[i + (long)c * ldx]));
+  __shared__ double red[256];
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (threadIdx.x < off) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + off]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicMax(out + c, (unsigned long long)__double_as_longlong(red[0]));
+}
+
+// out = max_i sum_j |A(i,j)|
+__global__ void row_abs_sum_kernel(int n, const double *A, long lda, unsigned long long *out) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  double sum = 0.0;
+  if (i < n)
+    for (int j = 0; j < n; ++j) sum += fabs(A[i + (long)j * lda]);
+  __shared__ double red[256];
+  red[threadIdx.x] = sum;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (threadIdx.x < off) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + off]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicMax(out, (unsigned long long)__double_as_longlong(red[0]));
+}
+
+static void col_amax(hipStream_t s, int n, int nrhs, const double *X, long ldx,
+                     unsigned long long *out) {
+  hipLaunchKernelGGL(col_amax_kernel, dim3(std::min(ceil_div(n, 256), 64u), nrhs), dim3(256), 0, s,
+                     n, X, ldx, out);
+  MG_HIP(hipGetLastError());
+}
+
+double norm_inf_f64(hipStream_t s, int n, const double *A, long lda) {
+  unsigned long long *d = (unsigned long long *)workspace(s).get("norm.A", 8);
+  fill_u32(s, (unsigned *)d, 2, 0u);
+  hipLaunchKernelGGL(row_abs_sum_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, A, lda, d);
+  MG_HIP(hipGetLastError());
+  unsigned long long h = 0;
+  MG_HIP(hipMemcpyAsync(&h, d, 8, hipMemcpyDeviceToHost, s));
+  MG_HIP(hipStreamSynchronize(s));
+  double v;
+  memcpy(&v, &h, 8);
+  return v;
+}
+
+// x += (double) d   (column-major blocks)
+__global__ void axpy_f32_to_f64_kernel(int n, int nrhs, const float *D, long ldd, double *X,
+                                       long ldx) {
+  const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= (long)n * nrhs) return;
+  const int i = (int)(idx % n), c = (int)(idx / n);
+  X[i + (long)c * ldx] += (double)D[i + (long)c * ldd];
+}
+
+template <typename Ti, typename To>
+__global__ void transpose_convert_kernel(int rows, int cols, const Ti *src, long lds, To *dst,
+                                         long ldd) {
+  __shared__ Ti tile[32][33];
+  const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int yy = ty; yy < 32; yy += 8) {
+    const int i = bx + tx, j = by + yy;
+    if (i < rows && j < cols) tile[yy][tx] = src[i + (long)j * lds];
+  }
+  __syncthreads();
+  for (int yy = ty; yy < 32; yy += 8) {
+    const int j = by + tx, i = bx + yy;
+    if (i < rows && j < cols) dst[j + (long)i * ldd] = (To)tile[tx][yy];
+  }
+}
+
+template <typename Ti, typename To>
+static void transpose_convert(hipStream_t s, int rows, int cols, const Ti *src, long lds, To *dst,
+                              long ldd) {
+  hipLaunchKernelGGL((transpose_convert_kernel<Ti, To>), dim3(ceil_div(rows, 32), ceil_div(cols, 32)),
+                     dim3(256), 0, s, rows, cols, src, lds, dst, ldd);
+  MG_HIP(hipGetLastError());
+}
+
+// Backward error per column, max over columns, from device vectors.
+struct BerrEval {
+  hipStream_t s;
+  int n, nrhs;
+  double normA;
+  unsigned long long *dev;  // [3][nrhs]: |r|, |x|, |b|
+  std::vector<unsigned long long> host;
+  std::vector<double> bnorm;
+
+  BerrEval(hipStream_t s_, int n_, int nrhs_, double normA_, const double *B, long ldb)
+      : s(s_), n(n_), nrhs(nrhs_), normA(normA_) {
+    dev = (unsigned long long *)workspace(s).get("berr", (size_t)3 * nrhs * 8);
+    host.resize(3 * nrhs);
+    fill_u32(s, (unsigned *)dev, 6L * nrhs, 0u);
+    col_amax(s, n, nrhs, B, ldb, dev + 2 * nrhs);
+  }
+  double eval(const double *R, long ldr, const double *X, long ldx) {
+    fill_u32(s, (unsigned *)dev, 4L * nrhs, 0u);
+    col_amax(s, n, nrhs, R, ldr, dev);
+    col_amax(s, n, nrhs, X, ldx, dev + nrhs);
+    MG_HIP(hipMemcpyAsync(host.data(), dev, (size_t)3 * nrhs * 8, hipMemcpyDeviceToHost, s));
+    MG_HIP(hipStreamSynchronize(s));
+    double worst = 0.0;
+    for (int c = 0; c < nrhs; ++c) {
+      double r, x, b;
+      memcpy(&r, &host[c], 8);
+      memcpy(&x, &host[nrhs + c], 8);
+      memcpy(&b, &host[2 * nrhs + c], 8);
+      const double den = normA * x + b;
+      const double e = den > 0 ? r / den : (r > 0 ? INFINITY : 0.0);
+      if (!(e <= worst)) worst = e;  // NaN propagates
+    }
+    return worst;
+  }
+};
+
+static double ms_since(std::chrono::steady_clock::time_point t0) {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// Plain fp64 partial-pivot LU solve (device pointers, column-major).
+int gesv_f64(hipStream_t s, int n, int nrhs, const double *A, long lda, const double *B, long ldb,
+             double *X, long ldx, int leaf) {
+  Workspace &ws = workspace(s);
+  double *W = (double *)ws.get("gesv64.W", (size_t)n * n * 8);
+  int *ipiv = (int *)ws.get("gesv64.ipiv", (size_t)n * 4);
+  int *perm = (int *)ws.get("gesv64.perm", (size_t)n * 4);
+  transpose<double>(s, n, n, A, lda, W, n);
+  LuOpts o;
+  o.leaf = leaf;
+  getrf_rowmajor_perm<double>(s, n, W, n, ipiv, perm, 1e-9, o);
+  getrs_rowmajor<double>(s, n, nrhs, W, n, perm, B, ldb, X, ldx);
+  return 0;
+}
+
+int gesv_mixed(hipStream_t s, int n, int nrhs, const double *A, long lda, const double *B, long ldb,
+               double *X, long ldx, const mat_gpu_ir_opts *opt, mat_gpu_ir_status *st) {
+  const auto t0 = std::chrono::steady_clock::now();
+  mat_gpu_ir_opts o{};
+  o.passes = 3;
+  o.max_iter = 30;
+  o.tol = 1e-14;
+  o.leaf = 32;
+  if (opt) o = *opt;
+  MG_CHECK(o.passes == 0 || o.passes == 1 || o.passes == 3 || o.passes == 6,
+           "gesv_mixed: passes must be 0 (fp32 MFMA), 1, 3 or 6");
+  Workspace &ws = workspace(s);
+  float *W = (float *)ws.get("gesvm.W", (size_t)n * n * 4);
+  int *ipiv = (int *)ws.get("gesvm.ipiv", (size_t)n * 4);
+  int *perm = (int *)ws.get("gesvm.perm", (size_t)n * 4);
+  double *R = (double *)ws.get("gesvm.R", (size_t)n * nrhs * 8);
+  float *D = (float *)ws.get("gesvm.D", (size_t)n * nrhs * 4);
+  float *D2 = (float *)ws.get("gesvm.D2", (size_t)n * nrhs * 4);
+
+  const double normA = norm_inf_f64(s, n, A, lda);
+  // fp32 row-major working copy, then the low-precision LU
+  transpose_convert<double, float>(s, n, n, A, lda, W, n);
+  LuOpts lo;
+  lo.leaf = o.leaf;
+  lo.mul = o.passes == 0 ? Mul::Native : (Mul)o.passes;
+  getrf_rowmajor_perm<float>(s, n, W, n, ipiv, perm, 1e-6f, lo);
+  MG_HIP(hipStreamSynchronize(s));
+  const double t_factor = ms_since(t0);
+
+  BerrEval be(s, n, nrhs, normA, B, ldb);
+  // x0 = LU^-1 P b (fp32), r = b - A x
+  convert2d<double, float>(s, n, nrhs, B, ldb, D, n);
+  getrs_rowmajor<float>(s, n, nrhs, W, n, perm, D, n, D2, n);
+  convert2d<float, double>(s, n, nrhs, D2, n, X, ldx);
+  int it = 0;
+  double berr = INFINITY, prev = INFINITY, prev2 = INFINITY;
+  bool ok = false;
+  while (true) {
+    residual_f64(s, n, nrhs, A, lda, X, ldx, B, ldb, R, n);
+    berr = be.eval(R, n, X, ldx);
+    if (berr <= o.tol) {
+      ok = true;
+      break;
+    }
+    const bool stalled = !(berr == berr) || it >= o.max_iter ||
+                         (it >= 2 && berr > 0.5 * prev2) || (it >= 1 && berr > 2.0 * prev);
+    if (stalled) break;
+    prev2 = prev;
+    prev = berr;
+    // d = LU^-1 P r (fp32); x += d (fp64)
+    convert2d<double, float>(s, n, nrhs, R, n, D, n);
+    getrs_rowmajor<float>(s, n, nrhs, W, n, perm, D, n, D2, n);
+    hipLaunchKernelGGL(axpy_f32_to_f64_kernel, dim3(ceil_div((size_t)n * nrhs, 256)), dim3(256), 0,
+                       s, n, nrhs, D2, n, X, ldx);
+    MG_HIP(hipGetLastError());
+    ++it;
+  }
+  if (st) {
+    st->iterations = it;
+    st->fell_back = 0;
+    st->backward_error = berr;
+    st->factor_ms = t_factor;
+    st->precision = o.passes;
+  }
+  if (!ok) {
+    // refinement stalled: plain fp64 factorization
+    gesv_f64(s, n, nrhs, A, lda, B, ldb, X, ldx, o.leaf);
+    residual_f64(s, n, nrhs, A, lda, X, ldx, B, ldb, R, n);
+    berr = be.eval(R, n, X, ldx);
+    if (st) {
+      st->fell_back = 1;
+      st->backward_error = berr;
+    }
+  }
+  if (st) st->total_ms = ms_since(t0);
+  MG_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+// Plain fp64 Cholesky solve (device pointers); returns -1 if not SPD.
+int posv_f64(hipStream_t s, int n, int nrhs, const double *A, long lda, const double *B, long ldb,
+             double *X, long ldx) {
+  Workspace &ws = workspace(s);
+  double *L = (double *)ws.get("posv64.L", (size_t)n * n * 8);
+  int *info = (int *)ws.get("posv64.info", 4);
+  copy_lower<double>(s, n, A, lda, L, n);
+  potrf_lower<double>(s, n, L, n, info);
+  int h = 0;
+  MG_HIP(hipMemcpyAsync(&h, info, 4, hipMemcpyDeviceToHost, s));
+  MG_HIP(hipStreamSynchronize(s));
+  if (h != 0) return -1;
+  convert2d<double, double>(s, n, nrhs, B, ldb, X, ldx);
+  potrs_lower<double>(s, n, nrhs, L, n, X, ldx);
+  return 0;
+}
+
+int posv_mixed(hipStream_t s, int n, int nrhs, const double *A, long lda, const double *B, long ldb,
+               double *X, long ldx, const mat_gpu_ir_opts *opt, mat_gpu_ir_status *st) {
+  const auto t0 = std::chrono::steady_clock::now();
+  mat_gpu_ir_opts o{};
+  o.passes = 8;
+  o.max_iter = 30;
+  o.tol = 1e-14;
+  o.leaf = 32;
+  if (opt) o = *opt;
+  MG_CHECK(o.passes == 0 || o.passes == 1 || o.passes == 3 || o.passes == 6 || o.passes == 8,
+           "posv_mixed: passes must be 0 (fp32), 1/3/6 (bf16) or 8 (fp8)");
+  Workspace &ws = workspace(s);
+  float *L = (float *)ws.get("posvm.L", (size_t)n * n * 4);
+  int *info = (int *)ws.get("posvm.info", 4);
+  double *R = (double *)ws.get("posvm.R", (size_t)n * nrhs * 8);
+  float *D = (float *)ws.get("posvm.D", (size_t)n * nrhs * 4);
+
+  const double normA = norm_inf_f64(s, n, A, lda);
+  convert2d<double, float>(s, n, n, A, lda, L, n);
+  potrf_lower<float>(s, n, L, n, info, o.passes == 0 ? Mul::Native : (Mul)o.passes);
+  int h = 0;
+  MG_HIP(hipMemcpyAsync(&h, info, 4, hipMemcpyDeviceToHost, s));
+  MG_HIP(hipStreamSynchronize(s));
+  const double t_factor = ms_since(t0);
+  int it = 0;
+  double berr = INFINITY;
+  bool ok = false;
+  BerrEval be(s, n, nrhs, normA, B, ldb);
+  if (h == 0) {
+    convert2d<double, float>(s, n, nrhs, B, ldb, D, n);
+    potrs_lower<float>(s, n, nrhs, L, n, D, n);
+    convert2d<float, double>(s, n, nrhs, D, n, X, ldx);
+    double prev = INFINITY, prev2 = INFINITY;
+    while (true) {
+      residual_f64(s, n, nrhs, A, lda, X, ldx, B, ldb, R, n);
+      berr = be.eval(R, n, X, ldx);
+      if (berr <= o.tol) {
+        ok = true;
+        break;
+      }
+      const bool stalled = !(berr == berr) || it >= o.max_iter ||
+                           (it >= 2 && berr > 0.5 * prev2) || (it >= 1 && berr > 2.0 * prev);
+      if (stalled) break;
+      prev2 = prev;
+      prev = berr;
+      convert2d<double, float>(s, n, nrhs, R, n, D, n);
+      potrs_lower<float>(s, n, nrhs, L, n, D, n);
+      hipLaunchKernelGGL(axpy_f32_to_f64_kernel, dim3(ceil_div((size_t)n * nrhs, 256)), dim3(256), 0,
+                         s, n, nrhs, D, n, X, ldx);
+      MG_HIP(hipGetLastError());
+      ++it;
+    }
+  }
+  if (st) {
+    st->iterations = it;
+    st->fell_back = 0;
+    st->backward_error = berr;
+    st->factor_ms = t_factor;
+    st->precision = o.passes;
+  }
+  int rc = 0;
+  if (!ok) {
+    rc = posv_f64(s, n, nrhs, A, lda, B, ldb, X, ldx);
+    if (rc == 0) {
+      residual_f64(s, n, nrhs, A, lda, X, ldx, B, ldb, R, n);
+      berr = be.eval(R, n, X, ldx);
+    }
+    if (st) {
+      st->fell_back = 1;
+      st->backward_error = berr;
+    }
+  }
+  if (st) st->total_ms = ms_since(t0);
+  MG_HIP(hipStreamSynchronize(s));
+  return rc;
+}
+
+}  // namespace mg

@@ -1,0 +1,381 @@
+/*
+ * CPU oracle for the dense GEMM + factor-and-solve path of libmat.
+ *
+ * TEST INFRASTRUCTURE ONLY. This file is a clean-room restatement of the
+ * x86 scalar code path of libmat (the path MAT_DISPATCH selects on x86 when
+ * neither NEON nor AVX2 is enabled).  It exists so that the tests, the
+ * smoke check and the CPU-baseline leg of the benchmark can check and time
+ * the GPU library against the reference algorithm.  Nothing in the shipped
+ * library links or calls it.
+ *
+ * Every routine repeats the reference's loop order and operation order so
+ * that, built with -O2 -ffp-contract=off and no -march, it reproduces the
+ * reference's outputs bit for bit.  That claim is pinned by
+ * tests/test_oracle_vs_ref.py (against oracle/_ref, the reference compiled
+ * from its own sources) and tests/test_oracle_golden.py (against committed
+ * fixtures and the reference's own known-answer tests).
+ *
+ * Compiled twice: ORACLE_F64 undefined -> float flavour (mat_elem_t=float,
+ * eps 1e-6f); ORACLE_F64 defined -> double flavour (eps 1e-9), mirroring
+ * mat.h:165-181.
+ *
+ * Storage is column-major with leading dimension = rows (mat.h:387-393).
+ */
+#include <math.h>
+#include <stddef.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifdef ORACLE_F64
+typedef double elem;
+#define ORC_EPS 1e-9
+#define ORC_ABS fabs
+#define ORC_SQRT sqrt
+#define ORC(name) orc_d_##name
+#else
+typedef float elem;
+#define ORC_EPS 1e-6f
+#define ORC_ABS fabsf
+#define ORC_SQRT sqrtf
+#define ORC(name) orc_s_##name
+#endif
+
+#define AT(p, ld, i, j) ((p)[(size_t)(j) * (ld) + (i)])
+
+/* y *= alpha with the reference's special cases: alpha == 1 returns,
+ * alpha == 0 zero-fills (mat.h:959-964, 993-1000). */
+static void scal(elem *y, elem alpha, size_t n) {
+  if (alpha == 1) return;
+  if (alpha == 0) {
+    memset(y, 0, n * sizeof(elem));
+    return;
+  }
+  for (size_t i = 0; i < n; i++) y[i] *= alpha;
+}
+
+/* y += alpha * x (mat.h:908-914). */
+static void axpy(elem *y, elem alpha, const elem *x, size_t n) {
+  for (size_t i = 0; i < n; i++) y[i] += alpha * x[i];
+}
+
+/* Index of the first element whose magnitude equals max|x|
+ * (mat.h:1195-1204 amax, 1245-1262 iamax). */
+static size_t iamax(const elem *x, size_t n) {
+  if (n == 0) return 0;
+  elem m = ORC_ABS(x[0]);
+  for (size_t i = 1; i < n; i++) {
+    elem v = ORC_ABS(x[i]);
+    if (v > m) m = v;
+  }
+  for (size_t i = 0; i < n; i++)
+    if (ORC_ABS(x[i]) == m) return i;
+  return 0;
+}
+
+/* C = alpha*A*B + beta*C, A MxK, B KxN (mat_gemm -> mat__gemm_scalar,
+ * mat.h:2762-2796): beta-scale, then i-k-j loops with alpha folded into
+ * A(i,k). */
+void ORC(gemm)(size_t M, size_t K, size_t N, elem alpha, const elem *A,
+               const elem *B, elem beta, elem *C) {
+  scal(C, beta, M * N);
+  for (size_t i = 0; i < M; i++) {
+    for (size_t k = 0; k < K; k++) {
+      elem aik = alpha * AT(A, M, i, k);
+      for (size_t j = 0; j < N; j++)
+        AT(C, M, i, j) = AT(C, M, i, j) + aik * AT(B, K, k, j);
+    }
+  }
+}
+
+/* Strided GEMM without transposes, scalar branch of mat__gemm_strided_
+ * (mat.h:2719-2743): per-column beta-scale, then j-k-i with alpha folded
+ * into B(k,j). */
+static void gemm_strided_nn(elem *C, size_t ldc, elem alpha, const elem *A,
+                            size_t lda, const elem *B, size_t ldb, size_t M,
+                            size_t K, size_t N, elem beta) {
+  for (size_t j = 0; j < N; j++) scal(&C[j * ldc], beta, M);
+  for (size_t j = 0; j < N; j++) {
+    for (size_t k = 0; k < K; k++) {
+      elem ab = alpha * B[j * ldb + k];
+      for (size_t i = 0; i < M; i++) C[j * ldc + i] += A[k * lda + i] * ab;
+    }
+  }
+}
+
+#define PLU_NB 24 /* MAT_PLU_BLOCK_SIZE, mat.h:4196 */
+
+/* In-place blocked right-looking partial-pivot LU with delayed row swaps
+ * (mat__plu_blocked_, mat.h:4201-4288).  p holds the row permutation
+ * (p[i] = original row now at row i); returns the number of row swaps. */
+static int plu_inplace(size_t n, elem *a, size_t *p) {
+  int swaps = 0;
+  size_t piv[PLU_NB];
+  for (size_t kb = 0; kb < n; kb += PLU_NB) {
+    size_t ke = kb + PLU_NB < n ? kb + PLU_NB : n;
+    size_t bk = ke - kb;
+    for (size_t k = kb; k < ke && k < n - 1; k++) {
+      elem *ck = &a[k * n];
+      size_t pr = k + iamax(&ck[k], n - k);
+      piv[k - kb] = pr;
+      if (pr != k) {
+        for (size_t j = 0; j < ke; j++) {
+          elem t = a[j * n + k];
+          a[j * n + k] = a[j * n + pr];
+          a[j * n + pr] = t;
+        }
+        size_t t = p[k];
+        p[k] = p[pr];
+        p[pr] = t;
+        swaps++;
+      }
+      if (ORC_ABS(ck[k]) < ORC_EPS) continue; /* near-zero pivot: skip */
+      elem inv = 1.0f / ck[k];
+      scal(&ck[k + 1], inv, n - (k + 1));
+      for (size_t j = k + 1; j < ke; j++)
+        axpy(&a[j * n + k + 1], -a[j * n + k], &ck[k + 1], n - (k + 1));
+    }
+    if (ke < n) {
+      for (size_t k = kb; k < ke && k < n - 1; k++) {
+        size_t pr = piv[k - kb];
+        if (pr != k) {
+          for (size_t j = ke; j < n; j++) {
+            elem t = a[j * n + k];
+            a[j * n + k] = a[j * n + pr];
+            a[j * n + pr] = t;
+          }
+        }
+      }
+      for (size_t k = kb; k < ke; k++)
+        for (size_t j = ke; j < n; j++)
+          axpy(&a[j * n + k + 1], -a[j * n + k], &a[k * n + k + 1],
+               ke - (k + 1));
+      gemm_strided_nn(&a[ke * n + ke], n, -1.0f, &a[kb * n + ke], n,
+                      &a[ke * n + kb], n, n - ke, bk, n - ke, 1.0f);
+    }
+  }
+  return swaps;
+}
+
+/* Split packed LU into unit-lower L and upper U (mat.h:4456-4470). */
+static void split_lu(size_t n, const elem *m, elem *L, elem *U) {
+  memset(L, 0, n * n * sizeof(elem));
+  memset(U, 0, n * n * sizeof(elem));
+  for (size_t i = 0; i < n; i++) L[i * n + i] = 1;
+  for (size_t j = 0; j < n; j++) {
+    for (size_t i = j + 1; i < n; i++) L[j * n + i] = m[j * n + i];
+    for (size_t i = 0; i <= j; i++) U[j * n + i] = m[j * n + i];
+  }
+}
+
+/* mat_plu (mat.h:4436-4474): PA = LU, returns row-swap count. */
+int ORC(plu)(size_t n, const elem *A, elem *L, elem *U, size_t *p) {
+  elem *m = (elem *)malloc(n * n * sizeof(elem));
+  memcpy(m, A, n * n * sizeof(elem));
+  for (size_t i = 0; i < n; i++) p[i] = i;
+  int s = plu_inplace(n, m, p);
+  split_lu(n, m, L, U);
+  free(m);
+  return s;
+}
+
+/* Packed form of mat_plu for tests that want the in-place factor. */
+int ORC(plu_packed)(size_t n, elem *a, size_t *p) {
+  for (size_t i = 0; i < n; i++) p[i] = i;
+  return plu_inplace(n, a, p);
+}
+
+/* mat_lu, x86 scalar branch (mat.h:4476-4578): full pivoting PAQ = LU.
+ * Pivot scan is row-major (i outer, j inner), strict '>' starting from
+ * |A(k,k)|; a step is skipped when the max is below eps; rows are swapped
+ * before columns.  Returns row swaps + column swaps. */
+int ORC(lu)(size_t n, const elem *A, elem *L, elem *U, size_t *p,
+            size_t *q) {
+  elem *d = (elem *)malloc(n * n * sizeof(elem));
+  memcpy(d, A, n * n * sizeof(elem));
+  for (size_t i = 0; i < n; i++) p[i] = q[i] = i;
+  int swaps = 0;
+  for (size_t k = 0; k + 1 < n; k++) {
+    size_t pr = k, pc = k;
+    elem mx = ORC_ABS(d[k * n + k]);
+    for (size_t i = k; i < n; i++)
+      for (size_t j = k; j < n; j++) {
+        elem v = ORC_ABS(d[j * n + i]);
+        if (v > mx) {
+          mx = v;
+          pr = i;
+          pc = j;
+        }
+      }
+    if (mx < ORC_EPS) continue;
+    if (pr != k) {
+      for (size_t j = 0; j < n; j++) {
+        elem t = d[j * n + k];
+        d[j * n + k] = d[j * n + pr];
+        d[j * n + pr] = t;
+      }
+      size_t t = p[k];
+      p[k] = p[pr];
+      p[pr] = t;
+      swaps++;
+    }
+    if (pc != k) {
+      for (size_t i = 0; i < n; i++) {
+        elem t = d[k * n + i];
+        d[k * n + i] = d[pc * n + i];
+        d[pc * n + i] = t;
+      }
+      size_t t = q[k];
+      q[k] = q[pc];
+      q[pc] = t;
+      swaps++;
+    }
+    elem inv = 1.0f / d[k * n + k];
+    for (size_t i = k + 1; i < n; i++) {
+      elem l = d[k * n + i] * inv;
+      d[k * n + i] = l;
+      for (size_t j = k + 1; j < n; j++) d[j * n + i] -= l * d[j * n + k];
+    }
+  }
+  split_lu(n, d, L, U);
+  free(d);
+  return swaps;
+}
+
+/* mat_chol (mat.h:6062-6095) with the x86 kernel mat__chol_scalar_generic_
+ * (mat.h:6031-6060): copy the lower triangle, zero the upper, then
+ * left-looking column Cholesky; -1 at the first non-positive pivot. */
+int ORC(chol)(size_t n, const elem *A, elem *L) {
+  for (size_t j = 0; j < n; j++) {
+    for (size_t i = 0; i < j; i++) L[j * n + i] = 0;
+    memcpy(&L[j * n + j], &A[j * n + j], (n - j) * sizeof(elem));
+  }
+  for (size_t j = 0; j < n; j++) {
+    elem s = 0;
+    for (size_t k = 0; k < j; k++) {
+      elem v = AT(L, n, j, k);
+      s += v * v;
+    }
+    elem dg = AT(L, n, j, j) - s;
+    if (dg <= 0) return -1;
+    elem ljj = ORC_SQRT(dg);
+    AT(L, n, j, j) = ljj;
+    elem inv = 1.0f / ljj;
+    for (size_t i = j + 1; i < n; i++) {
+      elem dot = 0;
+      for (size_t k = 0; k < j; k++) dot += AT(L, n, i, k) * AT(L, n, j, k);
+      AT(L, n, i, j) = (AT(L, n, i, j) - dot) * inv;
+    }
+  }
+  return 0;
+}
+
+/* Lx = b, non-unit lower (mat__solve_tril_scalar, mat.h:4584-4598). */
+void ORC(solve_tril)(size_t n, const elem *L, const elem *b, elem *x) {
+  for (size_t i = 0; i < n; i++) {
+    elem dot = 0;
+    for (size_t j = 0; j < i; j++) dot += AT(L, n, i, j) * x[j];
+    x[i] = (b[i] - dot) / AT(L, n, i, i);
+  }
+}
+
+/* Lx = b, unit lower (mat__solve_tril_unit_scalar, mat.h:4638-4652). */
+void ORC(solve_tril_unit)(size_t n, const elem *L, const elem *b, elem *x) {
+  for (size_t i = 0; i < n; i++) {
+    elem dot = 0;
+    for (size_t j = 0; j < i; j++) dot += AT(L, n, i, j) * x[j];
+    x[i] = b[i] - dot;
+  }
+}
+
+/* Ux = b, upper (mat__solve_triu_scalar, mat.h:4722-4736). */
+void ORC(solve_triu)(size_t n, const elem *U, const elem *b, elem *x) {
+  for (size_t i = n; i-- > 0;) {
+    elem s = b[i];
+    for (size_t j = i + 1; j < n; j++) s -= AT(U, n, i, j) * x[j];
+    x[i] = s / AT(U, n, i, i);
+  }
+}
+
+/* L^T x = b via column dots (mat__solve_trilt_scalar, mat.h:4773-4790). */
+void ORC(solve_trilt)(size_t n, const elem *L, const elem *b, elem *x) {
+  for (size_t j = n; j-- > 0;) {
+    const elem *c = &L[j * n];
+    elem dot = 0;
+    for (size_t k = j + 1; k < n; k++) dot += c[k] * x[k];
+    x[j] = (b[j] - dot) / c[j];
+  }
+}
+
+/* mat_solve (mat.h:4807-4837): PLU, permute b, unit-lower then upper
+ * substitution.  One right-hand side. */
+void ORC(solve)(size_t n, const elem *A, const elem *b, elem *x) {
+  elem *L = (elem *)malloc(n * n * sizeof(elem));
+  elem *U = (elem *)malloc(n * n * sizeof(elem));
+  size_t *p = (size_t *)malloc(n * sizeof(size_t));
+  elem *pb = (elem *)malloc(n * sizeof(elem));
+  elem *z = (elem *)malloc(n * sizeof(elem));
+  ORC(plu)(n, A, L, U, p);
+  for (size_t i = 0; i < n; i++) pb[i] = b[p[i]];
+  ORC(solve_tril_unit)(n, L, pb, z);
+  ORC(solve_triu)(n, U, z, x);
+  free(L);
+  free(U);
+  free(p);
+  free(pb);
+  free(z);
+}
+
+/* mat_solve_spd (mat.h:4839-4871): Cholesky, then L y = b, L^T x = y.
+ * Returns -1 (x untouched) when A is not positive definite. */
+int ORC(solve_spd)(size_t n, const elem *A, const elem *b, elem *x) {
+  elem *L = (elem *)malloc(n * n * sizeof(elem));
+  elem *y = (elem *)calloc(n, sizeof(elem));
+  int r = ORC(chol)(n, A, L);
+  if (r != 0) {
+    free(L);
+    free(y);
+    return -1;
+  }
+  ORC(solve_tril)(n, L, b, y);
+  ORC(solve_trilt)(n, L, y, x);
+  free(L);
+  free(y);
+  return 0;
+}
+
+/* Factor once, then k right-hand sides through the two substitutions,
+ * which is what mat_solve does internally per call (mat.h:4823-4830).
+ * B and X are n x nrhs, column-major. Used as the multi-RHS CPU baseline. */
+void ORC(solve_multi)(size_t n, size_t nrhs, const elem *A, const elem *B,
+                      elem *X) {
+  elem *L = (elem *)malloc(n * n * sizeof(elem));
+  elem *U = (elem *)malloc(n * n * sizeof(elem));
+  size_t *p = (size_t *)malloc(n * sizeof(size_t));
+  elem *pb = (elem *)malloc(n * sizeof(elem));
+  elem *z = (elem *)malloc(n * sizeof(elem));
+  ORC(plu)(n, A, L, U, p);
+  for (size_t c = 0; c < nrhs; c++) {
+    for (size_t i = 0; i < n; i++) pb[i] = B[c * n + p[i]];
+    ORC(solve_tril_unit)(n, L, pb, z);
+    ORC(solve_triu)(n, U, z, &X[c * n]);
+  }
+  free(L);
+  free(U);
+  free(p);
+  free(pb);
+  free(z);
+}
+
+/* mat_det (mat.h:6227-6249): (-1)^swaps * prod diag(U). */
+elem ORC(det)(size_t n, const elem *A) {
+  elem *L = (elem *)malloc(n * n * sizeof(elem));
+  elem *U = (elem *)malloc(n * n * sizeof(elem));
+  size_t *p = (size_t *)malloc(n * sizeof(size_t));
+  int s = ORC(plu)(n, A, L, U, p);
+  elem d = (s % 2 == 0) ? 1 : -1;
+  for (size_t i = 0; i < n; i++) d *= U[i * n + i];
+  free(L);
+  free(U);
+  free(p);
+  return d;
+}

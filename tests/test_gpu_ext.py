@@ -1,0 +1,138 @@
+"""Device-resident extension entry points (include/mat_gpu.h part 2) at
+sizes past the drop-in tests: MFMA GEMM shapes and transposes, the
+multi-workgroup pivoting path, multi-RHS solves, and full-size properties
+(sampled fp64 dot products for the 8192^3 GEMM)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+import torch
+
+from _util import DT, EPS, apply_row_perm, backward_error, from_dev, pivots_match, rel_fro, \
+    stream, to_dev
+from oracle.oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+PRECS = ["f32", "f64"]
+
+
+def ptr(t):
+    return C.c_void_p(t.data_ptr())
+
+
+@pytest.mark.parametrize("prec", PRECS)
+@pytest.mark.parametrize("m,n,k,ta,tb", [(1, 1, 1, 0, 0), (129, 70, 33, 0, 0), (64, 256, 200, 1, 0),
+                                         (300, 17, 128, 0, 1), (5, 400, 9, 1, 1),
+                                         (513, 257, 1000, 0, 0)])
+def test_gemm_shapes(gpu, prec, m, n, k, ta, tb):
+    g = gpu[prec]
+    rng = np.random.default_rng(m * 7 + n)
+    A = rng.uniform(-1, 1, (k, m) if ta else (m, k))
+    B = rng.uniform(-1, 1, (n, k) if tb else (k, n))
+    Cm = rng.uniform(-1, 1, (m, n))
+    dA, dB, dC = to_dev(A, DT[prec]), to_dev(B, DT[prec]), to_dev(Cm, DT[prec])
+    fn = getattr(g.lib, f"mat_gpu_gemm_{prec}")
+    rc = fn(stream(), ta, tb, m, n, k, 1.25, ptr(dA), A.shape[0], ptr(dB), B.shape[0], -0.5,
+            ptr(dC), m)
+    g.check(rc, "gemm")
+    torch.cuda.synchronize()
+    ref = 1.25 * (A.T if ta else A) @ (B.T if tb else B) - 0.5 * Cm
+    assert rel_fro(from_dev(dC, m, n), ref) <= 2 * k * EPS[prec]
+
+
+def test_gemm_8192_sampled():
+    """M=N=K=8192: 256 entries recomputed as fp64 dot products on the host."""
+    import libmat_amd
+    g = libmat_amd.load("f32")
+    n = 8192
+    gen = torch.Generator(device="cuda").manual_seed(1234)
+    dA = (torch.rand(n * n, device="cuda", generator=gen) * 2 - 1)
+    dB = (torch.rand(n * n, device="cuda", generator=gen) * 2 - 1)
+    dC = torch.empty(n * n, device="cuda")
+    g.check(g.lib.mat_gpu_gemm_f32(stream(), 0, 0, n, n, n, 1.0, ptr(dA), n, ptr(dB), n, 0.0,
+                                   ptr(dC), n), "gemm")
+    torch.cuda.synchronize()
+    rng = np.random.default_rng(9)
+    ii, jj = rng.integers(0, n, 256), rng.integers(0, n, 256)
+    A = dA.view(n, n)  # [col][row] since storage is column-major
+    B = dB.view(n, n)
+    rows = A[:, torch.as_tensor(ii, device="cuda")].T.double().cpu().numpy()  # A[i, :]
+    cols = B[torch.as_tensor(jj, device="cuda"), :].double().cpu().numpy()  # B[:, j]
+    ref = np.einsum("ik,ik->i", rows, cols)
+    got = dC.view(n, n)[torch.as_tensor(jj, device="cuda"), torch.as_tensor(ii, device="cuda")]
+    got = got.double().cpu().numpy()
+    scale = np.sqrt(np.einsum("ik,ik->i", rows ** 2, cols ** 2))
+    assert np.max(np.abs(got - ref) / scale) <= 2 * np.sqrt(n) * EPS["f32"] * 8
+
+
+@pytest.mark.parametrize("prec", PRECS)
+@pytest.mark.parametrize("n,leaf", [(300, 32), (1000, 24), (1000, 32), (2500, 32)])
+def test_getrf_vs_oracle(gpu, prec, n, leaf):
+    """Multi-workgroup pivoting (m > 256 rows) against the oracle's PLU."""
+    g = gpu[prec]
+    rng = np.random.default_rng(n + leaf)
+    A = rng.uniform(-1, 1, (n, n)).astype(DT[prec])
+    dA = to_dev(A, DT[prec])
+    perm = torch.zeros(n, dtype=torch.int32, device="cuda")
+    sw = C.c_int(0)
+    rc = getattr(g.lib, f"mat_gpu_getrf_{prec}")(stream(), n, ptr(dA), n, ptr(perm), leaf,
+                                                 C.byref(sw))
+    g.check(rc, "getrf")
+    torch.cuda.synchronize()
+    LU = from_dev(dA, n, n).astype(np.float64)
+    L = np.tril(LU, -1) + np.eye(n)
+    U = np.triu(LU)
+    p = perm.cpu().numpy()
+    assert sorted(p.tolist()) == list(range(n))
+    assert rel_fro(L @ U, apply_row_perm(A, p)) <= 8 * n * EPS[prec]
+    if n <= 1000:
+        Lr, Ur, pr, sr = Oracle(prec).plu(A)
+        ok, k = pivots_match(p, pr, Lr, prec)
+        assert ok, f"pivot mismatch at step {k}"
+        if k is None:
+            assert sw.value == sr
+    # solve with the packed factor, 5 right-hand sides
+    B = rng.uniform(-1, 1, (n, 5))
+    dB, dX = to_dev(B, DT[prec]), torch.zeros(n * 5, dtype=dA.dtype, device="cuda")
+    g.check(getattr(g.lib, f"mat_gpu_getrs_{prec}")(stream(), n, 5, ptr(dA), n, ptr(perm), ptr(dB),
+                                                    n, ptr(dX), n), "getrs")
+    torch.cuda.synchronize()
+    assert backward_error(A, from_dev(dX, n, 5), B) <= 8 * n * EPS[prec]
+
+
+@pytest.mark.parametrize("prec", PRECS)
+@pytest.mark.parametrize("n", [100, 700, 2049])
+def test_gesv_posv(gpu, prec, n):
+    g = gpu[prec]
+    rng = np.random.default_rng(n)
+    A = rng.uniform(-1, 1, (n, n))
+    G = rng.standard_normal((n, n))
+    S = G @ G.T / n + np.eye(n)
+    B = rng.uniform(-1, 1, (n, 8))
+    for fn, M in (("gesv", A), ("posv", S)):
+        dA, dB = to_dev(M, DT[prec]), to_dev(B, DT[prec])
+        dX = torch.zeros(n * 8, dtype=dA.dtype, device="cuda")
+        rc = getattr(g.lib, f"mat_gpu_{fn}_{prec}")(stream(), n, 8, ptr(dA), n, ptr(dB), n,
+                                                    ptr(dX), n)
+        assert g.check(rc, fn) == 0
+        torch.cuda.synchronize()
+        assert backward_error(M.astype(DT[prec]), from_dev(dX, n, 8), B) <= 8 * n * EPS[prec], fn
+        # A must be untouched
+        assert np.array_equal(from_dev(dA, n, n), M.astype(DT[prec]))
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_potrf_not_spd(gpu, prec):
+    g = gpu[prec]
+    n = 300
+    rng = np.random.default_rng(1)
+    G = rng.standard_normal((n, n))
+    S = G @ G.T / n + np.eye(n)
+    S[200, 200] = -5.0  # fails at (or before) column 200
+    dA = to_dev(S, DT[prec])
+    assert getattr(g.lib, f"mat_gpu_potrf_{prec}")(stream(), n, ptr(dA), n) == -1
+    dA = to_dev(G @ G.T / n + np.eye(n), DT[prec])
+    assert getattr(g.lib, f"mat_gpu_potrf_{prec}")(stream(), n, ptr(dA), n) == 0
+    L = from_dev(dA, n, n)
+    assert np.all(np.triu(L, 1) == 0)
+    assert rel_fro(L.astype(np.float64) @ L.T, G @ G.T / n + np.eye(n)) <= 8 * n * EPS[prec]

@@ -1,0 +1,195 @@
+"""The oracle pinned: bit-exact against the fixtures the reference wrote,
+against the reference's own known-answer tests (tests/test_mat_*.c), and
+against invariants / library routines that a plausible mistake would break."""
+import numpy as np
+import pytest
+import scipy.linalg as sla
+
+from _util import DT, EPS, apply_row_perm, golden, rel_fro
+from oracle.oracle import Oracle
+
+PRECS = ["f32", "f64"]
+
+
+@pytest.fixture(scope="module", params=PRECS)
+def orc(request):
+    return Oracle(request.param)
+
+
+def keys(g, prefix):
+    return sorted({k.rsplit("_", 1)[0] for k in g if k.startswith(prefix)})
+
+
+# ---------------------------------------------------- fixtures, bit-exact --
+def test_gemm_fixtures(orc):
+    g = golden(orc.prec)
+    for t in [k[:-2] for k in g if k.startswith("gemm_") and k.endswith("_A")]:
+        out = orc.gemm(0.7, g[t + "_A"], g[t + "_B"], 0.3, g[t + "_C0"])
+        assert np.array_equal(out, g[t + "_C"]), t
+
+
+def test_cfg1_fixture():
+    o, g = Oracle("f32"), golden("f32")
+    assert np.array_equal(o.mul(g["cfg1_A"], g["cfg1_B"]), g["cfg1_C"])
+    L, U, p, s = o.plu(g["cfg1_A"])
+    assert np.array_equal(L, g["cfg1_L"]) and np.array_equal(U, g["cfg1_U"])
+    assert np.array_equal(p, g["cfg1_p"]) and s == g["cfg1_swaps"]
+    assert np.array_equal(o.solve(g["cfg1_A"], g["cfg1_b"]), g["cfg1_x"])
+
+
+def test_plu_solve_det_fixtures(orc):
+    g = golden(orc.prec)
+    tags = [k[:-2] for k in g if (k.startswith("plu_") or k.startswith("sing") or
+                                  k.startswith("zerocol") or k.startswith("rank2")) and k.endswith("_A")]
+    assert len(tags) >= 15
+    for t in tags:
+        L, U, p, s = orc.plu(g[t + "_A"])
+        assert np.array_equal(L, g[t + "_L"]) and np.array_equal(U, g[t + "_U"]), t
+        assert np.array_equal(p, g[t + "_p"]) and s == g[t + "_swaps"], t
+        assert orc.det(g[t + "_A"]) == g[t + "_det"] or (np.isnan(g[t + "_det"])), t
+        if t + "_x" in g:
+            assert np.array_equal(orc.solve(g[t + "_A"], g[t + "_b"]), g[t + "_x"]), t
+
+
+def test_lu_fixtures(orc):
+    g = golden(orc.prec)
+    for t in [k[:-2] for k in g if k.startswith("lu_") and k.endswith("_A")]:
+        L, U, p, q, s = orc.lu(g[t + "_A"])
+        for a, b in ((L, "_L"), (U, "_U"), (p, "_p"), (q, "_q")):
+            assert np.array_equal(a, g[t + b]), t + b
+        assert s == g[t + "_swaps"]
+
+
+def test_chol_fixtures(orc):
+    g = golden(orc.prec)
+    for t in [k[:-2] for k in g if k.startswith("chol_") and k.endswith("_A")]:
+        L, rc = orc.chol(g[t + "_A"])
+        assert rc == g[t + "_rc"] == 0 and np.array_equal(L, g[t + "_L"]), t
+        x, rs = orc.solve_spd(g[t + "_A"], g[t + "_b"])
+        assert rs == 0 and np.array_equal(x, g[t + "_x"]), t
+    for t in ("notpd2", "indef40"):
+        assert orc.chol(g[t + "_A"])[1] == -1 == g[t + "_rc"]
+        assert orc.solve_spd(g[t + "_A"], np.ones(g[t + "_A"].shape[0]))[1] == -1
+
+
+def test_tri_fixtures(orc):
+    g = golden(orc.prec)
+    for t in [k[:-2] for k in g if k.startswith("tri_") and k.endswith("_T")]:
+        T, b = g[t + "_T"], g[t + "_b"]
+        assert np.array_equal(orc.solve_tril(np.tril(T), b), g[t + "_tril"])
+        assert np.array_equal(orc.solve_tril_unit(np.tril(T), b), g[t + "_tril_unit"])
+        assert np.array_equal(orc.solve_triu(np.triu(T), b), g[t + "_triu"])
+        assert np.array_equal(orc.solve_trilt(np.tril(T), b), g[t + "_trilt"])
+
+
+# ------------------------------------------ reference known-answer tests --
+def test_mul_known_answers(orc):
+    # tests/test_mat_mul.c:6-90
+    A = np.array([[1, 2], [3, 4]])
+    assert np.array_equal(orc.mul(A, [[5, 6], [7, 8]]), [[19, 22], [43, 50]])
+    assert np.array_equal(orc.mul(A, np.eye(2)), A)
+    assert np.array_equal(orc.mul([[1, 2, 3], [4, 5, 6]], [[7, 8], [9, 10], [11, 12]]),
+                          [[58, 64], [139, 154]])
+    assert np.array_equal(orc.mul(A, [[5], [6]]), [[17], [39]])
+
+
+def test_plu_known_answers(orc):
+    # tests/test_mat_plu.c:17-80, 121-155
+    for A in ([[4, 3], [6, 3]], [[1, 2, 3], [0, 1, 4], [5, 6, 0]]):
+        L, U, p, s = orc.plu(A)
+        assert np.abs(apply_row_perm(A, p) - L @ U).max() <= 1e-5
+    A = np.array([[6, 1, 1], [4, -2, 5], [2, 8, 7]])
+    L, U, p, q, s = orc.lu(A)
+    det_lu = (1 if s % 2 == 0 else -1) * np.prod(np.diag(U))
+    assert abs(orc.det(A) - det_lu) < 1e-4
+    assert abs(orc.det(A) - np.linalg.det(A)) < 1e-3
+
+
+def test_solve_known_answers(orc):
+    # tests/test_mat_solve.c:7-136
+    assert np.allclose(orc.solve([[5]], [15]), [3], atol=1e-6)
+    assert np.allclose(orc.solve([[2, 3], [4, 5]], [8, 14]), [1, 2], atol=1e-5)
+    assert np.allclose(orc.solve([[1, 2, 3], [0, 1, 4], [5, 6, 0]], [14, 14, 17]), [1, 2, 3],
+                       atol=1e-4)
+
+
+def test_spd_known_answers(orc):
+    # tests/test_mat_chol.c:67-131, test_mat_solve_spd.c:50-122
+    L, rc = orc.chol([[4, 2], [2, 5]])
+    assert rc == 0 and np.allclose(L, [[2, 0], [1, 2]], atol=1e-6)
+    A = np.array([[4, 12, -16], [12, 37, -43], [-16, -43, 98]], float)
+    L, rc = orc.chol(A)
+    assert rc == 0 and np.allclose(L, [[2, 0, 0], [6, 1, 0], [-8, 5, 3]], atol=1e-5)
+    x, rc = orc.solve_spd([[4, 2], [2, 3]], [8, 8])
+    assert rc == 0 and np.allclose(x, [1, 2], atol=1e-5)
+    x, rc = orc.solve_spd([[5, 4, 1], [4, 6, 4], [1, 4, 5]], [16, 28, 24])
+    assert rc == 0 and np.allclose(x, [1, 2, 3], atol=1e-4)
+    assert orc.solve_spd([[1, 2], [2, 1]], [1, 1])[1] == -1
+    assert orc.chol([[1, 2], [2, 1]])[1] == -1
+
+
+def test_tri_known_answers(orc):
+    # tests/test_mat_solve_tri.c:7-123
+    assert np.allclose(orc.solve_tril([[2, 0], [3, 4]], [2, 11]), [1, 2])
+    assert np.allclose(orc.solve_tril_unit([[1, 0], [3, 1]], [1, 5]), [1, 2])
+    assert np.allclose(orc.solve_triu([[2, 3], [0, 4]], [8, 8]), [1, 2])
+    assert np.allclose(orc.solve_trilt([[2, 0], [3, 4]], [8, 8]), [1, 2])
+
+
+def test_det_singular(orc):
+    # tests/test_mat_det.c:55-71: the near-zero pivot is skipped, det -> 0
+    assert abs(orc.det([[1, 2, 3], [4, 5, 6], [5, 7, 9]])) < 1e-5
+
+
+# ----------------------------------------- mathematics pins the rest ------
+@pytest.mark.parametrize("n", [5, 37, 100])
+def test_plu_invariants_and_lapack(orc, n):
+    rng = np.random.default_rng(n)
+    A = rng.uniform(-1, 1, (n, n)).astype(DT[orc.prec])
+    L, U, p, s = orc.plu(A)
+    assert sorted(p.tolist()) == list(range(n))
+    assert np.allclose(np.triu(L, 1), 0) and np.allclose(np.diag(L), 1)
+    assert np.allclose(np.tril(U, -1), 0) and np.abs(L).max() <= 1
+    assert rel_fro(L @ U, apply_row_perm(A, p)) <= 4 * n * EPS[orc.prec]
+    # LAPACK getrf picks the same pivot rule (first max |a|); compare the
+    # permutation (well-separated random data) and the sign of det P
+    lu, piv = sla.lu_factor(A.astype(np.float64))
+    perm = np.arange(n)
+    for k, r in enumerate(piv):
+        perm[[k, r]] = perm[[r, k]]
+    assert np.array_equal(perm, p)
+    assert (-1) ** s == np.linalg.det(np.eye(n)[p])
+
+
+def test_gemm_brute_force(orc):
+    rng = np.random.default_rng(3)
+    A, B, Cm = (rng.uniform(-1, 1, s) for s in ((31, 45), (45, 12), (31, 12)))
+    out = orc.gemm(1.5, A, B, -0.5, Cm)
+    ref = 1.5 * A @ B - 0.5 * Cm
+    assert rel_fro(out, ref) <= 45 * EPS[orc.prec]
+    # beta == 0 must not read C (the reference zero-fills, mat.h:994-998)
+    nanC = np.full((31, 12), np.nan)
+    assert np.all(np.isfinite(orc.gemm(1.0, A, B, 0.0, nanC)))
+
+
+@pytest.mark.parametrize("n", [6, 50])
+def test_chol_vs_lapack(orc, n):
+    rng = np.random.default_rng(n + 1)
+    g = rng.standard_normal((n, n))
+    A = (g @ g.T / n + np.eye(n)).astype(DT[orc.prec])
+    L, rc = orc.chol(A)
+    assert rc == 0
+    assert rel_fro(L, np.linalg.cholesky(A.astype(np.float64))) <= 8 * n * EPS[orc.prec]
+    # only the lower triangle is read (mat.h:6072-6082)
+    junk = A + np.triu(np.full((n, n), 7.0), 1)
+    assert np.array_equal(orc.chol(junk)[0], L)
+
+
+def test_lu_full_pivot_invariants(orc):
+    rng = np.random.default_rng(11)
+    A = rng.uniform(-10, 10, (30, 30)).astype(DT[orc.prec])
+    L, U, p, q, s = orc.lu(A)
+    PAQ = A[p][:, q]
+    assert rel_fro(L @ U, PAQ) <= 200 * EPS[orc.prec]
+    # full pivoting: |U[k,k]| is the max of the trailing block at step k
+    assert np.all(np.abs(np.diag(U))[:-1] >= np.abs(np.triu(U, 1)).max(axis=1)[:-1] * (1 - 1e-6))
