@@ -17,7 +17,7 @@
  *
  * Part 2 is the device-resident extension API (no counterpart in the
  * reference): plain device pointers, column-major, explicit leading
- * dimensions, a HIP stream passed as void* (NULL = the library's stream).
+ * dimensions, a HIP stream passed as void* (NULL = the HIP null stream).
  * They return 0 on success, a negative code on error (see
  * mat_gpu_last_error()), and -1 from the Cholesky routines when the matrix
  * is not positive definite, as mat_chol does.
@@ -126,6 +126,13 @@ int mat_gpu_gemm_f64(void *stream, int trans_a, int trans_b, size_t m, size_t n,
                      double alpha, const double *a, size_t lda, const double *b, size_t ldb,
                      double beta, double *c, size_t ldc);
 
+/* C += alpha * A * B (all column-major fp32) on low-precision MFMA:
+ * passes 1/3/6 = BF16 with 1/3/6 exact-split products, 8 = FP8 e4m3 with
+ * one power-of-two scale per 128x128 tile; fp32 accumulation.  This is the
+ * trailing-update multiply of the mixed-precision factorizations. */
+int mat_gpu_gemm_lowp(void *stream, int passes, size_t m, size_t n, size_t k, float alpha,
+                      const float *a, size_t lda, const float *b, size_t ldb, float *c, size_t ldc);
+
 /* In-place partial-pivot LU of the n x n column-major a (packed: unit-lower
  * L below the diagonal, U on and above), libmat's pivot rules and eps-skip.
  * perm (device, n int32) gets libmat's Perm; *swaps (host) the swap count.
@@ -161,7 +168,8 @@ int mat_gpu_posv_f64(void *stream, size_t n, size_t nrhs, const double *a, size_
                      const double *b, size_t ldb, double *x, size_t ldx);
 
 /* Mixed-precision solvers with fp64 iterative refinement.
- *   passes: trailing-update multiply of the fp32 factorization:
+ *   passes: trailing-update multiply of the fp32 factorization (default
+ *           6 for gesv, 8 for posv):
  *           0 = fp32 MFMA, 1/3/6 = BF16 MFMA with 1/3/6 split products,
  *           8 = FP8 e4m3 with per-128x128-tile scales (posv only).
  *   The residual b - A x (A in fp64) and x += d are the only fp64 steps.
@@ -169,10 +177,14 @@ int mat_gpu_posv_f64(void *stream, size_t n, size_t nrhs, const double *a, size_
  *   <= tol (infinity norms); falls back to the plain fp64 factorization
  *   when refinement stalls.  Synchronous. */
 typedef struct {
-  int passes;   /* default 3 (gesv) / 8 (posv) */
+  int passes;   /* default 6 (gesv) / 8 (posv) */
   int max_iter; /* default 30 */
   double tol;   /* default 1e-14 */
   int leaf;     /* LU pivoting block width, default 32 */
+  int pivoting; /* gesv only: 0 = partial pivoting, libmat's rule
+                   (default); 1 = recursive random butterfly transform +
+                   no pivoting (faster factorization, but its growth can
+                   stall refinement on ill-conditioned matrices) */
 } mat_gpu_ir_opts;
 typedef struct {
   int iterations;        /* refinement steps taken */

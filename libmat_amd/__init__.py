@@ -33,7 +33,7 @@ DROPIN_SYMBOLS = [
 ]
 EXT_SYMBOLS = [
     "mat_gpu_last_error", "mat_gpu_elem_size",
-    "mat_gpu_gemm_f32", "mat_gpu_gemm_f64", "mat_gpu_getrf_f32", "mat_gpu_getrf_f64",
+    "mat_gpu_gemm_f32", "mat_gpu_gemm_f64", "mat_gpu_gemm_lowp", "mat_gpu_getrf_f32", "mat_gpu_getrf_f64",
     "mat_gpu_getrs_f32", "mat_gpu_getrs_f64", "mat_gpu_potrf_f32", "mat_gpu_potrf_f64",
     "mat_gpu_potrs_f32", "mat_gpu_potrs_f64", "mat_gpu_gesv_f32", "mat_gpu_gesv_f64",
     "mat_gpu_posv_f32", "mat_gpu_posv_f64", "mat_gpu_ir_defaults", "mat_gpu_gesv_mixed",
@@ -48,7 +48,7 @@ def lib_path(prec: str) -> str:
 
 class IrOpts(C.Structure):
     _fields_ = [("passes", C.c_int), ("max_iter", C.c_int), ("tol", C.c_double),
-                ("leaf", C.c_int)]
+                ("leaf", C.c_int), ("pivoting", C.c_int)]
 
 
 class IrStatus(C.Structure):
@@ -107,6 +107,7 @@ class GpuLib:
         s("mat_gpu_elem_size", i32, [])
         s("mat_gpu_gemm_f32", i32, [vp, i32, i32, sz, sz, sz, f32, vp, sz, vp, sz, f32, vp, sz])
         s("mat_gpu_gemm_f64", i32, [vp, i32, i32, sz, sz, sz, dbl, vp, sz, vp, sz, dbl, vp, sz])
+        s("mat_gpu_gemm_lowp", i32, [vp, i32, sz, sz, sz, f32, vp, sz, vp, sz, vp, sz])
         for t in ("f32", "f64"):
             s(f"mat_gpu_getrf_{t}", i32, [vp, sz, vp, sz, vp, i32, C.POINTER(i32)])
             s(f"mat_gpu_getrs_{t}", i32, [vp, sz, sz, vp, sz, vp, vp, sz, vp, sz])

@@ -355,7 +355,7 @@ int mat_gpu_batch_create_f32(void *stream, size_t count, const int32_t *n, const
                              mat_gpu_batch_plan **out) {
   return guarded([&] {
     MG_CHECK(out != nullptr && n != nullptr && kind != nullptr, "batch: null argument");
-    hipStream_t s = stream ? (hipStream_t)stream : mg::default_stream();
+    hipStream_t s = (hipStream_t)stream;
     Plan *pl = new Plan();
     std::vector<int> hn(n, n + count);
     MG_HIP(hipMalloc(&pl->d_n, std::max<size_t>(1, count) * sizeof(int)));
@@ -414,7 +414,7 @@ int mat_gpu_batch_run(void *stream, mat_gpu_batch_plan *plan) {
   return guarded([&] {
     MG_CHECK(plan != nullptr, "batch: null plan");
     Plan *pl = reinterpret_cast<Plan *>(plan);
-    hipStream_t s = stream ? (hipStream_t)stream : mg::default_stream();
+    hipStream_t s = (hipStream_t)stream;
     MG_HIP(hipGraphLaunch(pl->exec, s));
     return 0;
   });
@@ -453,7 +453,7 @@ int mat_gpu_batch_solve_f32(void *stream, size_t count, const int32_t *n, const 
                             const float *a, const int64_t *a_off, const float *b, float *x,
                             const int64_t *v_off, int32_t *info) {
   return guarded([&] {
-    hipStream_t s = stream ? (hipStream_t)stream : mg::default_stream();
+    hipStream_t s = (hipStream_t)stream;
     mg::Workspace &ws = mg::workspace(s);
     std::vector<int> order;
     std::vector<BatchTask> tasks;

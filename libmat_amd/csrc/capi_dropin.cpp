@@ -90,6 +90,12 @@ T *plu_device(hipStream_t s, const Mat *A, int **perm_dev, std::vector<int> &ipi
 
 namespace mg {
 template <typename U>
+int tri_block_size();
+template <typename U>
+void tri_invert_blocks(hipStream_t, bool, bool, int, View<const U>, U *);
+template <typename U>
+void tri_solve_blocked(hipStream_t, bool, int, int, View<const U>, const U *, U *, long);
+template <typename U>
 void getrs_rowmajor(hipStream_t, int, int, const U *, long, const int *, const U *, long, U *, long);
 template <typename U>
 void potrs_lower(hipStream_t, int, int, const U *, long, U *, long);
@@ -290,7 +296,10 @@ static void tri_solve(const char *fn, Vec *x, const Mat *Tm, const Vec *b, bool 
     MG_HIP(hipMemcpyAsync(dx, b->data, (size_t)n * sizeof(T), hipMemcpyHostToDevice, s));
     View<const T> v{dT, 1, n};
     if (transpose) v = v.t();
-    mg::trsm_left<T>(s, lower, unit, n, 1, v, View<T>{dx, 1, n});
+    const int tb = mg::tri_block_size<T>();
+    T *D = (T *)mg::workspace(s).get("dropin.D", (size_t)((n + tb - 1) / tb) * tb * tb * sizeof(T));
+    mg::tri_invert_blocks<T>(s, lower, unit, n, v, D);
+    mg::tri_solve_blocked<T>(s, lower, n, 1, v, D, dx, n);
     MG_HIP(hipMemcpyAsync(x->data, dx, (size_t)n * sizeof(T), hipMemcpyDeviceToHost, s));
     finish(s);
   });

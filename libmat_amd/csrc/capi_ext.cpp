@@ -22,7 +22,8 @@ using namespace mg;
 
 namespace {
 
-hipStream_t as_stream(void *s) { return s ? (hipStream_t)s : default_stream(); }
+// NULL selects the HIP null stream, like any HIP API taking a stream.
+hipStream_t as_stream(void *s) { return (hipStream_t)s; }
 
 template <typename F>
 int guarded(F &&f) {
@@ -236,12 +237,29 @@ int mat_gpu_posv_f64(void *s, size_t n, size_t nrhs, const double *a, size_t lda
   return posv_ext<double>(s, n, nrhs, a, lda, b, ldb, x, ldx);
 }
 
+int mat_gpu_gemm_lowp(void *s, int passes, size_t m, size_t n, size_t k, float alpha,
+                      const float *a, size_t lda, const float *b, size_t ldb, float *c, size_t ldc) {
+  return guarded([&] {
+    check_dims(m);
+    check_dims(n);
+    check_dims(k);
+    View<const float> A{a, 1, (long)lda}, B{b, 1, (long)ldb};
+    View<float> Cv{c, 1, (long)ldc};
+    if (passes == 8)
+      gemm_fp8(as_stream(s), (long)m, (long)n, (long)k, alpha, A, B, Cv, false);
+    else
+      gemm_bf16x(as_stream(s), (long)m, (long)n, (long)k, alpha, A, B, Cv, passes);
+    return 0;
+  });
+}
+
 void mat_gpu_ir_defaults(mat_gpu_ir_opts *o, int spd) {
   if (!o) return;
-  o->passes = spd ? 8 : 3;
+  o->passes = spd ? 8 : 6;
   o->max_iter = 30;
   o->tol = 1e-14;
   o->leaf = 32;
+  o->pivoting = 0;
 }
 
 int mat_gpu_gesv_mixed(void *s, size_t n, size_t nrhs, const double *a, size_t lda, const double *b,

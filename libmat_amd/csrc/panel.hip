@@ -8,26 +8,34 @@
 // column below by 1/pivot and apply the rank-1 update to the block's
 // remaining columns.  Columns k >= n-1 are not pivoted (mat.h:4214).
 //
-// MI355X design: the panel's rows are spread over P workgroups, each
-// holding its rows (all leaf columns) in registers for the whole leaf.  One
-// column step costs one cross-workgroup rendezvous: every workgroup
-// publishes its best candidate (|value|, row) plus the candidate row's
-// values, and the owner of row k publishes row k; after the rendezvous every
-// workgroup independently reduces the P candidates (same data, same answer),
-// swaps, scales and updates its own rows.  With P == 1 the rendezvous is an
-// LDS barrier.  The grid is launched cooperatively (all workgroups
-// resident); every spin wait is bounded and reports a timeout instead of
-// hanging.
+// MI355X design.  The panel's rows are spread over P workgroups of 512
+// threads, each thread holding RPT whole rows of the leaf in registers for
+// the whole leaf (register rotation keeps the current column in slot 0, so
+// every register index is static while the column loop stays a runtime
+// loop).  One column step:
+//   * each wavefront finds its best row with shuffles and parks that row,
+//     plus row k if it owns it, in LDS (double-buffered by step parity);
+//   * one barrier; every thread reduces the eight wavefront candidates;
+//   * with P > 1, wavefront 0 publishes the workgroup's candidate (value,
+//     row index, row) to global memory as self-validating 64-bit words
+//     (payload in the low half, a tag unique to this launch and step in the
+//     high half), polls the other workgroups' records until every tag
+//     matches, reduces the P candidates (same data, same answer in every
+//     workgroup) and parks the pivot row and row k in LDS; one more barrier.
+//     No fences, atomics or counters are needed and nothing is reset between
+//     launches: stale words carry an older tag.
+//   * swap, scale and rank-1 update of the thread's own rows.
+// Every poll is bounded and reports a timeout instead of hanging.
 #include <algorithm>
+#include <cstdint>
 
 #include "ops.hpp"
 
 namespace mg {
 
-constexpr int PANEL_NT = 256;
-
-
-constexpr long SPIN_LIMIT = 1L << 26;
+constexpr int PANEL_NT = 512;
+constexpr int PANEL_NW = PANEL_NT / 64;
+constexpr long SPIN_LIMIT = 1L << 24;
 
 template <typename T>
 struct PanelP {
@@ -40,12 +48,10 @@ struct PanelP {
   int nwg;     // workgroups taking part
   T eps;
   int *ipiv;
-  unsigned *bar;      // rendezvous counter, zero at launch
-  T *candv;           // [2][nwg]
-  int *candi;         // [2][nwg]
-  T *rowbuf;          // [2][nwg][WMAX]
-  T *diagbuf;         // [2][WMAX]
-  int *err;           // set to 1 on rendezvous timeout
+  unsigned epoch;        // distinct per launch (tags)
+  uint64_t *rec;         // [2][nwg][REC] tagged words: value, row, row values
+  uint64_t *krec;        // [2][KREC] tagged words: row k
+  int *err;              // set to 1 on a poll timeout
 };
 
 template <typename T>
@@ -56,24 +62,159 @@ __device__ inline void better(T &bv, int &bi, T v, int i) {
   }
 }
 
+// A value of type T as 32-bit payload words.
+template <typename T>
+struct Words {
+  static constexpr int N = sizeof(T) / 4;
+};
+
+// Wavefront max of 64-bit keys: DPP within each 16-lane row, readlane
+// across the four rows; every lane gets the result.
+template <int CTRL>
+__device__ inline uint64_t dpp_max(uint64_t k) {
+  const uint32_t lo = __builtin_amdgcn_update_dpp((uint32_t)k, (uint32_t)k, CTRL, 0xf, 0xf, false);
+  const uint32_t hi =
+      __builtin_amdgcn_update_dpp((uint32_t)(k >> 32), (uint32_t)(k >> 32), CTRL, 0xf, 0xf, false);
+  const uint64_t o = ((uint64_t)hi << 32) | lo;
+  return o > k ? o : k;
+}
+
+__device__ inline uint64_t wave_max_u64(uint64_t k) {
+  k = dpp_max<0xB1>(k);   // quad_perm(1,0,3,2)
+  k = dpp_max<0x4E>(k);   // quad_perm(2,3,0,1)
+  k = dpp_max<0x141>(k);  // row_half_mirror
+  k = dpp_max<0x140>(k);  // row_mirror
+  uint64_t r = 0;
+#pragma unroll
+  for (int row = 0; row < 4; ++row) {
+    const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)k, row * 16);
+    const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(k >> 32), row * 16);
+    const uint64_t o = ((uint64_t)hi << 32) | lo;
+    r = o > r ? o : r;
+  }
+  return r;
+}
+
+// Pivot candidates as keys: larger key = larger |value|, then lower row.
+// Key 0 means "no candidate" (row not eligible, or NaN, which the
+// reference's strict '>' scan never selects).  f32: (|v| bits, ~row) in one
+// 64-bit key.  f64: the value key (|v| bits + 1) and the row key (~row) are
+// reduced one after the other.
+template <typename T>
+struct PivKey;
+template <>
+struct PivKey<float> {
+  uint64_t k;
+  __device__ void clear() { k = 0; }
+  __device__ void offer(float v, int row) {
+    const float a = fabsf(v);
+    if (!(a == a)) return;
+    const uint64_t c = ((uint64_t)__float_as_uint(a) << 32) | (uint32_t)~(uint32_t)row;
+    if (c > k) k = c;
+  }
+  __device__ void wave_reduce() { k = wave_max_u64(k); }
+  __device__ void merge(const PivKey &o) {
+    if (o.k > k) k = o.k;
+  }
+  __device__ bool valid() const { return k != 0; }
+  __device__ int row() const { return (int)~(uint32_t)k; }
+  static constexpr int NW = 2;  // 32-bit payload words
+  __device__ uint32_t word(int i) const { return i == 0 ? (uint32_t)k : (uint32_t)(k >> 32); }
+  __device__ void set_word(int i, uint32_t w) {
+    k = i == 0 ? ((k & 0xffffffff00000000ull) | w) : ((k & 0xffffffffull) | ((uint64_t)w << 32));
+  }
+};
+template <>
+struct PivKey<double> {
+  uint64_t v;  // |value| bits + 1
+  uint32_t r;  // ~row
+  __device__ void clear() {
+    v = 0;
+    r = 0;
+  }
+  __device__ void offer(double x, int row) {
+    const double a = fabs(x);
+    if (!(a == a)) return;
+    const uint64_t c = (uint64_t)__double_as_longlong(a) + 1ull;
+    const uint32_t cr = ~(uint32_t)row;
+    if (c > v || (c == v && cr > r)) {
+      v = c;
+      r = cr;
+    }
+  }
+  __device__ void wave_reduce() {
+    const uint64_t vm = wave_max_u64(v);
+    r = (uint32_t)wave_max_u64(v == vm ? (uint64_t)r : 0ull);
+    v = vm;
+  }
+  __device__ void merge(const PivKey &o) {
+    if (o.v > v || (o.v == v && o.r > r)) {
+      v = o.v;
+      r = o.r;
+    }
+  }
+  __device__ bool valid() const { return v != 0; }
+  __device__ int row() const { return (int)~r; }
+  static constexpr int NW = 3;
+  __device__ uint32_t word(int i) const {
+    return i == 0 ? (uint32_t)v : (i == 1 ? (uint32_t)(v >> 32) : r);
+  }
+  __device__ void set_word(int i, uint32_t w) {
+    if (i == 0) v = (v & 0xffffffff00000000ull) | w;
+    else if (i == 1) v = (v & 0xffffffffull) | ((uint64_t)w << 32);
+    else r = w;
+  }
+};
+
+__device__ inline uint64_t tagged(unsigned tag, uint32_t payload) {
+  return ((uint64_t)tag << 32) | payload;
+}
+
+template <typename T>
+__device__ inline uint32_t word_of(T v, int h) {
+  if constexpr (sizeof(T) == 4) {
+    return __float_as_uint(v);
+  } else {
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    return h ? (uint32_t)(u >> 32) : (uint32_t)u;
+  }
+}
+
+template <typename T>
+__device__ inline T value_of(const uint32_t *w) {
+  if constexpr (sizeof(T) == 4) {
+    return __uint_as_float(w[0]);
+  } else {
+    const unsigned long long u = ((unsigned long long)w[1] << 32) | w[0];
+    return __longlong_as_double((long long)u);
+  }
+}
+
 template <typename T, int WMAX, int RPT>
 __global__ __launch_bounds__(PANEL_NT) void panel_gepp_kernel(PanelP<T> p) {
   constexpr int RPW = PANEL_NT * RPT;
+  constexpr int NWD = Words<T>::N;
+  // record words: [0..KW) key, [ROW0..) row
+  constexpr int KW = PivKey<T>::NW;
+  constexpr int ROW0 = 4;
+  constexpr int REC = ROW0 + WMAX * NWD;
+  constexpr int KREC = WMAX * NWD;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int b = blockIdx.x;
   const int P = p.nwg;
   const int rend = p.r0 + p.m;
   const int w = p.w;
 
-  __shared__ T prow[WMAX], drow[WMAX];
-  __shared__ T wv[PANEL_NT / 64];
-  __shared__ int wi[PANEL_NT / 64];
-  __shared__ int s_piv, s_wgw, s_abort;
+  __shared__ __align__(16) T wrow[2][PANEL_NW][WMAX];  // each wavefront's candidate row
+  __shared__ __align__(16) PivKey<T> wkey[2][PANEL_NW];
+  __shared__ __align__(16) T krow[2][WMAX];            // row k (if this workgroup owns it)
+  __shared__ __align__(16) uint32_t stage[2][2 * WMAX * NWD];  // winner row, row k (payload words)
+  __shared__ int s_piv[2];
+  __shared__ int s_abort;
 
   // Register rotation: at step j, slot 0 holds column j, slots 1..w-1-j the
-  // columns still to factor, and the finished columns rotate to the tail.
-  // Whole rows (all slots) move on a row interchange, so finished L entries
-  // travel with their rows.  After w steps column c sits in slot
+  // columns still to factor, and finished columns rotate to the tail (whole
+  // rows move on an interchange).  After w steps column c is in slot
   // (c - w) mod WMAX.
   T a[RPT][WMAX];
   int row[RPT];
@@ -90,138 +231,154 @@ __global__ __launch_bounds__(PANEL_NT) void panel_gepp_kernel(PanelP<T> p) {
   for (int j = 0; j < w; ++j) {
     const int k = p.c0 + j;
     if (k < p.kmax) {
-      // 1. candidate among this thread's rows (first row of max |a|)
-      T bv = T(-1);
-      int bi = 0x7fffffff;
+      const int par = step & 1;
+      // 1. wavefront candidate (first row of max |a|); park it and row k
+      PivKey<T> key;
+      key.clear();
 #pragma unroll
       for (int q = 0; q < RPT; ++q)
-        if (row[q] < rend && row[q] >= k) better(bv, bi, (T)fabs(a[q][0]), row[q]);
-      // 2. wavefront reduction
+        if (row[q] < rend && row[q] >= k) key.offer(a[q][0], row[q]);
+      key.wave_reduce();
+      if (lane == 0) wkey[par][wave] = key;
+      const int wbest = key.valid() ? key.row() : -1;
 #pragma unroll
-      for (int off = 32; off >= 1; off >>= 1) {
-        T ov = __shfl_xor(bv, off);
-        int oi = __shfl_xor(bi, off);
-        better(bv, bi, ov, oi);
-      }
-      if (lane == 0) {
-        wv[wave] = bv;
-        wi[wave] = bi;
+      for (int q = 0; q < RPT; ++q) {
+        if (row[q] == wbest)
+#pragma unroll
+          for (int c = 0; c < WMAX; ++c) wrow[par][wave][c] = a[q][c];
+        if (row[q] == k)
+#pragma unroll
+          for (int c = 0; c < WMAX; ++c) krow[par][c] = a[q][c];
       }
       __syncthreads();
-      if (t == 0) {
+      // 2. workgroup candidate
+      PivKey<T> gkey = wkey[par][0];
+      int gw = 0;
 #pragma unroll
-        for (int w2 = 1; w2 < PANEL_NT / 64; ++w2) better(wv[0], wi[0], wv[w2], wi[w2]);
-        if (wi[0] == 0x7fffffff) wi[0] = k;  // no comparable value (all NaN)
+      for (int x = 1; x < PANEL_NW; ++x) {
+        const PivKey<T> o = wkey[par][x];
+        PivKey<T> m = gkey;
+        m.merge(o);
+        if (m.row() != gkey.row() || (m.valid() && !gkey.valid())) gw = x;
+        gkey = m;
       }
-      __syncthreads();
-      const int wg_idx = wi[0];
-      const int par = step & 1;
-
+      int piv;
+      const T *win_row = nullptr;
+      const T *k_row = nullptr;
       if (P == 1) {
-#pragma unroll
-        for (int q = 0; q < RPT; ++q) {
-          if (row[q] == wg_idx)
-#pragma unroll
-            for (int c = 0; c < WMAX; ++c) prow[c] = a[q][c];
-          if (row[q] == k)
-#pragma unroll
-            for (int c = 0; c < WMAX; ++c) drow[c] = a[q][c];
-        }
-        if (t == 0) s_piv = wg_idx;
-        __syncthreads();
+        piv = gkey.valid() ? gkey.row() : k;
       } else {
-        // publish candidate, candidate row and (owner only) row k
-        if (t == 0) {
-          p.candv[par * P + b] = wv[0];
-          p.candi[par * P + b] = wg_idx;
-        }
-#pragma unroll
-        for (int q = 0; q < RPT; ++q) {
-          if (row[q] == wg_idx) {
-            T *dst = p.rowbuf + ((long)par * P + b) * WMAX;
-#pragma unroll
-            for (int c = 0; c < WMAX; ++c) dst[c] = a[q][c];
+        // 3. exchange between workgroups (wavefront 0 of every workgroup):
+        //    publish this workgroup's key and row (and row k by its owner),
+        //    poll the P keys, then fetch the winner's row and row k
+        const unsigned tag = p.epoch * 64u + (unsigned)step + 1u;
+        if (wave == 0) {
+          uint64_t *mine = p.rec + ((long)par * P + b) * REC;
+          if (lane < KW)
+            __hip_atomic_store(mine + lane, tagged(tag, gkey.word(lane)), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+          for (int x = lane; x < WMAX * NWD; x += 64) {
+            const T v = wrow[par][gw][x / NWD];
+            __hip_atomic_store(mine + ROW0 + x, tagged(tag, word_of<T>(v, x % NWD)),
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           }
-          if (row[q] == k) {
-            T *dst = p.diagbuf + (long)par * WMAX;
-#pragma unroll
-            for (int c = 0; c < WMAX; ++c) dst[c] = a[q][c];
-          }
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        __syncthreads();
-        if (t == 0) {
-          __hip_atomic_fetch_add(p.bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          const unsigned target = (unsigned)P * (unsigned)(step + 1);
-          long spins = 0;
-          while (__hip_atomic_load(p.bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-            if (++spins > SPIN_LIMIT) {
-              s_abort = 1;
-              __hip_atomic_store(p.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-              break;
+          const bool own_k = (k - p.r0) / RPW == b;
+          if (own_k)
+            for (int x = lane; x < KREC; x += 64) {
+              const T v = krow[par][x / NWD];
+              __hip_atomic_store(p.krec + (long)par * KREC + x, tagged(tag, word_of<T>(v, x % NWD)),
+                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
-            __builtin_amdgcn_s_sleep(1);
+          bool timeout = false;
+          // phase 1: lane x < P reads workgroup x's key words
+          PivKey<T> ck;
+          ck.clear();
+          if (lane < P) {
+            const uint64_t *rx = p.rec + ((long)par * P + lane) * REC;
+            uint64_t wd[KW];
+#pragma unroll
+            for (int h = 0; h < KW; ++h)
+              wd[h] = __hip_atomic_load(rx + h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+            for (int h = 0; h < KW; ++h) {
+              long spins = 0;
+              while ((wd[h] >> 32) != tag) {
+                if (++spins > SPIN_LIMIT) {
+                  timeout = true;
+                  break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+                wd[h] = __hip_atomic_load(rx + h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              }
+              ck.set_word(h, (uint32_t)wd[h]);
+            }
           }
+          PivKey<T> best = ck;
+          best.wave_reduce();
+          const int gpiv = best.valid() ? best.row() : k;
+          // the workgroup that owns the winning row (lowest lane with the key)
+          const bool is_win = lane < P && ck.valid() && best.valid() && ck.row() == gpiv;
+          const unsigned long long wm = __ballot(is_win);
+          const int cw = wm ? __builtin_ctzll(wm) : 0;
+          // phase 2: winner row and row k
+          const uint64_t *wrec = p.rec + ((long)par * P + cw) * REC + ROW0;
+#pragma unroll
+          for (int u = 0; u < (2 * WMAX * NWD + 63) / 64; ++u) {
+            const int x = lane + 64 * u;
+            if (x < 2 * WMAX * NWD) {
+              const bool is_k = x >= WMAX * NWD;
+              const int xx = is_k ? x - WMAX * NWD : x;
+              const uint64_t *addr = is_k ? p.krec + (long)par * KREC + xx : wrec + xx;
+              uint64_t wd = __hip_atomic_load(addr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              long spins = 0;
+              while ((wd >> 32) != tag) {
+                if (++spins > SPIN_LIMIT) {
+                  timeout = true;
+                  break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+                wd = __hip_atomic_load(addr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              }
+              stage[par][x] = (uint32_t)wd;
+            }
+          }
+          if (__any(timeout) && lane == 0) {
+            s_abort = 1;
+            __hip_atomic_store(p.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          }
+          if (lane == 0) s_piv[par] = gpiv;
         }
         __syncthreads();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         if (s_abort) return;
-        if (t < 64) {
-          // every workgroup reduces the same P candidates
-          T cv = T(-1);
-          int ci = 0x7fffffff, cw = 0;
-          for (int x = lane; x < P; x += 64) {
-            T v = p.candv[par * P + x];
-            int i = p.candi[par * P + x];
-            if (v > cv || (v == cv && i < ci)) {
-              cv = v;
-              ci = i;
-              cw = x;
-            }
-          }
-#pragma unroll
-          for (int off = 32; off >= 1; off >>= 1) {
-            T ov = __shfl_xor(cv, off);
-            int oi = __shfl_xor(ci, off);
-            int ow = __shfl_xor(cw, off);
-            if (ov > cv || (ov == cv && oi < ci)) {
-              cv = ov;
-              ci = oi;
-              cw = ow;
-            }
-          }
-          if (lane == 0) {
-            s_piv = ci;
-            s_wgw = cw;
-          }
-        }
-        __syncthreads();
-        if (t < WMAX) {
-          const T dv = p.diagbuf[(long)par * WMAX + t];
-          drow[t] = dv;
-          prow[t] = (s_piv == k) ? dv : p.rowbuf[((long)par * P + s_wgw) * WMAX + t];
-        }
-        __syncthreads();
+        piv = s_piv[par];
+        win_row = reinterpret_cast<const T *>(&stage[par][0]);
+        k_row = reinterpret_cast<const T *>(&stage[par][WMAX * NWD]);
       }
-
-      // 3. swap rows k and piv (all slots)
-      const int piv = s_piv;
+      // 4. pivot row and row k for everyone
+      const T *pr;
+      const T *dr;
+      if (P == 1) {
+        pr = (piv == k) ? krow[par] : wrow[par][gw];
+        dr = krow[par];
+      } else {
+        pr = (piv == k) ? k_row : win_row;
+        dr = k_row;
+      }
       if (b == 0 && t == 0) p.ipiv[k] = piv;
       if (piv != k) {
 #pragma unroll
         for (int q = 0; q < RPT; ++q) {
           if (row[q] == k) {
 #pragma unroll
-            for (int c = 0; c < WMAX; ++c) a[q][c] = prow[c];
+            for (int c = 0; c < WMAX; ++c) a[q][c] = pr[c];
           } else if (row[q] == piv) {
 #pragma unroll
-            for (int c = 0; c < WMAX; ++c) a[q][c] = drow[c];
+            for (int c = 0; c < WMAX; ++c) a[q][c] = dr[c];
           }
         }
       }
-      // 4. scale and rank-1 update of the columns still to factor
-      //    (skipped for a near-zero pivot)
-      const T pv = prow[0];
+      // 5. scale and rank-1 update of the columns still to factor
+      const T pv = pr[0];
       if (!((T)fabs(pv) < p.eps)) {
         const T inv = T(1) / pv;
         const int rem = w - j;  // slots 1..rem-1 hold unfinished columns
@@ -231,15 +388,16 @@ __global__ __launch_bounds__(PANEL_NT) void panel_gepp_kernel(PanelP<T> p) {
             const T l = a[q][0] * inv;
             a[q][0] = l;
 #pragma unroll
-            for (int c = 1; c < WMAX; ++c)
-              if (c < rem) a[q][c] = fma(-prow[c], l, a[q][c]);
+            for (int c = 1; c < WMAX; ++c) {
+              const T lm = (c < rem) ? l : T(0);  // finished slots keep their L values
+              a[q][c] = fma(-pr[c], lm, a[q][c]);
+            }
           }
         }
       }
       ++step;
-      __syncthreads();  // prow/drow are rewritten next step
     }
-    // 5. rotate: the finished column moves to the tail
+    // 6. rotate: the finished column moves to the tail
 #pragma unroll
     for (int q = 0; q < RPT; ++q) {
       const T f = a[q][0];
@@ -248,7 +406,6 @@ __global__ __launch_bounds__(PANEL_NT) void panel_gepp_kernel(PanelP<T> p) {
       a[q][WMAX - 1] = f;
     }
   }
-  // after w rotations column c is in slot (c - w) mod WMAX
 #pragma unroll
   for (int q = 0; q < RPT; ++q) {
     if (row[q] < rend) {
@@ -343,26 +500,26 @@ __global__ __launch_bounds__(256) void rowswap_kernel(T *W, long ld, int ncols, 
     if (x < np) W[(long)dst[x] * ld + c] = v[x];
 }
 
+
 template <typename T, int WMAX, int RPT>
 static void launch_panel(hipStream_t s, PanelP<T> &p) {
   constexpr int RPW = PANEL_NT * RPT;
   p.nwg = ceil_div(p.m, RPW);
-  void *args[] = {&p};
-  if (p.nwg == 1) {
-    hipLaunchKernelGGL((panel_gepp_kernel<T, WMAX, RPT>), dim3(1), dim3(PANEL_NT), 0, s, p);
-  } else {
-    MG_HIP(hipLaunchCooperativeKernel((const void *)panel_gepp_kernel<T, WMAX, RPT>,
-                                      dim3(p.nwg), dim3(PANEL_NT), args, 0, s));
-  }
+  // At most 32 workgroups of 512 threads: all resident at once on an idle
+  // device (one per CU), which the exchange needs; a launch that could not
+  // be resident would end in the bounded poll's timeout, not in a hang.
+  hipLaunchKernelGGL((panel_gepp_kernel<T, WMAX, RPT>), dim3(p.nwg), dim3(PANEL_NT), 0, s, p);
   MG_HIP(hipGetLastError());
 }
 
 // Factor the leaf block W[c0:n, c0:c0+w] (w <= 32) and apply its row swaps
-// to all other columns of the n x n row-major matrix.
+// to all other columns of the n x n row-major matrix (and to perm).
 template <typename T>
 void lu_leaf(hipStream_t s, int n, T *W, long ld, int c0, int w, int *ipiv, int *perm, T eps) {
   constexpr int WMAX = 32;
+  constexpr int REC = 4 + WMAX * Words<T>::N;
   MG_CHECK(w <= WMAX, "lu_leaf: leaf too wide");
+  static unsigned epoch = 0;  // tags of the exchange words; never 0
   Workspace &ws = workspace(s);
   const int m = n - c0;
   PanelP<T> p{};
@@ -375,24 +532,19 @@ void lu_leaf(hipStream_t s, int n, T *W, long ld, int c0, int w, int *ipiv, int 
   p.kmax = n - 1;
   p.eps = eps;
   p.ipiv = ipiv;
-  const int maxwg = 1024;
-  p.candv = (T *)ws.get("panel.candv", 2 * maxwg * sizeof(T));
-  p.candi = (int *)ws.get("panel.candi", 2 * maxwg * sizeof(int));
-  p.rowbuf = (T *)ws.get("panel.rowbuf", 2L * maxwg * WMAX * sizeof(T));
-  p.diagbuf = (T *)ws.get("panel.diagbuf", 2 * WMAX * sizeof(T));
+  const int maxwg = 32;
+  p.rec = (uint64_t *)ws.get("panel.rec", 2L * maxwg * REC * 8);
+  p.krec = (uint64_t *)ws.get("panel.krec", 2L * WMAX * Words<T>::N * 8);
   p.err = (int *)ws.get("panel.err", sizeof(int));
-  p.bar = (unsigned *)ws.get("panel.bar", sizeof(unsigned));
-  // rows per workgroup: keep the number of workgroups in the rendezvous small
-  // double rows use twice the registers: at most 2 rows per thread
-  const int rpt_max = sizeof(T) == 4 ? 4 : 2;
-  const int rpt = std::min(rpt_max, (m > 256 * 8) ? 4 : (m > 256 * 2 ? 2 : 1));
-  const int nwg = ceil_div(m, 256 * rpt);
+  p.epoch = ++epoch;
+  if (p.epoch >= (1u << 25)) epoch = 0, p.epoch = ++epoch;
+  // rows per thread: an f32 row takes 32 registers, an f64 row 64; two wavefronts
+  // per SIMD leave 256 registers per thread
+  const int rpt = (sizeof(T) == 4 && m > PANEL_NT) ? 2 : 1;
+  const int nwg = ceil_div(m, PANEL_NT * rpt);
   MG_CHECK(nwg <= maxwg, "lu_leaf: matrix too tall");
-  if (nwg > 1) fill_u32(s, p.bar, 1, 0u);
-  if (rpt == 4) {
-    if constexpr (sizeof(T) == 4) launch_panel<T, WMAX, 4>(s, p);
-  } else if (rpt == 2) {
-    launch_panel<T, WMAX, 2>(s, p);
+  if (rpt == 2) {
+    if constexpr (sizeof(T) == 4) launch_panel<T, WMAX, 2>(s, p);
   } else {
     launch_panel<T, WMAX, 1>(s, p);
   }

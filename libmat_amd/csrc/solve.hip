@@ -27,19 +27,100 @@
 namespace mg {
 
 template <typename T>
+int tri_block_size();
+template <typename T>
+void tri_invert_blocks(hipStream_t s, bool lower, bool unit, int n, View<const T> Tm, T *D);
+template <typename T>
+void tri_solve_blocked(hipStream_t s, bool lower, int n, int nrhs, View<const T> Tm, const T *D,
+                       T *X, long ldx);
+
+// A pair of triangular factors with their inverted diagonal blocks, ready
+// for repeated solves (one preparation per factorization).
+template <typename T>
+struct TriPair {
+  int n = 0;
+  View<const T> lo{}, up{};
+  bool lo_unit = false, up_unit = false;
+  T *dlo = nullptr, *dup = nullptr;
+
+  void prepare(hipStream_t s, const char *tag) {
+    const int tb = tri_block_size<T>();
+    const size_t blocks = (size_t)((n + tb - 1) / tb) * tb * tb;
+    Workspace &ws = workspace(s);
+    dlo = (T *)ws.get(std::string(tag) + ".dlo", blocks * sizeof(T));
+    dup = (T *)ws.get(std::string(tag) + ".dup", blocks * sizeof(T));
+    tri_invert_blocks<T>(s, true, lo_unit, n, lo, dlo);
+    tri_invert_blocks<T>(s, false, up_unit, n, up, dup);
+  }
+  // X := (lo * up)^-1 X
+  void solve(hipStream_t s, int nrhs, T *X, long ldx) const {
+    if (nrhs <= 64) {
+      tri_solve_blocked<T>(s, true, n, nrhs, lo, dlo, X, ldx);
+      tri_solve_blocked<T>(s, false, n, nrhs, up, dup, X, ldx);
+    } else {
+      trsm_left<T>(s, true, lo_unit, n, nrhs, lo, View<T>{X, 1, ldx});
+      trsm_left<T>(s, false, up_unit, n, nrhs, up, View<T>{X, 1, ldx});
+    }
+  }
+};
+
+void rbt_matrix(hipStream_t, int, int, const double *, long, const double *, const double *, float *,
+                long);
+void rbt_vec_t(hipStream_t, int, int, int, const double *, const double *, long, float *, long);
+void rbt_vec_n(hipStream_t, int, int, int, const double *, const float *, long, double *, long, bool);
+void getrf_nopiv_rowmajor(hipStream_t, int, float *, long, float, Mul);
+
+// Butterfly diagonals exp(rho/10), rho ~ U[-1/2, 1/2], from a fixed seed
+// (splitmix64), laid out [r0 (h), r1 (h), s0, s1, t0, t1 (q each)].
+static std::vector<double> butterfly_diagonals(int n4, uint64_t seed) {
+  std::vector<double> v((size_t)2 * n4);
+  uint64_t x = seed;
+  for (auto &e : v) {
+    x += 0x9E3779B97F4A7C15ull;
+    uint64_t z = x;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    const double u = (double)(z >> 11) * 0x1.0p-53;  // [0, 1)
+    e = std::exp((u - 0.5) / 10.0);
+  }
+  return v;
+}
+
+template <typename T>
+TriPair<T> lu_pair(int n, const T *W, long ld) {
+  TriPair<T> p;
+  p.n = n;
+  p.lo = View<const T>{W, ld, 1};
+  p.up = p.lo;
+  p.lo_unit = true;
+  return p;
+}
+
+template <typename T>
+TriPair<T> chol_pair(int n, const T *L, long ld) {
+  TriPair<T> p;
+  p.n = n;
+  p.lo = View<const T>{L, 1, ld};
+  p.up = p.lo.t();
+  p.lo_unit = false;
+  return p;
+}
+
+template <typename T>
 void getrs_rowmajor(hipStream_t s, int n, int nrhs, const T *W, long ld, const int *perm,
                     const T *B, long ldb, T *X, long ldx) {
   gather_rows<T>(s, n, nrhs, B, ldb, perm, X, ldx);
-  View<const T> LU{W, ld, 1};
-  trsm_left<T>(s, true, true, n, nrhs, LU, View<T>{X, 1, ldx});
-  trsm_left<T>(s, false, false, n, nrhs, LU, View<T>{X, 1, ldx});
+  TriPair<T> tp = lu_pair<T>(n, W, ld);
+  if (nrhs <= 64) tp.prepare(s, "getrs");
+  tp.solve(s, nrhs, X, ldx);
 }
 
 template <typename T>
 void potrs_lower(hipStream_t s, int n, int nrhs, const T *L, long ld, T *X, long ldx) {
-  View<const T> Lv{L, 1, ld};
-  trsm_left<T>(s, true, false, n, nrhs, Lv, View<T>{X, 1, ldx});
-  trsm_left<T>(s, false, false, n, nrhs, Lv.t(), View<T>{X, 1, ldx});
+  TriPair<T> tp = chol_pair<T>(n, L, ld);
+  if (nrhs <= 64) tp.prepare(s, "potrs");
+  tp.solve(s, nrhs, X, ldx);
 }
 
 template void getrs_rowmajor<float>(hipStream_t, int, int, const float *, long, const int *,
@@ -51,28 +132,34 @@ template void potrs_lower<double>(hipStream_t, int, int, const double *, long, d
 
 // ------------------------------------------------------ fp64 residual ----
 // R = B - A X, A n x n column-major fp64, up to 16 right-hand sides per
-// pass.  Split-K: workgroup (rb, kc) owns 256 rows and one K chunk; each
-// thread keeps one row's 16 partial sums; X(k, :) is wave-uniform (scalar
-// loads).  Partials are reduced in a fixed order (deterministic).
+// pass.  Split-K: workgroup (rb, kc) owns 256 rows and one 256-deep K chunk;
+// the chunk of X is staged in LDS (broadcast reads), each thread streams
+// one row of A (coalesced across the workgroup) into 16 partial sums.
+// Partials are reduced in a fixed order (deterministic).
 constexpr int RES_NR = 16;
-constexpr int RES_KC = 512;
+constexpr int RES_KC = 256;
 
 __global__ __launch_bounds__(256) void residual_partial_kernel(int n, int nrhs, const double *A,
                                                                long lda, const double *X, long ldx,
                                                                double *part) {
+  __shared__ double Xs[RES_KC * RES_NR];  // [k][c]
   const int i = blockIdx.x * 256 + threadIdx.x;
   const int k0 = blockIdx.y * RES_KC;
-  const int k1 = min(n, k0 + RES_KC);
+  const int kn = min(RES_KC, n - k0);
+  for (int idx = threadIdx.x; idx < RES_KC * RES_NR; idx += 256) {
+    const int kk = idx % RES_KC, c = idx / RES_KC;
+    Xs[kk * RES_NR + c] = (kk < kn && c < nrhs) ? X[k0 + kk + (long)c * ldx] : 0.0;
+  }
+  __syncthreads();
   double acc[RES_NR];
 #pragma unroll
   for (int c = 0; c < RES_NR; ++c) acc[c] = 0.0;
   if (i < n) {
-    const double *a = A + i;
-    for (int k = k0; k < k1; ++k) {
-      const double v = a[(long)k * lda];
+    const double *a = A + i + (long)k0 * lda;
+    for (int kk = 0; kk < kn; ++kk) {
+      const double v = a[(long)kk * lda];
 #pragma unroll
-      for (int c = 0; c < RES_NR; ++c)
-        if (c < nrhs) acc[c] = fma(v, X[k + (long)c * ldx], acc[c]);
+      for (int c = 0; c < RES_NR; ++c) acc[c] = fma(v, Xs[kk * RES_NR + c], acc[c]);
     }
     double *pp = part + ((long)blockIdx.y * RES_NR) * n + i;
 #pragma unroll
@@ -122,14 +209,25 @@ __global__ void col_amax_kernel(int n, const double *X, long ldx, unsigned long 
   if (threadIdx.x == 0) atomicMax(out + c, (unsigned long long)__double_as_longlong(red[0]));
 }
 
-// out = max_i sum_j |A(i,j)|
-__global__ void row_abs_sum_kernel(int n, const double *A, long lda, unsigned long long *out) {
+// partial row sums of |A| over 256-column chunks: part[chunk][i]
+__global__ __launch_bounds__(256) void row_abs_partial_kernel(int n, const double *A, long lda,
+                                                              double *part) {
   const int i = blockIdx.x * 256 + threadIdx.x;
-  double sum = 0.0;
+  const int j0 = blockIdx.y * 256, j1 = min(n, j0 + 256);
+  if (i >= n) return;
+  double s = 0.0;
+  for (int j = j0; j < j1; ++j) s += fabs(A[i + (long)j * lda]);
+  part[(long)blockIdx.y * n + i] = s;
+}
+
+__global__ void row_abs_max_kernel(int n, int nchunks, const double *part,
+                                   unsigned long long *out) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  double s = 0.0;
   if (i < n)
-    for (int j = 0; j < n; ++j) sum += fabs(A[i + (long)j * lda]);
+    for (int q = 0; q < nchunks; ++q) s += part[(long)q * n + i];
   __shared__ double red[256];
-  red[threadIdx.x] = sum;
+  red[threadIdx.x] = s;
   __syncthreads();
   for (int off = 128; off > 0; off >>= 1) {
     if (threadIdx.x < off) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + off]);
@@ -145,10 +243,16 @@ static void col_amax(hipStream_t s, int n, int nrhs, const double *X, long ldx,
   MG_HIP(hipGetLastError());
 }
 
+// ||A||_inf (max row sum), fp64, synchronous.
 double norm_inf_f64(hipStream_t s, int n, const double *A, long lda) {
-  unsigned long long *d = (unsigned long long *)workspace(s).get("norm.A", 8);
+  const int nchunks = ceil_div(n, 256);
+  Workspace &ws = workspace(s);
+  unsigned long long *d = (unsigned long long *)ws.get("norm.A", 8);
+  double *part = (double *)ws.get("norm.part", (size_t)nchunks * n * 8);
   fill_u32(s, (unsigned *)d, 2, 0u);
-  hipLaunchKernelGGL(row_abs_sum_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, A, lda, d);
+  hipLaunchKernelGGL(row_abs_partial_kernel, dim3(ceil_div(n, 256), nchunks), dim3(256), 0, s, n, A,
+                     lda, part);
+  hipLaunchKernelGGL(row_abs_max_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, nchunks, part, d);
   MG_HIP(hipGetLastError());
   unsigned long long h = 0;
   MG_HIP(hipMemcpyAsync(&h, d, 8, hipMemcpyDeviceToHost, s));
@@ -251,36 +355,79 @@ int gesv_mixed(hipStream_t s, int n, int nrhs, const double *A, long lda, const 
                double *X, long ldx, const mat_gpu_ir_opts *opt, mat_gpu_ir_status *st) {
   const auto t0 = std::chrono::steady_clock::now();
   mat_gpu_ir_opts o{};
-  o.passes = 3;
+  o.passes = 6;
   o.max_iter = 30;
   o.tol = 1e-14;
   o.leaf = 32;
+  o.pivoting = 0;
   if (opt) o = *opt;
   MG_CHECK(o.passes == 0 || o.passes == 1 || o.passes == 3 || o.passes == 6,
            "gesv_mixed: passes must be 0 (fp32 MFMA), 1, 3 or 6");
+  MG_CHECK(o.pivoting == 0 || o.pivoting == 1, "gesv_mixed: pivoting must be 0 (partial) or 1 (rbt)");
+  const bool rbt = o.pivoting == 1;
+  const int n4 = rbt ? (n + 3) / 4 * 4 : n;
   Workspace &ws = workspace(s);
-  float *W = (float *)ws.get("gesvm.W", (size_t)n * n * 4);
-  int *ipiv = (int *)ws.get("gesvm.ipiv", (size_t)n * 4);
-  int *perm = (int *)ws.get("gesvm.perm", (size_t)n * 4);
+  float *W = (float *)ws.get("gesvm.W", (size_t)n4 * n4 * 4);
+  int *ipiv = (int *)ws.get("gesvm.ipiv", (size_t)n4 * 4);
+  int *perm = (int *)ws.get("gesvm.perm", (size_t)n4 * 4);
   double *R = (double *)ws.get("gesvm.R", (size_t)n * nrhs * 8);
-  float *D = (float *)ws.get("gesvm.D", (size_t)n * nrhs * 4);
-  float *D2 = (float *)ws.get("gesvm.D2", (size_t)n * nrhs * 4);
+  float *D = (float *)ws.get("gesvm.D", (size_t)n4 * nrhs * 4);
+  float *D2 = (float *)ws.get("gesvm.D2", (size_t)n4 * nrhs * 4);
+  double *ubf = nullptr, *vbf = nullptr;
+  const Mul mul = o.passes == 0 ? Mul::Native : (Mul)o.passes;
 
   const double normA = norm_inf_f64(s, n, A, lda);
-  // fp32 row-major working copy, then the low-precision LU
-  transpose_convert<double, float>(s, n, n, A, lda, W, n);
-  LuOpts lo;
-  lo.leaf = o.leaf;
-  lo.mul = o.passes == 0 ? Mul::Native : (Mul)o.passes;
-  getrf_rowmajor_perm<float>(s, n, W, n, ipiv, perm, 1e-6f, lo);
+  TriPair<float> tp;
+  if (rbt) {
+    ubf = (double *)ws.get("gesvm.ubf", (size_t)2 * n4 * 8);
+    vbf = (double *)ws.get("gesvm.vbf", (size_t)2 * n4 * 8);
+    const std::vector<double> hu = butterfly_diagonals(n4, 0x1234567ull + (uint64_t)n4);
+    const std::vector<double> hv = butterfly_diagonals(n4, 0x7654321ull + (uint64_t)n4);
+    MG_HIP(hipMemcpyAsync(ubf, hu.data(), hu.size() * 8, hipMemcpyHostToDevice, s));
+    MG_HIP(hipMemcpyAsync(vbf, hv.data(), hv.size() * 8, hipMemcpyHostToDevice, s));
+    // W (column-major) = U^T A V, i.e. row-major W holds M = A_r^T;  M = L U
+    // gives A_r = U^T L^T: lower non-unit then upper unit, both read through
+    // the column-major view of W.
+    rbt_matrix(s, n, n4, A, lda, ubf, vbf, W, n4);
+    getrf_nopiv_rowmajor(s, n4, W, n4, 1e-6f, mul);
+    tp.n = n4;
+    tp.lo = View<const float>{W, 1, n4};
+    tp.up = tp.lo;
+    tp.lo_unit = false;
+    tp.up_unit = true;
+    MG_HIP(hipStreamSynchronize(s));  // host vectors above must outlive the copies
+  } else {
+    transpose_convert<double, float>(s, n, n, A, lda, W, n);
+    LuOpts lo;
+    lo.leaf = o.leaf;
+    lo.mul = mul;
+    getrf_rowmajor_perm<float>(s, n, W, n, ipiv, perm, 1e-6f, lo);
+    tp = lu_pair<float>(n, W, n);
+  }
+  tp.prepare(s, "gesvm");
   MG_HIP(hipStreamSynchronize(s));
   const double t_factor = ms_since(t0);
 
+  // d = A^-1 r through the fp32 factors (r = B on the first pass)
+  auto correct = [&](const double *Rm, long ldr, bool assign) {
+    if (rbt) {
+      rbt_vec_t(s, n, n4, nrhs, ubf, Rm, ldr, D2, n4);
+      tp.solve(s, nrhs, D2, n4);
+      rbt_vec_n(s, n, n4, nrhs, vbf, D2, n4, X, ldx, assign);
+    } else {
+      convert2d<double, float>(s, n, nrhs, Rm, ldr, D, n);
+      gather_rows<float>(s, n, nrhs, D, n, perm, D2, n);
+      tp.solve(s, nrhs, D2, n);
+      if (assign) convert2d<float, double>(s, n, nrhs, D2, n, X, ldx);
+      else
+        hipLaunchKernelGGL(axpy_f32_to_f64_kernel, dim3(ceil_div((size_t)n * nrhs, 256)), dim3(256),
+                           0, s, n, nrhs, D2, n, X, ldx);
+    }
+    MG_HIP(hipGetLastError());
+  };
+
   BerrEval be(s, n, nrhs, normA, B, ldb);
-  // x0 = LU^-1 P b (fp32), r = b - A x
-  convert2d<double, float>(s, n, nrhs, B, ldb, D, n);
-  getrs_rowmajor<float>(s, n, nrhs, W, n, perm, D, n, D2, n);
-  convert2d<float, double>(s, n, nrhs, D2, n, X, ldx);
+  correct(B, ldb, true);
   int it = 0;
   double berr = INFINITY, prev = INFINITY, prev2 = INFINITY;
   bool ok = false;
@@ -296,12 +443,7 @@ int gesv_mixed(hipStream_t s, int n, int nrhs, const double *A, long lda, const 
     if (stalled) break;
     prev2 = prev;
     prev = berr;
-    // d = LU^-1 P r (fp32); x += d (fp64)
-    convert2d<double, float>(s, n, nrhs, R, n, D, n);
-    getrs_rowmajor<float>(s, n, nrhs, W, n, perm, D, n, D2, n);
-    hipLaunchKernelGGL(axpy_f32_to_f64_kernel, dim3(ceil_div((size_t)n * nrhs, 256)), dim3(256), 0,
-                       s, n, nrhs, D2, n, X, ldx);
-    MG_HIP(hipGetLastError());
+    correct(R, n, false);
     ++it;
   }
   if (st) {
@@ -351,6 +493,7 @@ int posv_mixed(hipStream_t s, int n, int nrhs, const double *A, long lda, const 
   o.max_iter = 30;
   o.tol = 1e-14;
   o.leaf = 32;
+  o.pivoting = 0;
   if (opt) o = *opt;
   MG_CHECK(o.passes == 0 || o.passes == 1 || o.passes == 3 || o.passes == 6 || o.passes == 8,
            "posv_mixed: passes must be 0 (fp32), 1/3/6 (bf16) or 8 (fp8)");
@@ -371,9 +514,11 @@ int posv_mixed(hipStream_t s, int n, int nrhs, const double *A, long lda, const 
   double berr = INFINITY;
   bool ok = false;
   BerrEval be(s, n, nrhs, normA, B, ldb);
+  TriPair<float> tp = chol_pair<float>(n, L, n);
   if (h == 0) {
+    tp.prepare(s, "posvm");
     convert2d<double, float>(s, n, nrhs, B, ldb, D, n);
-    potrs_lower<float>(s, n, nrhs, L, n, D, n);
+    tp.solve(s, nrhs, D, n);
     convert2d<float, double>(s, n, nrhs, D, n, X, ldx);
     double prev = INFINITY, prev2 = INFINITY;
     while (true) {
@@ -389,7 +534,7 @@ int posv_mixed(hipStream_t s, int n, int nrhs, const double *A, long lda, const 
       prev2 = prev;
       prev = berr;
       convert2d<double, float>(s, n, nrhs, R, n, D, n);
-      potrs_lower<float>(s, n, nrhs, L, n, D, n);
+      tp.solve(s, nrhs, D, n);
       hipLaunchKernelGGL(axpy_f32_to_f64_kernel, dim3(ceil_div((size_t)n * nrhs, 256)), dim3(256), 0,
                          s, n, nrhs, D, n, X, ldx);
       MG_HIP(hipGetLastError());

@@ -27,24 +27,34 @@ __device__ __forceinline__ void sfor(F &&f) {
 
 constexpr int TRSM_TB = 64;
 
+constexpr int TRSM_NT = 64;  // one wavefront per workgroup: more workgroups per solve
+
 template <typename T, bool LOWER, bool UNIT>
-__global__ __launch_bounds__(256) void trsm_leaf_kernel(int n, int nrhs, View<const T> Tm,
-                                                        View<T> B) {
+__global__ __launch_bounds__(TRSM_NT) void trsm_leaf_kernel(int n, int nrhs, View<const T> Tm,
+                                                            View<T> B) {
   constexpr int TB = TRSM_TB;
   __shared__ T S[TB * (TB + 1)];  // S[c*(TB+1) + r] = T(r, c)
-  for (int idx = threadIdx.x; idx < TB * TB; idx += 256) {
-    int r, c;
-    if (Tm.rs == 1) {
-      r = idx % TB;
-      c = idx / TB;
-    } else {
-      c = idx % TB;
-      r = idx / TB;
+  {
+    // issue every load before the LDS stores (one latency, not 64)
+    constexpr int PER = TB * TB / TRSM_NT;
+    T v[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int idx = threadIdx.x + u * TRSM_NT;
+      const int r = Tm.rs == 1 ? idx % TB : idx / TB;
+      const int c = Tm.rs == 1 ? idx / TB : idx % TB;
+      v[u] = (r < n && c < n) ? Tm(r, c) : T(0);
     }
-    S[c * (TB + 1) + r] = (r < n && c < n) ? Tm(r, c) : T(0);
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int idx = threadIdx.x + u * TRSM_NT;
+      const int r = Tm.rs == 1 ? idx % TB : idx / TB;
+      const int c = Tm.rs == 1 ? idx / TB : idx % TB;
+      S[c * (TB + 1) + r] = v[u];
+    }
   }
   __syncthreads();
-  const int j = blockIdx.x * 256 + threadIdx.x;
+  const int j = blockIdx.x * TRSM_NT + threadIdx.x;
   if (j >= nrhs) return;
   T x[TB];
   sfor<0, TB>([&](auto rc) {
@@ -96,30 +106,56 @@ static void gemm_sub(hipStream_t s, long M, long N, long K, View<const T> A, Vie
 }
 
 template <typename T>
+int tri_block_size();
+template <typename T>
+void tri_invert_blocks(hipStream_t s, bool lower, bool unit, int n, View<const T> Tm, T *D);
+
+// Blocked recursion whose leaves apply precomputed inverses of the TB x TB
+// diagonal blocks with one MFMA GEMM each.  In-place is safe: a leaf's
+// GEMM has at most one tile along the triangle's dimension, so every
+// workgroup reads exactly the block of B it overwrites, and reads it all
+// before its epilogue writes.
+template <typename T>
+static void trsm_inv_rec(hipStream_t s, bool lower, int n, int nrhs, View<const T> Tm, View<T> B,
+                         const T *D, int b0, int tb, Mul mul) {
+  if (n <= tb) {
+    View<const T> Dv{D + (long)b0 * tb * tb, tb, 1};
+    gemm<T>(s, n, nrhs, n, T(1), Dv, cview(B), T(0), B, false);
+    return;
+  }
+  int h = ((n / 2 + tb - 1) / tb) * tb;
+  if (h >= n) h = n - tb;
+  const int r = n - h;
+  if (lower) {
+    trsm_inv_rec<T>(s, true, h, nrhs, Tm, B, D, b0, tb, mul);
+    gemm_sub<T>(s, r, nrhs, h, Tm.sub(h, 0), cview(B), B.sub(h, 0), mul);
+    trsm_inv_rec<T>(s, true, r, nrhs, Tm.sub(h, h), B.sub(h, 0), D, b0 + h / tb, tb, mul);
+  } else {
+    trsm_inv_rec<T>(s, false, r, nrhs, Tm.sub(h, h), B.sub(h, 0), D, b0 + h / tb, tb, mul);
+    gemm_sub<T>(s, h, nrhs, r, Tm.sub(0, h), cview(B.sub(h, 0)), B, mul);
+    trsm_inv_rec<T>(s, false, h, nrhs, Tm, B, D, b0, tb, mul);
+  }
+}
+
+template <typename T>
 void trsm_left(hipStream_t s, bool lower, bool unit, int n, int nrhs, View<const T> Tm, View<T> B,
                Mul mul) {
   if (n <= 0 || nrhs <= 0) return;
-  if (n <= TRSM_TB) {
-    dim3 grid(ceil_div(nrhs, 256));
-    if (lower && unit) hipLaunchKernelGGL((trsm_leaf_kernel<T, true, true>), grid, dim3(256), 0, s, n, nrhs, Tm, B);
-    else if (lower) hipLaunchKernelGGL((trsm_leaf_kernel<T, true, false>), grid, dim3(256), 0, s, n, nrhs, Tm, B);
-    else if (unit) hipLaunchKernelGGL((trsm_leaf_kernel<T, false, true>), grid, dim3(256), 0, s, n, nrhs, Tm, B);
-    else hipLaunchKernelGGL((trsm_leaf_kernel<T, false, false>), grid, dim3(256), 0, s, n, nrhs, Tm, B);
+  if (n <= TRSM_TB && nrhs <= 4096) {
+    // small: one launch of the register-resident leaf
+    dim3 grid(ceil_div(nrhs, TRSM_NT)), blk(TRSM_NT);
+    if (lower && unit) hipLaunchKernelGGL((trsm_leaf_kernel<T, true, true>), grid, blk, 0, s, n, nrhs, Tm, B);
+    else if (lower) hipLaunchKernelGGL((trsm_leaf_kernel<T, true, false>), grid, blk, 0, s, n, nrhs, Tm, B);
+    else if (unit) hipLaunchKernelGGL((trsm_leaf_kernel<T, false, true>), grid, blk, 0, s, n, nrhs, Tm, B);
+    else hipLaunchKernelGGL((trsm_leaf_kernel<T, false, false>), grid, blk, 0, s, n, nrhs, Tm, B);
     MG_HIP(hipGetLastError());
     return;
   }
-  int h = ((n / 2 + TRSM_TB - 1) / TRSM_TB) * TRSM_TB;
-  if (h >= n) h = n - TRSM_TB;
-  const int r = n - h;
-  if (lower) {
-    trsm_left<T>(s, true, unit, h, nrhs, Tm, B, mul);
-    gemm_sub<T>(s, r, nrhs, h, Tm.sub(h, 0), cview(B), B.sub(h, 0), mul);
-    trsm_left<T>(s, true, unit, r, nrhs, Tm.sub(h, h), B.sub(h, 0), mul);
-  } else {
-    trsm_left<T>(s, false, unit, r, nrhs, Tm.sub(h, h), B.sub(h, 0), mul);
-    gemm_sub<T>(s, h, nrhs, r, Tm.sub(0, h), cview(B.sub(h, 0)), B, mul);
-    trsm_left<T>(s, false, unit, h, nrhs, Tm, B, mul);
-  }
+  const int tb = tri_block_size<T>();
+  const size_t dsz = (size_t)((n + tb - 1) / tb) * tb * tb * sizeof(T);
+  T *D = (T *)workspace(s).get("trsm.dinv", dsz);
+  tri_invert_blocks<T>(s, lower, unit, n, Tm, D);
+  trsm_inv_rec<T>(s, lower, n, nrhs, Tm, B, D, 0, tb, mul);
 }
 
 template void trsm_left<float>(hipStream_t, bool, bool, int, int, View<const float>, View<float>, Mul);
