@@ -116,6 +116,13 @@ bool mat_equals_tol(const Mat *a, const Mat *b, mat_elem_t epsilon);  /* mat.h:4
 const char *mat_gpu_last_error(void);
 /* Element type of this library build: 4 (float) or 8 (double). */
 int mat_gpu_elem_size(void);
+/* Synchronizes the stream; returns non-zero (message in
+ * mat_gpu_last_error()) if a multi-workgroup kernel queued on it since the
+ * last check gave up a bounded wait for another workgroup.  Synchronous
+ * entry points make this check themselves; asynchronous ones (getrs,
+ * potrs, gesv, posv on a stream) are checked by the next synchronous call
+ * on the same stream or by this one. */
+int mat_gpu_stream_check(void *stream);
 
 /* C = alpha * op(A) * op(B) + beta * C; trans_* = 0 (N) or 1 (T).
  * MFMA fp32 / fp64.  beta == 0 never reads C.  Asynchronous on stream. */

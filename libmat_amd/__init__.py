@@ -32,7 +32,7 @@ DROPIN_SYMBOLS = [
     "mat_equals_tol",
 ]
 EXT_SYMBOLS = [
-    "mat_gpu_last_error", "mat_gpu_elem_size",
+    "mat_gpu_last_error", "mat_gpu_elem_size", "mat_gpu_stream_check",
     "mat_gpu_gemm_f32", "mat_gpu_gemm_f64", "mat_gpu_gemm_lowp", "mat_gpu_getrf_f32", "mat_gpu_getrf_f64",
     "mat_gpu_getrs_f32", "mat_gpu_getrs_f64", "mat_gpu_potrf_f32", "mat_gpu_potrf_f64",
     "mat_gpu_potrs_f32", "mat_gpu_potrs_f64", "mat_gpu_gesv_f32", "mat_gpu_gesv_f64",
@@ -105,6 +105,7 @@ class GpuLib:
         s("mat_free_mat", None, [M])
         s("mat_gpu_last_error", C.c_char_p, [])
         s("mat_gpu_elem_size", i32, [])
+        s("mat_gpu_stream_check", i32, [vp])
         s("mat_gpu_gemm_f32", i32, [vp, i32, i32, sz, sz, sz, f32, vp, sz, vp, sz, f32, vp, sz])
         s("mat_gpu_gemm_f64", i32, [vp, i32, i32, sz, sz, sz, dbl, vp, sz, vp, sz, dbl, vp, sz])
         s("mat_gpu_gemm_lowp", i32, [vp, i32, sz, sz, sz, f32, vp, sz, vp, sz, vp, sz])

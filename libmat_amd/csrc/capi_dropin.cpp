@@ -65,7 +65,7 @@ void download(hipStream_t s, Mat *m, const T *d) {
   MG_HIP(hipMemcpyAsync(m->data, d, elems(m) * sizeof(T), hipMemcpyDeviceToHost, s));
 }
 
-void finish(hipStream_t s) { MG_HIP(hipStreamSynchronize(s)); }
+void finish(hipStream_t s) { mg::sync_and_check(s); }
 
 // PLU of host A into the device row-major working matrix; returns it and
 // fills the host ipiv.

@@ -81,7 +81,7 @@ int getrf_ext(void *stream, size_t n_, T *a, size_t lda, int32_t *perm, int leaf
     transpose<T>(s, n, n, W, n, a, (long)lda);
     std::vector<int> h(n);
     MG_HIP(hipMemcpyAsync(h.data(), ipiv, (size_t)n * 4, hipMemcpyDeviceToHost, s));
-    MG_HIP(hipStreamSynchronize(s));
+    sync_and_check(s);
     std::vector<size_t> p(n);
     const int sw = ipiv_to_perm(n, h.data(), p.data());
     if (swaps) *swaps = sw;
@@ -115,7 +115,7 @@ int potrf_ext(void *stream, size_t n, T *a, size_t lda) {
     zero_upper<T>(s, (int)n, a, (long)lda);
     int h = 0;
     MG_HIP(hipMemcpyAsync(&h, info, 4, hipMemcpyDeviceToHost, s));
-    MG_HIP(hipStreamSynchronize(s));
+    sync_and_check(s);
     return h != 0 ? -1 : 0;
   });
 }
@@ -169,7 +169,7 @@ int posv_ext(void *stream, size_t n_, size_t nrhs, const T *a, size_t lda, const
     potrf_lower<T>(s, n, L, n, info);
     int h = 0;
     MG_HIP(hipMemcpyAsync(&h, info, 4, hipMemcpyDeviceToHost, s));
-    MG_HIP(hipStreamSynchronize(s));
+    sync_and_check(s);
     if (h != 0) return -1;
     convert2d<T, T>(s, n, (int)nrhs, b, (long)ldb, x, (long)ldx);
     potrs_lower<T>(s, n, (int)nrhs, L, n, x, (long)ldx);
@@ -182,6 +182,13 @@ int posv_ext(void *stream, size_t n_, size_t nrhs, const T *a, size_t lda, const
 extern "C" {
 
 const char *mat_gpu_last_error(void) { return last_error(); }
+
+int mat_gpu_stream_check(void *s) {
+  return guarded([&] {
+    sync_and_check(as_stream(s));
+    return 0;
+  });
+}
 
 int mat_gpu_gemm_f32(void *s, int ta, int tb, size_t m, size_t n, size_t k, float alpha,
                      const float *a, size_t lda, const float *b, size_t ldb, float beta, float *c,

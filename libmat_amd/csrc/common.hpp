@@ -45,10 +45,19 @@ const char *last_error();
 // same stream (stream order makes reuse safe).  Freed at process exit.
 struct Workspace {
   std::unordered_map<std::string, std::pair<void *, size_t>> slots;
-  void *get(const std::string &name, size_t bytes);
+  // fresh (optional) is set when the slot was (re)allocated: its contents
+  // are undefined
+  void *get(const std::string &name, size_t bytes, bool *fresh = nullptr);
   ~Workspace();
 };
 Workspace &workspace(hipStream_t s);
+
+// One int per stream workspace, zero at creation, set to 1 by any
+// multi-workgroup kernel whose bounded wait for another workgroup expired.
+// sync_and_check() synchronizes the stream and, if the word is set, clears
+// it and throws.
+int *sync_error_word(hipStream_t s);
+void sync_and_check(hipStream_t s);
 
 // Library-owned default stream for the drop-in (synchronous) entry points.
 hipStream_t default_stream();

@@ -437,15 +437,17 @@ __global__ __launch_bounds__(256) void rowswap_kernel(T *W, long ld, int ncols, 
     // Lane i tracks window row k0+i (my_win = original row now there);
     // lane x also tracks the x-th distinct row below the window.
     const int lane = t;
+    // every index below is wave-uniform, so lane reads are v_readlane
     int my_win = k0 + lane, my_outr = -1, my_outc = -1, nout = 0;
+    const int my_piv = lane < w ? ipiv[k0 + lane] : 0;
     for (int kk = 0; kk < w; ++kk) {
       const int k = k0 + kk;
-      const int pr = ipiv[k];
+      const int pr = __builtin_amdgcn_readlane(my_piv, kk);
       if (pr == k) continue;
-      const int a = __shfl(my_win, kk);
+      const int a = __builtin_amdgcn_readlane(my_win, kk);
       int bsrc;
       if (pr < k0 + w) {
-        bsrc = __shfl(my_win, pr - k0);
+        bsrc = __builtin_amdgcn_readlane(my_win, pr - k0);
         if (lane == pr - k0) my_win = a;
       } else {
         const unsigned long long m = __ballot(my_outr == pr);
@@ -456,7 +458,7 @@ __global__ __launch_bounds__(256) void rowswap_kernel(T *W, long ld, int ncols, 
           slot = nout++;
           if (lane == slot) my_outr = my_outc = pr;
         }
-        bsrc = __shfl(my_outc, slot);
+        bsrc = __builtin_amdgcn_readlane(my_outc, slot);
         if (lane == slot) my_outc = a;
       }
       if (lane == kk) my_win = bsrc;
@@ -535,7 +537,7 @@ void lu_leaf(hipStream_t s, int n, T *W, long ld, int c0, int w, int *ipiv, int 
   const int maxwg = 32;
   p.rec = (uint64_t *)ws.get("panel.rec", 2L * maxwg * REC * 8);
   p.krec = (uint64_t *)ws.get("panel.krec", 2L * WMAX * Words<T>::N * 8);
-  p.err = (int *)ws.get("panel.err", sizeof(int));
+  p.err = sync_error_word(s);
   p.epoch = ++epoch;
   if (p.epoch >= (1u << 25)) epoch = 0, p.epoch = ++epoch;
   // rows per thread: an f32 row takes 32 registers, an f64 row 64; two wavefronts

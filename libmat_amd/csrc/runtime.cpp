@@ -11,10 +11,12 @@ static thread_local std::string g_last_error;
 void set_last_error(const std::string &m) { g_last_error = m; }
 const char *last_error() { return g_last_error.c_str(); }
 
-void *Workspace::get(const std::string &name, size_t bytes) {
+void *Workspace::get(const std::string &name, size_t bytes, bool *fresh) {
   if (bytes == 0) bytes = 1;
+  if (fresh) *fresh = false;
   auto it = slots.find(name);
   if (it != slots.end() && it->second.second >= bytes) return it->second.first;
+  if (fresh) *fresh = true;
   if (it != slots.end()) {
     // growing: the old buffer may still be used by queued work on this stream
     MG_HIP(hipDeviceSynchronize());
@@ -45,6 +47,32 @@ Workspace &workspace(hipStream_t s) {
   Workspace *w = new Workspace();
   g_ws[key] = w;
   return *w;
+}
+
+int *sync_error_word(hipStream_t s) {
+  Workspace &ws = workspace(s);
+  auto it = ws.slots.find("sync.err");
+  if (it != ws.slots.end()) return (int *)it->second.first;
+  int *p = (int *)ws.get("sync.err", sizeof(int));
+  MG_HIP(hipMemset(p, 0, sizeof(int)));  // once, at creation (never inside a capture)
+  return p;
+}
+
+void sync_and_check(hipStream_t s) {
+  Workspace &ws = workspace(s);
+  auto it = ws.slots.find("sync.err");
+  if (it == ws.slots.end()) {
+    MG_HIP(hipStreamSynchronize(s));
+    return;
+  }
+  int h = 0;
+  MG_HIP(hipMemcpyAsync(&h, it->second.first, sizeof(int), hipMemcpyDeviceToHost, s));
+  MG_HIP(hipStreamSynchronize(s));
+  if (h != 0) {
+    MG_HIP(hipMemsetAsync(it->second.first, 0, sizeof(int), s));
+    MG_HIP(hipStreamSynchronize(s));
+    throw Error{-5, "a multi-workgroup kernel timed out waiting for another workgroup"};
+  }
 }
 
 hipStream_t default_stream() {

@@ -256,7 +256,7 @@ double norm_inf_f64(hipStream_t s, int n, const double *A, long lda) {
   MG_HIP(hipGetLastError());
   unsigned long long h = 0;
   MG_HIP(hipMemcpyAsync(&h, d, 8, hipMemcpyDeviceToHost, s));
-  MG_HIP(hipStreamSynchronize(s));
+  sync_and_check(s);
   double v;
   memcpy(&v, &h, 8);
   return v;
@@ -317,7 +317,7 @@ struct BerrEval {
     col_amax(s, n, nrhs, R, ldr, dev);
     col_amax(s, n, nrhs, X, ldx, dev + nrhs);
     MG_HIP(hipMemcpyAsync(host.data(), dev, (size_t)3 * nrhs * 8, hipMemcpyDeviceToHost, s));
-    MG_HIP(hipStreamSynchronize(s));
+    sync_and_check(s);
     double worst = 0.0;
     for (int c = 0; c < nrhs; ++c) {
       double r, x, b;
@@ -395,7 +395,7 @@ int gesv_mixed(hipStream_t s, int n, int nrhs, const double *A, long lda, const 
     tp.up = tp.lo;
     tp.lo_unit = false;
     tp.up_unit = true;
-    MG_HIP(hipStreamSynchronize(s));  // host vectors above must outlive the copies
+    sync_and_check(s);  // host vectors above must outlive the copies
   } else {
     transpose_convert<double, float>(s, n, n, A, lda, W, n);
     LuOpts lo;
@@ -405,7 +405,7 @@ int gesv_mixed(hipStream_t s, int n, int nrhs, const double *A, long lda, const 
     tp = lu_pair<float>(n, W, n);
   }
   tp.prepare(s, "gesvm");
-  MG_HIP(hipStreamSynchronize(s));
+  sync_and_check(s);
   const double t_factor = ms_since(t0);
 
   // d = A^-1 r through the fp32 factors (r = B on the first pass)
@@ -464,7 +464,7 @@ int gesv_mixed(hipStream_t s, int n, int nrhs, const double *A, long lda, const 
     }
   }
   if (st) st->total_ms = ms_since(t0);
-  MG_HIP(hipStreamSynchronize(s));
+  sync_and_check(s);
   return 0;
 }
 
@@ -478,7 +478,7 @@ int posv_f64(hipStream_t s, int n, int nrhs, const double *A, long lda, const do
   potrf_lower<double>(s, n, L, n, info);
   int h = 0;
   MG_HIP(hipMemcpyAsync(&h, info, 4, hipMemcpyDeviceToHost, s));
-  MG_HIP(hipStreamSynchronize(s));
+  sync_and_check(s);
   if (h != 0) return -1;
   convert2d<double, double>(s, n, nrhs, B, ldb, X, ldx);
   potrs_lower<double>(s, n, nrhs, L, n, X, ldx);
@@ -508,7 +508,7 @@ int posv_mixed(hipStream_t s, int n, int nrhs, const double *A, long lda, const 
   potrf_lower<float>(s, n, L, n, info, o.passes == 0 ? Mul::Native : (Mul)o.passes);
   int h = 0;
   MG_HIP(hipMemcpyAsync(&h, info, 4, hipMemcpyDeviceToHost, s));
-  MG_HIP(hipStreamSynchronize(s));
+  sync_and_check(s);
   const double t_factor = ms_since(t0);
   int it = 0;
   double berr = INFINITY;
@@ -561,7 +561,7 @@ int posv_mixed(hipStream_t s, int n, int nrhs, const double *A, long lda, const 
     }
   }
   if (st) st->total_ms = ms_since(t0);
-  MG_HIP(hipStreamSynchronize(s));
+  sync_and_check(s);
   return rc;
 }
 

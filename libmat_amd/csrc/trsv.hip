@@ -5,12 +5,15 @@
 // MI355X design: the recursion used for wide right-hand sides degenerates
 // for 1..64 columns (one thread per column in the leaves).  Here the
 // triangle is cut into TB x TB diagonal blocks that are inverted once per
-// factorization (one workgroup per block, in LDS).  A solve is then nb+1
-// launches; launch k updates every remaining row block with the block
-// column just solved (X_r -= T(r,k) X_k, all row blocks in parallel, T
-// streamed once from HBM) and the workgroup of the next row block finishes
-// it with the inverted diagonal block (X_r = D_r^-1 X_r).  Each solve reads
-// the triangle once.
+// factorization (one workgroup per block, in LDS).  A solve is then one
+// launch of a chained kernel: each row block is owned by one workgroup,
+// which applies the already-solved block columns as they are published and
+// finishes with its inverted diagonal block.  Every tile of the triangle is
+// read once, and the only serial part is one tile product, one diagonal
+// product and one flag hand-off per block.
+#include <algorithm>
+#include <atomic>
+
 #include "ops.hpp"
 
 namespace mg {
@@ -54,119 +57,325 @@ __device__ inline void stage_tile(View<const T> V, int r0, int nr, int c0, int n
 }
 
 // Inverse of diagonal block b (rows/cols b*TB ..), written row-major to
-// D + b*TB*TB; a short last block is padded with the identity.  Right-looking
-// sweep with all threads: X = I; for each pivot j, scale row j (non-unit),
-// then X(i,:) -= T(i,j) X(j,:) for the rows below (lower) / above (upper).
+// D + b*TB*TB; a short last block is padded with the identity.
+// Divide and conquer: the 16 x 16 diagonal sub-blocks are inverted by
+// substitution, one thread per column with the column in registers; then
+// sizes double (16 -> 32 -> ... -> TB) with
+//   [A 0; C B]^-1 = [A^-1 0; -B^-1 C A^-1  B^-1]          (lower)
+//   [A C; 0 B]^-1 = [A^-1  -A^-1 C B^-1; 0 B^-1]          (upper)
+// two small LDS products per level, every pair of the level at once.
 template <typename T, bool LOWER, bool UNIT>
 __global__ __launch_bounds__(256) void tri_inv_kernel(int n, View<const T> Tm, T *D) {
   constexpr int TB = TriTB<T>::TB;
   constexpr int LD = TB + 1;
-  __shared__ T S[TB * LD];  // the block
-  __shared__ T X[TB * LD];  // its inverse
+  constexpr int BS = 16;
+  __shared__ T S[TB * LD];       // the block
+  __shared__ T X[TB * LD];       // its inverse
+  __shared__ T Tmp[TB * TB / 4]; // C A^-1 (lower) / C B^-1 (upper), all pairs of a level
   const int b = blockIdx.x, o = b * TB;
   const int nr = min(TB, n - o);
+  const int t = threadIdx.x;
   stage_tile<T, TB, 256>(Tm, o, nr, o, nr, S, LD);
-  for (int idx = threadIdx.x; idx < TB * TB; idx += 256) {
-    const int i = idx / TB, j = idx % TB;
-    X[i * LD + j] = (i == j) ? T(1) : T(0);
-    if (i == j && (i >= nr || UNIT)) S[i * LD + j] = T(1);
+  __syncthreads();
+  if (t < TB && (t >= nr || UNIT)) S[t * LD + t] = T(1);
+  for (int e = t; e < TB * TB; e += 256) X[(e / TB) * LD + e % TB] = T(0);
+  __syncthreads();
+  if (t < TB) {
+    const int q0 = (t / BS) * BS, col = t % BS;
+    T x[BS];
+    if (LOWER) {
+#pragma unroll
+      for (int r = 0; r < BS; ++r) {
+        T acc = (r == col) ? T(1) : T(0);
+#pragma unroll
+        for (int k = 0; k < r; ++k) acc = fma(-S[(q0 + r) * LD + q0 + k], x[k], acc);
+        x[r] = UNIT ? acc : acc / S[(q0 + r) * LD + q0 + r];
+      }
+    } else {
+#pragma unroll
+      for (int r = BS - 1; r >= 0; --r) {
+        T acc = (r == col) ? T(1) : T(0);
+#pragma unroll
+        for (int k = r + 1; k < BS; ++k) acc = fma(-S[(q0 + r) * LD + q0 + k], x[k], acc);
+        x[r] = UNIT ? acc : acc / S[(q0 + r) * LD + q0 + r];
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < BS; ++r) X[(q0 + r) * LD + q0 + col] = x[r];
   }
   __syncthreads();
-  for (int s = 0; s < TB; ++s) {
-    const int j = LOWER ? s : TB - 1 - s;
-    if (!UNIT) {
-      const T inv = T(1) / S[j * LD + j];
-      for (int c = threadIdx.x; c < TB; c += 256) X[j * LD + c] *= inv;
-      __syncthreads();
+  // each thread owns 4 x 4 micro-tiles of the off-diagonal blocks; the
+  // triangular operands are zero outside their triangle (X starts at zero),
+  // so the k ranges may overshoot to the micro-tile's bound
+  constexpr int MT = 4;
+  for (int sz = BS; sz < TB; sz *= 2) {
+    const int tiles = sz / MT;                    // micro-tiles per block side
+    const int nmt = (TB / (2 * sz)) * tiles * tiles;
+    for (int e = t; e < nmt; e += 256) {
+      const int p = e / (tiles * tiles), i0 = (e % (tiles * tiles)) / tiles * MT, j0 = e % tiles * MT;
+      const int q = 2 * sz * p;
+      T acc[MT][MT] = {};
+      if (LOWER) {  // Tmp = C A^-1, A^-1(k, j) = 0 for k < j
+        for (int k = j0; k < sz; ++k) {
+          T x[MT], y[MT];
+#pragma unroll
+          for (int u = 0; u < MT; ++u) x[u] = S[(q + sz + i0 + u) * LD + q + k];
+#pragma unroll
+          for (int u = 0; u < MT; ++u) y[u] = X[(q + k) * LD + q + j0 + u];
+#pragma unroll
+          for (int u = 0; u < MT; ++u)
+#pragma unroll
+            for (int v = 0; v < MT; ++v) acc[u][v] = fma(x[u], y[v], acc[u][v]);
+        }
+      } else {  // Tmp = C B^-1, B^-1(k, j) = 0 for k > j
+        for (int k = 0; k < j0 + MT; ++k) {
+          T x[MT], y[MT];
+#pragma unroll
+          for (int u = 0; u < MT; ++u) x[u] = S[(q + i0 + u) * LD + q + sz + k];
+#pragma unroll
+          for (int u = 0; u < MT; ++u) y[u] = X[(q + sz + k) * LD + q + sz + j0 + u];
+#pragma unroll
+          for (int u = 0; u < MT; ++u)
+#pragma unroll
+            for (int v = 0; v < MT; ++v) acc[u][v] = fma(x[u], y[v], acc[u][v]);
+        }
+      }
+      T *Tp = Tmp + p * sz * sz;
+#pragma unroll
+      for (int u = 0; u < MT; ++u)
+#pragma unroll
+        for (int v = 0; v < MT; ++v) Tp[(i0 + u) * sz + j0 + v] = acc[u][v];
     }
-    // rows i on the far side of j; row j of X is non-zero only on columns
-    // c <= j (lower) / c >= j (upper)
-    const int nrows = LOWER ? TB - 1 - j : j;
-    const int ncols = LOWER ? j + 1 : TB - j;
-    for (int idx = threadIdx.x; idx < nrows * ncols; idx += 256) {
-      const int i = LOWER ? j + 1 + idx / ncols : idx / ncols;
-      const int c = LOWER ? idx % ncols : j + idx % ncols;
-      X[i * LD + c] -= S[i * LD + j] * X[j * LD + c];
+    __syncthreads();
+    for (int e = t; e < nmt; e += 256) {
+      const int p = e / (tiles * tiles), i0 = (e % (tiles * tiles)) / tiles * MT, j0 = e % tiles * MT;
+      const int q = 2 * sz * p;
+      const T *Tp = Tmp + p * sz * sz;
+      T acc[MT][MT] = {};
+      // lower: -B^-1 Tmp, B^-1(i, k) = 0 for k > i;  upper: -A^-1 Tmp, A^-1(i, k) = 0 for k < i
+      const int k0 = LOWER ? 0 : i0, k1 = LOWER ? i0 + MT : sz;
+      const int xr = LOWER ? q + sz : q, xc = LOWER ? q + sz : q;
+      for (int k = k0; k < k1; ++k) {
+        T x[MT], y[MT];
+#pragma unroll
+        for (int u = 0; u < MT; ++u) x[u] = X[(xr + i0 + u) * LD + xc + k];
+#pragma unroll
+        for (int u = 0; u < MT; ++u) y[u] = Tp[k * sz + j0 + u];
+#pragma unroll
+        for (int u = 0; u < MT; ++u)
+#pragma unroll
+          for (int v = 0; v < MT; ++v) acc[u][v] = fma(x[u], y[v], acc[u][v]);
+      }
+      const int orow = LOWER ? q + sz : q, ocol = LOWER ? q : q + sz;
+#pragma unroll
+      for (int u = 0; u < MT; ++u)
+#pragma unroll
+        for (int v = 0; v < MT; ++v) X[(orow + i0 + u) * LD + ocol + j0 + v] = -acc[u][v];
     }
     __syncthreads();
   }
   T *Db = D + (long)b * TB * TB;
-  for (int idx = threadIdx.x; idx < TB * TB; idx += 256) Db[idx] = X[(idx / TB) * LD + idx % TB];
+  for (int e = t; e < TB * TB; e += 256) Db[e] = X[(e / TB) * LD + e % TB];
 }
 
-// One block step of the forward (LOWER) or backward solve.
-//   k: the block column solved by the previous launch (-1 / nb for the first)
-//   workgroup w handles row block r = k+1+w (lower) or k-1-w (upper)
-// The off-diagonal block and, for the finishing workgroup, the inverted
-// diagonal block are staged through LDS with coalesced loads.
+// Chained block solve, one launch.  Workgroups claim row blocks in solve
+// order from an atomic counter (forward: 0, 1, ...; backward: nb-1, ...).
+// The workgroup owning row block r streams T(r, k) for every solved block
+// k in order (tile k+1 is prefetched into registers while tile k is
+// applied), reads X_k, and finally applies the inverted diagonal block,
+// X_r = D_r^-1 (b_r - sum T(r,k) X_k).  Solved blocks are handed over as
+// 64-bit words (launch tag << 32 | 32 payload bits) written and polled with
+// agent-scope atomics, so a consumer never needs a fence or a flag: a word
+// with the current tag is the value.  A workgroup only waits for blocks
+// claimed before its own, by workgroups already running, so the chain
+// cannot deadlock whatever the residency.  Waits are bounded: on expiry the
+// workgroup sets the stream's error word and exits.
+constexpr int CHAIN_NC = 16;  // right-hand sides per launch
+constexpr long CHAIN_SPIN = 1L << 22;
+
+template <typename T>
+struct ChainNT;
+template <>
+struct ChainNT<float> {
+  static constexpr int NT = 512;
+};
+template <>
+struct ChainNT<double> {
+  static constexpr int NT = 256;
+};
+
+// Tile (nr x nc at (r0, c0) of V, zero padded) into per-thread registers in
+// the coalesced order of V's storage: thread t holds the elements at inner
+// index t % TB (contiguous in memory) and outer index t / TB + u * NT / TB.
+template <typename T, int TB, int NT>
+__device__ __forceinline__ void chain_load(View<const T> V, int r0, int nr, int c0, int nc,
+                                           T (&v)[TB * TB / NT]) {
+  constexpr int NG = NT / TB;
+  const bool rm = V.cs == 1;
+  const int in = threadIdx.x % TB, out0 = threadIdx.x / TB;
+  const int nin = rm ? nc : nr, nout = rm ? nr : nc;
+  const long ostride = rm ? V.rs : V.cs;
+  const T *base = rm ? &V(r0 + out0, c0 + in) : &V(r0 + in, c0 + out0);
+#pragma unroll
+  for (int u = 0; u < TB * TB / NT; ++u)
+    v[u] = (in < nin && out0 + u * NG < nout) ? base[u * NG * ostride] : T(0);
+}
+
+// Registers from chain_load into S[i*(TB+1) + j]; rm as in chain_load.
+template <typename T, int TB, int NT>
+__device__ __forceinline__ void chain_store(bool rm, const T (&v)[TB * TB / NT], T *S) {
+  constexpr int NG = NT / TB;
+  const int in = threadIdx.x % TB, out0 = threadIdx.x / TB;
+  T *base = rm ? S + out0 * (TB + 1) + in : S + in * (TB + 1) + out0;
+  const int ostep = rm ? NG * (TB + 1) : NG;
+#pragma unroll
+  for (int u = 0; u < TB * TB / NT; ++u) base[u * ostep] = v[u];
+}
+
+// 32-bit payload words of a value (one for float, two for double)
+template <typename T>
+struct ChainWords {
+  static constexpr int N = sizeof(T) / 4;
+};
+
+template <typename T>
+__device__ __forceinline__ uint32_t chain_word(T v, int h) {
+  if constexpr (sizeof(T) == 4) {
+    return __float_as_uint(v);
+  } else {
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    return h ? (uint32_t)(u >> 32) : (uint32_t)u;
+  }
+}
+
 template <typename T, bool LOWER>
-__global__ __launch_bounds__(256) void tri_step_kernel(int n, int nrhs, int k, View<const T> Tm,
-                                                       const T *D, T *X, long ldx) {
-  constexpr int TB = TriTB<T>::TB;
-  constexpr int NC = 16;  // right-hand sides per pass
-  constexpr int RPT = TB * NC / 256;
-  constexpr int LDT = TB + 1;
-  __shared__ T Ts[TB * LDT];  // Ts[i*LDT + j]
-  __shared__ T Xk[TB * NC];   // solved block column, [j][c]
-  __shared__ T Y[TB * NC];    // this row block, [i][c]
+__global__ __launch_bounds__(ChainNT<T>::NT) void tri_chain_kernel(int n, int nrhs, View<const T> Tm,
+                                                                    const T *D, T *X, long ldx,
+                                                                    unsigned *claim, uint64_t *pub,
+                                                                    unsigned tag, int *err) {
+  constexpr int TB = TriTB<T>::TB, NT = ChainNT<T>::NT, NC = CHAIN_NC;
+  constexpr int NG = NT / TB;   // column groups
+  constexpr int CPT = NC / NG;  // right-hand sides per thread
+  constexpr int PER = TB * TB / NT;
+  constexpr int LDS = TB + 1;
+  constexpr int NW = ChainWords<T>::N;
+  constexpr int BW = TB * NC * NW;  // published words per block, [c][j][h]
+  __shared__ T S[2][TB * LDS];
+  __shared__ T Xs[TB * NC];  // [row j][rhs c]
+  __shared__ int s_rb, s_fail;
   const int nb = (n + TB - 1) / TB;
-  const int w = blockIdx.x;
-  const int r = LOWER ? k + 1 + w : k - 1 - w;
-  const bool finish = LOWER ? (r == k + 1) : (r == k - 1);
-  const bool has_k = LOWER ? (k >= 0) : (k < nb);
-  const int ro = r * TB, ko = k * TB;
-  const int rrows = min(TB, n - ro);
-  const int krows = has_k ? min(TB, n - ko) : 0;
-  const int t = threadIdx.x;
-  const int i = t % TB;           // row inside the block
-  const int c0 = t / TB;          // first rhs column of this thread
-  constexpr int CSTEP = 256 / TB;  // columns interleave
-  if (has_k) stage_tile<T, TB, 256>(Tm, ro, rrows, ko, krows, Ts, LDT);
-  for (int cb = 0; cb < nrhs; cb += NC) {
-    const int nc = min(NC, nrhs - cb);
-    if (has_k)
-      for (int idx = t; idx < TB * NC; idx += 256) {
-        const int j = idx % TB, c = idx / TB;
-        Xk[j * NC + c] = (j < krows && c < nc) ? X[(ko + j) + (long)(cb + c) * ldx] : T(0);
-      }
+  const int t = threadIdx.x, i = t % TB, g = t / TB;
+  const bool rm = Tm.cs == 1;
+  for (;;) {
+    if (t == 0) {
+      s_rb = (int)atomicAdd(claim, 1u);
+      s_fail = 0;
+    }
     __syncthreads();
-    T acc[RPT];
-#pragma unroll
-    for (int q = 0; q < RPT; ++q) {
-      const int c = c0 + q * CSTEP;
-      acc[q] = (i < rrows && c < nc) ? X[(ro + i) + (long)(cb + c) * ldx] : T(0);
+    const int rb = s_rb;
+    if (rb >= nb) return;
+    const int r = LOWER ? rb : nb - 1 - rb;
+    const int ro = r * TB, rrows = min(TB, n - ro);
+    const int cnt = rb;  // blocks solved before this one
+    View<const T> Dv{D + (long)r * TB * TB, TB, 1};
+    T v[PER];
+    if (cnt > 0) {
+      const int ko = (LOWER ? 0 : nb - 1) * TB;
+      chain_load<T, TB, NT>(Tm, ro, rrows, ko, min(TB, n - ko), v);
+    } else {
+      chain_load<T, TB, NT>(Dv, 0, TB, 0, TB, v);
     }
-    if (has_k) {
-      for (int j = 0; j < krows; ++j) {
-        const T a = Ts[i * LDT + j];
+    T acc[CPT];
 #pragma unroll
-        for (int q = 0; q < RPT; ++q) acc[q] -= a * Xk[j * NC + c0 + q * CSTEP];
+    for (int q = 0; q < CPT; ++q) {
+      const int c = g * CPT + q;
+      acc[q] = (i < rrows && c < nrhs) ? X[(ro + i) + (long)c * ldx] : T(0);
+    }
+    for (int idx = 0; idx < cnt; ++idx) {
+      const int k = LOWER ? idx : nb - 1 - idx;
+      T *Sb = S[idx & 1];
+      chain_store<T, TB, NT>(rm, v, Sb);
+      if (idx + 1 < cnt) {
+        const int k2 = LOWER ? idx + 1 : nb - 2 - idx;
+        chain_load<T, TB, NT>(Tm, ro, rrows, k2 * TB, min(TB, n - k2 * TB), v);
+      } else {
+        chain_load<T, TB, NT>(Dv, 0, TB, 0, TB, v);
       }
-    }
-    if (finish) {
+      // X_k from its published words
+      const uint64_t *src = pub + (long)k * BW;
+      bool timeout = false;
 #pragma unroll
-      for (int q = 0; q < RPT; ++q) Y[i * NC + c0 + q * CSTEP] = acc[q];
-      __syncthreads();
-      stage_tile<T, TB, 256>(View<const T>{D + (long)r * TB * TB, TB, 1}, 0, TB, 0, TB, Ts, LDT);
-      __syncthreads();
+      for (int u = 0; u < TB * NC / NT; ++u) {
+        const int e = t + u * NT;  // = c * TB + j
+        uint32_t w[NW];
 #pragma unroll
-      for (int q = 0; q < RPT; ++q) acc[q] = T(0);
+        for (int h = 0; h < NW; ++h) {
+          uint64_t wd = __hip_atomic_load(src + (long)e * NW + h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          long spins = 0;
+          while ((unsigned)(wd >> 32) != tag) {
+            if (++spins > CHAIN_SPIN) {
+              timeout = true;
+              break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+            wd = __hip_atomic_load(src + (long)e * NW + h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          }
+          w[h] = (uint32_t)wd;
+        }
+        T val;
+        if constexpr (NW == 1) {
+          val = __uint_as_float(w[0]);
+        } else {
+          val = __longlong_as_double((long long)(((unsigned long long)w[1] << 32) | w[0]));
+        }
+        Xs[(e % TB) * NC + e / TB] = val;
+      }
+      if (timeout) s_fail = 1;
+      __syncthreads();
+      if (s_fail) {
+        if (t == 0) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return;
+      }
+#pragma unroll 8
       for (int j = 0; j < TB; ++j) {
-        const T a = Ts[i * LDT + j];
+        const T a = Sb[i * LDS + j];
 #pragma unroll
-        for (int q = 0; q < RPT; ++q) acc[q] += a * Y[j * NC + c0 + q * CSTEP];
+        for (int q = 0; q < CPT; ++q) acc[q] = fma(-a, Xs[j * NC + g * CPT + q], acc[q]);
       }
+      __syncthreads();
+    }
+    // X_r = D_r^-1 acc
+    T *Sd = S[cnt & 1];
+    chain_store<T, TB, NT>(true, v, Sd);
+#pragma unroll
+    for (int q = 0; q < CPT; ++q) Xs[i * NC + g * CPT + q] = acc[q];
+    __syncthreads();
+    T out[CPT];
+#pragma unroll
+    for (int q = 0; q < CPT; ++q) out[q] = T(0);
+#pragma unroll 8
+    for (int j = 0; j < TB; ++j) {
+      const T a = Sd[i * LDS + j];
+#pragma unroll
+      for (int q = 0; q < CPT; ++q) out[q] = fma(a, Xs[j * NC + g * CPT + q], out[q]);
+    }
+    // publish first (the chain waits on it), then the result itself
+    uint64_t *dst = pub + (long)r * BW;
+#pragma unroll
+    for (int q = 0; q < CPT; ++q) {
+      const int c = g * CPT + q;
+#pragma unroll
+      for (int h = 0; h < NW; ++h)
+        __hip_atomic_store(dst + ((long)c * TB + i) * NW + h,
+                           ((uint64_t)tag << 32) | chain_word<T>(out[q], h), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
     }
 #pragma unroll
-    for (int q = 0; q < RPT; ++q) {
-      const int c = c0 + q * CSTEP;
-      if (i < rrows && c < nc) X[(ro + i) + (long)(cb + c) * ldx] = acc[q];
+    for (int q = 0; q < CPT; ++q) {
+      const int c = g * CPT + q;
+      if (i < rrows && c < nrhs) X[(ro + i) + (long)c * ldx] = out[q];
     }
-    __syncthreads();
-    if (finish && has_k && cb + NC < nrhs) {
-      // Ts now holds D_r^-1; restore T(r, k) for the next column pass
-      stage_tile<T, TB, 256>(Tm, ro, rrows, ko, krows, Ts, LDT);
-    }
+    __syncthreads();  // Xs and S are reused by the next claim
   }
 }
 
@@ -188,6 +397,10 @@ void tri_invert_blocks(hipStream_t s, bool lower, bool unit, int n, View<const T
   MG_HIP(hipGetLastError());
 }
 
+// Tags of the published words: one counter for every element type, since
+// f32 and f64 solves on a stream share the publication buffer.
+static std::atomic<unsigned> g_chain_launches{0};
+
 // X := op(T)^-1 X, X column-major n x nrhs; D from tri_invert_blocks.
 template <typename T>
 void tri_solve_blocked(hipStream_t s, bool lower, int n, int nrhs, View<const T> Tm, const T *D,
@@ -195,16 +408,28 @@ void tri_solve_blocked(hipStream_t s, bool lower, int n, int nrhs, View<const T>
   constexpr int TB = TriTB<T>::TB;
   const int nb = (n + TB - 1) / TB;
   if (nb == 0 || nrhs <= 0) return;
-  if (lower) {
-    for (int k = -1; k < nb - 1; ++k)
-      hipLaunchKernelGGL((tri_step_kernel<T, true>), dim3(nb - 1 - k), dim3(256), 0, s, n, nrhs, k, Tm,
-                         D, X, ldx);
-  } else {
-    for (int k = nb; k > 0; --k)
-      hipLaunchKernelGGL((tri_step_kernel<T, false>), dim3(k), dim3(256), 0, s, n, nrhs, k, Tm, D, X,
-                         ldx);
+  Workspace &ws = workspace(s);
+  unsigned *claim = (unsigned *)ws.get("tri.claim", 4);
+  bool fresh = false;
+  const size_t words = (size_t)nb * TB * CHAIN_NC * ChainWords<T>::N;
+  uint64_t *pub = (uint64_t *)ws.get("tri.pub", words * 8, &fresh);
+  if (fresh) fill_u32(s, (unsigned *)pub, (long)words * 2, 0u);  // tag 0 is never used
+  int *err = sync_error_word(s);
+  // one workgroup per CU fits the LDS; more would only wait
+  const int grid = std::min(nb, num_cus());
+  for (int cb = 0; cb < nrhs; cb += CHAIN_NC) {
+    const int nc = std::min(CHAIN_NC, nrhs - cb);
+    unsigned tag = ++g_chain_launches;
+    if (tag == 0) tag = ++g_chain_launches;
+    fill_u32(s, claim, 1, 0u);
+    if (lower)
+      hipLaunchKernelGGL((tri_chain_kernel<T, true>), dim3(grid), dim3(ChainNT<T>::NT), 0, s, n, nc, Tm,
+                         D, X + (long)cb * ldx, ldx, claim, pub, tag, err);
+    else
+      hipLaunchKernelGGL((tri_chain_kernel<T, false>), dim3(grid), dim3(ChainNT<T>::NT), 0, s, n, nc, Tm,
+                         D, X + (long)cb * ldx, ldx, claim, pub, tag, err);
+    MG_HIP(hipGetLastError());
   }
-  MG_HIP(hipGetLastError());
 }
 
 #define MG_TRI_INST(T)                                                                           \
