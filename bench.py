@@ -1,21 +1,38 @@
-"""Benchmark: mixed-precision dense solve on one MI355X.
+"""Benchmarks of the dense factor-and-solve path on one MI355X.
 
-Workload (the metric's configuration): N = 16384 general dense system,
-16 right-hand sides ~ uniform[-1,1], A = U diag(sigma) V^T with U, V the Q
-factors of seeded Gaussian matrices and sigma geometrically spaced for
-condition numbers kappa in {1e2, 1e4, 1e6}.  One step = solving all three
-systems with mat_gpu_gesv_mixed (fp32 working LU with BF16-MFMA trailing
-updates + fp64 iterative refinement to backward error <= 1e-14).  The
-comparison is the plain fp64 GPU LU solve (mat_gpu_gesv_f64) on the same
-systems.  Both are credited with the same nominal FLOPs,
+Default workload (the headline metric): mixed-precision general solve.
+N = 16384 dense system, 16 right-hand sides ~ U[-1,1], A = U diag(sigma) V^T
+with U, V the Q factors of seeded Gaussian matrices and sigma geometrically
+spaced for condition numbers kappa in {1e2, 1e4, 1e6}.  One step = solving
+all three systems with mat_gpu_gesv_mixed (fp32 working LU with split-BF16
+MFMA trailing updates + fp64 iterative refinement to backward error
+<= 1e-14).  The comparison is the plain fp64 GPU LU solve (mat_gpu_gesv_f64)
+on the same systems, plus torch.linalg.solve (the vendor fp64 solver) for
+reference.  Both are credited with the same nominal FLOPs,
 2/3 n^3 + 2 n^2 * nrhs per system (the reference's own FLOP model,
 tests/bench/compare/bench_solve.cpp:79).  Inputs are resident in HBM when
 the timed region starts.
 
-The CPU baseline is the reference's x86 scalar path (oracle/_ref, compiled
-from the reference sources; the clean-room oracle when that is absent):
-factor once + 16 x (forward, backward) substitutions, fp64, one core, on a
-bounded sample size, reported as GFLOP/s.
+Other workloads (--workload):
+  gemm       fp32 C = A B, 8192^3 (mat_gemm's GPU form)
+  chol       fp32 Cholesky solve, N = 8192, 8 right-hand sides, SPD
+             A = G G^T / N + I
+  mixed-spd  N = 32768 SPD, A = Q diag(lambda) Q^T, lambda log-uniform on
+             [1, kappa], kappa in {1e2, 1e3}, 4 right-hand sides ~ N(0,1):
+             mat_gpu_posv_mixed (FP8 tile-scaled trailing updates + fp64
+             refinement) against mat_gpu_posv_f64
+  batched    200000 systems with n log-uniform on [3, 64], half general
+             (U[-1,1] + n I), half SPD (G G^T + I), one right-hand side
+             each, fp32: one persistent size-binned launch replayed from a
+             hipGraph, against one launch per matrix
+
+Every line carries "roofline" for the workload's dominant kernel family:
+achieved = algorithmic bytes (or flops) of its launches / their summed
+durations, measured with HIP events recorded by the library on the launch
+stream during the timed steps; and "cpu_baseline": the reference's x86
+scalar path (oracle/_ref, compiled from the reference sources; the
+clean-room oracle restatement when that is absent) timed on one host core
+on a bounded sample.
 
 Prints one JSON line.
 """
@@ -33,14 +50,128 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK = {"bf16_tflops": 2500.0, "fp32_tflops": 157.3, "fp64_tflops": 78.6, "hbm_gbs": 8000.0}
+# MI355X published dense peaks
+PEAK = {"bf16": 2500.0, "fp8": 5000.0, "fp32": 157.3, "fp64": 78.6, "hbm": 8000.0}
+FAMILIES = ["panel", "rowswap", "gemm", "gemm_bf16x", "gemm_fp8", "pack", "trsm", "tri_chain",
+            "chol_leaf"]
+# family -> (bound, unit, peak key for the given dtype)
+BOUND = {"panel": ("hbm", "GB/s"), "rowswap": ("hbm", "GB/s"), "pack": ("hbm", "GB/s"),
+         "tri_chain": ("hbm", "GB/s"), "chol_leaf": ("hbm", "GB/s"), "trsm": ("hbm", "GB/s"),
+         "gemm": ("mfma", "TFLOP/s"), "gemm_bf16x": ("mfma", "TFLOP/s"),
+         "gemm_fp8": ("mfma", "TFLOP/s")}
 
 
 def ptr(t):
     return C.c_void_p(t.data_ptr())
 
 
-def make_systems(torch, n, kappas, nrhs, seed=2026):
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+# ------------------------------------------------------------ kernel timing
+def kernel_table(lib):
+    out = {}
+    for fam in FAMILIES:
+        n = C.c_int64(0)
+        ms = C.c_double(0.0)
+        work = C.c_double(0.0)
+        rc = lib.mat_gpu_timing_query(fam.encode(), C.byref(n), C.byref(ms), C.byref(work))
+        assert rc == 0
+        if n.value:
+            out[fam] = {"launches": n.value, "ms": round(ms.value, 3), "work": work.value}
+    return out
+
+
+def roofline(table, family, gemm_dtype="fp32", steps=1):
+    """Roofline object for one kernel family from the library's HIP-event table."""
+    rec = table[family]
+    bound, unit = BOUND[family]
+    sec = rec["ms"] * 1e-3
+    if bound == "hbm":
+        achieved = rec["work"] / sec / 1e9
+        peak = PEAK["hbm"]
+    else:
+        achieved = rec["work"] / sec / 1e12
+        peak = PEAK[{"gemm_bf16x": "bf16", "gemm_fp8": "fp8"}.get(family, gemm_dtype)]
+    return {"kernel": family, "bound": bound, "achieved": round(achieved, 3), "peak": peak,
+            "unit": unit, "frac": round(achieved / peak, 5), "traffic": None,
+            "launches": rec["launches"], "avg_launch_us": round(rec["ms"] * 1e3 / rec["launches"], 3),
+            "ms_per_step": round(rec["ms"] / steps, 3)}
+
+
+def dominant(table, exclude=()):
+    fams = [f for f in table if f not in exclude and table[f]["work"] > 0]
+    return max(fams, key=lambda f: table[f]["ms"])
+
+
+def breakdown(table, steps):
+    return {f: {"launches": r["launches"] // steps, "ms_per_step": round(r["ms"] / steps, 3)}
+            for f, r in sorted(table.items(), key=lambda x: -x[1]["ms"])}
+
+
+# ------------------------------------------------------------ CPU baselines
+def _oracle(prec):
+    from oracle import oracle as orc
+    kind = "reference" if orc.available(prec, "ref") else "port"
+    return orc.Oracle(prec, kind="ref" if kind == "reference" else "port"), kind
+
+
+def _cpu_loop(fn, flops, budget_s, max_runs=5):
+    times = []
+    t_start = time.perf_counter()
+    while True:
+        t0 = time.perf_counter()
+        fn()
+        times.append(time.perf_counter() - t0)
+        if time.perf_counter() - t_start > budget_s or len(times) >= max_runs:
+            break
+    t = float(np.median(times))
+    return flops / t / 1e9, t, len(times)
+
+
+def cpu_solve_multi(n, nrhs=16, budget_s=20.0):
+    o, kind = _oracle("f64")
+    rng = np.random.default_rng(0)
+    A = rng.uniform(-1, 1, (n, n))
+    B = rng.uniform(-1, 1, (n, nrhs))
+    v, t, runs = _cpu_loop(lambda: o.solve_multi(A, B), 2.0 / 3.0 * n ** 3 + 2.0 * n ** 2 * nrhs,
+                           budget_s)
+    return {"value": round(v, 4), "unit": "GFLOP/s", "cores": 1, "kind": kind,
+            "sample": f"mat_plu + {nrhs}x(mat_solve_tril_unit, mat_solve_triu), fp64, n={n}, "
+                      f"median of {runs} runs ({t:.2f} s each), single thread"}
+
+
+def cpu_gemm(n, budget_s=20.0):
+    o, kind = _oracle("f32")
+    rng = np.random.default_rng(0)
+    A = rng.uniform(-1, 1, (n, n)).astype(np.float32)
+    B = rng.uniform(-1, 1, (n, n)).astype(np.float32)
+    v, t, runs = _cpu_loop(lambda: o.mul(A, B), 2.0 * n ** 3, budget_s, max_runs=3)
+    return {"value": round(v, 4), "unit": "GFLOP/s", "cores": 1, "kind": kind,
+            "sample": f"mat_mul fp32, n={n}, median of {runs} runs ({t:.2f} s each), single thread"}
+
+
+def cpu_chol(n, nrhs, prec, budget_s=20.0):
+    o, kind = _oracle(prec)
+    rng = np.random.default_rng(0)
+    G = rng.standard_normal((n, n))
+    A = (G @ G.T / n + np.eye(n)).astype(o.dtype)
+    bs = [rng.uniform(-1, 1, n).astype(o.dtype) for _ in range(nrhs)]
+
+    def run():
+        L, r = o.chol(A)
+        for b in bs:
+            o.solve_trilt(L, o.solve_tril(L, b))
+
+    v, t, runs = _cpu_loop(run, n ** 3 / 3.0 + 2.0 * n ** 2 * nrhs, budget_s)
+    return {"value": round(v, 4), "unit": "GFLOP/s", "cores": 1, "kind": kind,
+            "sample": f"mat_chol + {nrhs}x(mat_solve_tril, mat_solve_trilt), {prec}, n={n}, "
+                      f"median of {runs} runs ({t:.2f} s each), single thread"}
+
+
+# ------------------------------------------------------------ mixed-lu (default)
+def make_general(torch, n, kappas, nrhs, seed=2026):
     """A_kappa = U diag(sigma_kappa) V^T (fp64, column-major buffers), one B."""
     gen = torch.Generator(device="cuda").manual_seed(seed)
     G1 = torch.randn(n, n, dtype=torch.float64, device="cuda", generator=gen)
@@ -54,8 +185,7 @@ def make_systems(torch, n, kappas, nrhs, seed=2026):
         sig = torch.logspace(0, -np.log10(kappa), n, dtype=torch.float64, device="cuda")
         # the column-major buffer of (U diag(sig) V^T)^T = V diag(sig) U^T is the
         # row-major tensor U diag(sig) V^T: either way a member of the same family
-        A = (U * sig) @ V.T
-        mats.append(A.contiguous())
+        mats.append(((U * sig) @ V.T).contiguous())
     del U, V
     B = (torch.rand(nrhs, n, dtype=torch.float64, device="cuda", generator=gen) * 2 - 1)
     torch.cuda.synchronize()
@@ -74,124 +204,56 @@ def berr_fp64(torch, A, X, B):
     return float(e.max())
 
 
-def cpu_baseline(budget_s=20.0, n=None):
-    """Reference x86 scalar path, fp64, one core: factor once + 16 RHS."""
-    from oracle import oracle as orc
-    kind = "reference" if orc.available("f64", "ref") else "port"
-    o = orc.Oracle("f64", kind="ref" if kind == "reference" else "port")
-    rng = np.random.default_rng(0)
-    nrhs = 16
-    size = n or 1024
-    times = []
-    t_start = time.perf_counter()
-    while True:
-        A = rng.uniform(-1, 1, (size, size))
-        B = rng.uniform(-1, 1, (size, nrhs))
-        t0 = time.perf_counter()
-        o.solve_multi(A, B)
-        times.append(time.perf_counter() - t0)
-        if time.perf_counter() - t_start > budget_s or len(times) >= 5:
-            break
-    t = float(np.median(times))
-    flops = 2.0 / 3.0 * size ** 3 + 2.0 * size ** 2 * nrhs
-    return {"value": flops / t / 1e9, "unit": "GFLOP/s", "cores": 1, "kind": kind,
-            "sample": f"mat_plu + 16x(mat_solve_tril_unit, mat_solve_triu), fp64, n={size}, "
-                      f"median of {len(times)} runs ({t:.2f} s each), single thread"}
-
-
-def gemm_roofline(torch, lib, n, passes, reps=5):
-    """The dominant compute kernel of the mixed LU: the split-BF16 MFMA GEMM
-    at the top-level trailing-update shape (n/2)^3, timed with HIP events on
-    the stream it is launched on (torch's current stream)."""
-    m = n // 2
-    A = torch.rand(m * m, device="cuda") - 0.5
-    B = torch.rand(m * m, device="cuda") - 0.5
-    Cc = torch.zeros(m * m, device="cuda")
-    s = torch.cuda.current_stream()
-    call = lambda: lib.mat_gpu_gemm_lowp(s.cuda_stream, passes, m, m, m, -1.0, ptr(A), m, ptr(B), m,
-                                         ptr(Cc), m)
-    lib_rc = call()
-    assert lib_rc == 0
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record(s)
-    for _ in range(reps):
-        call()
-    e1.record(s)
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / reps
-    mfma_flops = 2.0 * m ** 3 * passes
-    achieved = mfma_flops / (ms * 1e-3) / 1e12
-    del A, B, Cc
-    return {"kernel": f"gemm_bf16x_kernel (passes={passes}, {m}x{m}x{m}, incl. operand packing)",
-            "bound": "mfma", "achieved": round(achieved, 2), "peak": PEAK["bf16_tflops"],
-            "unit": "TFLOP/s", "frac": round(achieved / PEAK["bf16_tflops"], 4),
-            "traffic": None, "ms_per_launch": round(ms, 4)}
-
-
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--n", type=int, default=16384)
-    ap.add_argument("--nrhs", type=int, default=16)
-    ap.add_argument("--kappas", default="1e2,1e4,1e6")
-    ap.add_argument("--passes", type=int, default=6)
-    ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-n", type=int, default=1024)
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); the "
-              "multi-GPU layout is a design note, not a measured path", file=sys.stderr)
-        sys.exit(2)
-
-    import torch
-    import libmat_amd
-
-    torch.cuda.set_device(0)
-    lib = libmat_amd.load("f64").lib
+def run_mixed(torch, libmat_amd, args, spd):
     g = libmat_amd.load("f64")
+    lib = g.lib
     n, nrhs = args.n, args.nrhs
     kappas = [float(k) for k in args.kappas.split(",")]
-    print(f"[bench] generating {len(kappas)} systems n={n}", file=sys.stderr, flush=True)
-    mats, B = make_systems(torch, n, kappas, nrhs)
+    log(f"generating {len(kappas)} {'SPD' if spd else 'general'} systems n={n}")
+    mats, B = (make_spd(torch, n, kappas, nrhs) if spd else make_general(torch, n, kappas, nrhs))
     X = torch.zeros(nrhs * n, dtype=torch.float64, device="cuda")
     stream = torch.cuda.current_stream().cuda_stream
     opts = libmat_amd.IrOpts()
-    lib.mat_gpu_ir_defaults(C.byref(opts), 0)
-    opts.passes = args.passes
+    lib.mat_gpu_ir_defaults(C.byref(opts), 1 if spd else 0)
+    if args.passes is not None:
+        opts.passes = args.passes
     status = [libmat_amd.IrStatus() for _ in kappas]
+    solve = lib.mat_gpu_posv_mixed if spd else lib.mat_gpu_gesv_mixed
+    plain = lib.mat_gpu_posv_f64 if spd else lib.mat_gpu_gesv_f64
 
     def step():
         for i, A in enumerate(mats):
-            rc = lib.mat_gpu_gesv_mixed(stream, n, nrhs, ptr(A), n, ptr(B), n, ptr(X), n,
-                                        C.byref(opts), C.byref(status[i]))
-            g.check(rc, "gesv_mixed")
+            g.check(solve(stream, n, nrhs, ptr(A), n, ptr(B), n, ptr(X), n, C.byref(opts),
+                          C.byref(status[i])), "mixed solve")
 
     for w in range(args.warmup):
-        print(f"[bench] warmup {w}", file=sys.stderr, flush=True)
         step()
+        log(f"warmup {w} done")
     torch.cuda.synchronize()
+    g.check(lib.mat_gpu_timing_enable(1), "timing")
     t0 = time.perf_counter()
     for k in range(args.steps):
         step()
-        print(f"[bench] step {k} done", file=sys.stderr, flush=True)
+        log(f"step {k} done")
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    g.check(lib.mat_gpu_timing_enable(0), "timing")
+    table = kernel_table(lib)
     ms_per_step = elapsed / args.steps * 1e3
-    flops_per_solve = 2.0 / 3.0 * n ** 3 + 2.0 * n ** 2 * nrhs
+    if spd:
+        flops_per_solve = n ** 3 / 3.0 + 2.0 * n ** 2 * nrhs
+    else:
+        flops_per_solve = 2.0 / 3.0 * n ** 3 + 2.0 * n ** 2 * nrhs
     value = flops_per_solve * len(kappas) * args.steps / elapsed / 1e9
 
-    # per-kappa accuracy and the plain fp64 comparison (outside the timed region)
+    # per-kappa accuracy and the fp64 comparisons (outside the timed region)
     per = []
     Xp = torch.zeros_like(X)
     plain_ms_total = 0.0
+    vendor_ms_total = 0.0
     for i, (kappa, A) in enumerate(zip(kappas, mats)):
-        rc = lib.mat_gpu_gesv_mixed(stream, n, nrhs, ptr(A), n, ptr(B), n, ptr(X), n,
-                                    C.byref(opts), C.byref(status[i]))
-        g.check(rc, "gesv_mixed")
+        g.check(solve(stream, n, nrhs, ptr(A), n, ptr(B), n, ptr(X), n, C.byref(opts),
+                      C.byref(status[i])), "mixed solve")
         st = status[i]
         entry = {"kappa": kappa, "iterations": st.iterations, "fell_back": bool(st.fell_back),
                  "backward_error": berr_fp64(torch, A, X, B),
@@ -200,7 +262,7 @@ def main():
         if not args.no_plain:
             torch.cuda.synchronize()
             tp = time.perf_counter()
-            g.check(lib.mat_gpu_gesv_f64(stream, n, nrhs, ptr(A), n, ptr(B), n, ptr(Xp), n), "gesv_f64")
+            g.check(plain(stream, n, nrhs, ptr(A), n, ptr(B), n, ptr(Xp), n), "plain fp64")
             torch.cuda.synchronize()
             tp = time.perf_counter() - tp
             plain_ms_total += tp * 1e3
@@ -208,28 +270,407 @@ def main():
             entry["plain_fp64_ms"] = round(tp * 1e3, 3)
             entry["plain_fp64_backward_error"] = berr_fp64(torch, A, Xp, B)
             entry["forward_error_vs_fp64"] = float((Xm - Xr).abs().max() / Xr.abs().max())
+            # vendor fp64 solver through torch (rocSOLVER / hipSOLVER), same system
+            Am, Bm = A.T, B.T
+            torch.cuda.synchronize()
+            tv = time.perf_counter()
+            if spd:
+                Xv = torch.cholesky_solve(Bm, torch.linalg.cholesky(Am))
+            else:
+                Xv = torch.linalg.solve(Am, Bm)
+            torch.cuda.synchronize()
+            tv = time.perf_counter() - tv
+            vendor_ms_total += tv * 1e3
+            entry["vendor_fp64_ms"] = round(tv * 1e3, 3)
+            entry["vendor_fp64_backward_error"] = berr_fp64(torch, A, Xv.T.contiguous(), B)
+            del Xv
         per.append(entry)
-        print(f"[bench] kappa={kappa:g}: {entry}", file=sys.stderr, flush=True)
+        log(f"kappa={kappa:g}: {entry}")
 
+    name = "mixed-precision SPD solve" if spd else "mixed-precision general solve"
     out = {"metric": "solve GFLOP/s: mixed-prec LU/Chol+IR vs plain FP64 at equal backward error",
            "value": round(value, 2), "unit": "GFLOP/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True,
-           "scaling": "weak", "vs_baseline": None, "dtype": "f64 (bf16/fp32 factorization + fp64 refinement)",
-           "data": "synthetic (seeded U diag(sigma) V^T, U,V = QR of Gaussian)",
-           "config": {"workload": "mixed-precision general solve", "n": n, "nrhs": nrhs,
-                      "kappas": kappas, "trailing_update": f"bf16 x{args.passes} split MFMA",
-                      "refinement": "fp64 residual + fp64 update", "parallelism": "single GPU"},
-           "per_kappa": per}
+           "scaling": "weak", "vs_baseline": None,
+           "dtype": ("f32+fp8 factorization, f64 refinement" if spd else
+                     "f32+bf16 factorization, f64 refinement"),
+           "data": "synthetic (seeded; " + ("Q diag(lambda) Q^T, Q = QR of Gaussian" if spd else
+                                             "U diag(sigma) V^T, U,V = QR of Gaussian") + ")",
+           "config": {"workload": name, "n": n, "nrhs": nrhs, "kappas": kappas,
+                      "factorization": ("fp32 Cholesky, FP8 e4m3 per-128x128-tile-scaled MFMA "
+                                        "trailing updates" if spd else
+                                        f"fp32 LU (partial pivoting), BF16 x{opts.passes} split "
+                                        "MFMA trailing updates"),
+                      "refinement": "fp64 residual + fp64 update, fallback to fp64 on stall",
+                      "parallelism": "single GPU"},
+           "per_kappa": per, "kernels": breakdown(table, args.steps)}
     if not args.no_plain:
         plain_gflops = flops_per_solve * len(kappas) / (plain_ms_total * 1e-3) / 1e9
         out["plain_fp64"] = {"value": round(plain_gflops, 2), "unit": "GFLOP/s",
                              "ms_per_step": round(plain_ms_total, 3)}
         out["speedup_vs_plain_fp64"] = round(value / plain_gflops, 3)
+        vendor_gflops = flops_per_solve * len(kappas) / (vendor_ms_total * 1e-3) / 1e9
+        out["vendor_fp64"] = {"value": round(vendor_gflops, 2), "unit": "GFLOP/s",
+                              "ms_per_step": round(vendor_ms_total, 3),
+                              "solver": ("torch.linalg.cholesky + cholesky_solve (fp64)" if spd
+                                         else "torch.linalg.solve (fp64)")}
     del mats
     torch.cuda.empty_cache()
-    out["roofline"] = gemm_roofline(torch, lib, n, args.passes if args.passes in (1, 3, 6) else 3)
+    dom = dominant(table)
+    out["roofline"] = roofline(table, dom, steps=args.steps)
+    gem = "gemm_fp8" if spd else "gemm_bf16x"
+    if gem in table and gem != dom:
+        out["roofline_trailing_update"] = roofline(table, gem, steps=args.steps)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(n=args.cpu_n)
+        out["cpu_baseline"] = (cpu_chol(args.cpu_n, nrhs, "f64") if spd else
+                               cpu_solve_multi(args.cpu_n, nrhs))
+    return out
+
+
+def orthonormal(torch, n, gen):
+    """Q factor (positive-diagonal R) of an n x n Gaussian by CholeskyQR2:
+    Q = G R^-1 with R = chol(G^T G), applied twice; each pass is a GEMM, a
+    Cholesky and a triangular solve, logged separately."""
+    Q = torch.randn(n, n, dtype=torch.float64, device="cuda", generator=gen)
+    for it in range(2):
+        R = torch.linalg.cholesky(Q.T @ Q, upper=True)
+        Q = torch.linalg.solve_triangular(R, Q, upper=True, left=False)
+        del R
+        torch.cuda.synchronize()
+        log(f"orthonormalization pass {it} done")
+    return Q
+
+
+def make_spd(torch, n, kappas, nrhs, seed=2027):
+    gen = torch.Generator(device="cuda").manual_seed(seed)
+    Q = orthonormal(torch, n, gen)
+    mats = []
+    for kappa in kappas:
+        u = torch.rand(n, dtype=torch.float64, device="cuda", generator=gen)
+        lam = torch.exp(u * np.log(kappa))
+        A = (Q * lam) @ Q.T
+        A = (A + A.T) * 0.5
+        mats.append(A.contiguous())
+        del A
+        torch.cuda.synchronize()
+        log(f"kappa={kappa:g} system built")
+    del Q
+    B = torch.randn(nrhs, n, dtype=torch.float64, device="cuda", generator=gen)
+    torch.cuda.synchronize()
+    return mats, B.contiguous()
+
+
+# ------------------------------------------------------------ gemm
+def run_gemm(torch, libmat_amd, args):
+    g = libmat_amd.load("f32")
+    lib = g.lib
+    n = args.n
+    gen = torch.Generator(device="cuda").manual_seed(7)
+    A = torch.rand(n * n, device="cuda", generator=gen) * 2 - 1
+    B = torch.rand(n * n, device="cuda", generator=gen) * 2 - 1
+    Cc = torch.zeros(n * n, device="cuda")
+    s = torch.cuda.current_stream().cuda_stream
+
+    def step():
+        g.check(lib.mat_gpu_gemm_f32(s, 0, 0, n, n, n, 1.0, ptr(A), n, ptr(B), n, 0.0, ptr(Cc), n),
+                "gemm")
+
+    for w in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    g.check(lib.mat_gpu_timing_enable(1), "timing")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    g.check(lib.mat_gpu_timing_enable(0), "timing")
+    table = kernel_table(lib)
+    # sampled check against an fp64 product of the same operands
+    rows = torch.randint(0, n, (64,), device="cuda", generator=gen)
+    Am = A.view(n, n).T  # column-major buffers
+    Bm = B.view(n, n).T
+    ref = Am[rows].double() @ Bm.double()
+    got = Cc.view(n, n).T[rows].double()
+    rel = float((got - ref).abs().max() / ref.abs().max())
+    value = 2.0 * n ** 3 * args.steps / elapsed / 1e9
+    out = {"metric": "fp32 GEMM GFLOP/s", "value": round(value, 2), "unit": "GFLOP/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": round(elapsed / args.steps * 1e3, 3), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic U[-1,1]",
+           "config": {"workload": "fp32 GEMM (mat_gemm)", "m": n, "n": n, "k": n,
+                      "parallelism": "single GPU"},
+           "max_rel_err_vs_fp64_sampled_rows": rel, "kernels": breakdown(table, args.steps),
+           "roofline": roofline(table, "gemm", "fp32", args.steps)}
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_gemm(min(args.cpu_n, 1024))
+    return out
+
+
+# ------------------------------------------------------------ chol
+def run_chol(torch, libmat_amd, args):
+    g = libmat_amd.load("f32")
+    lib = g.lib
+    n, nrhs = args.n, args.nrhs
+    gen = torch.Generator(device="cuda").manual_seed(11)
+    G = torch.randn(n, n, device="cuda", generator=gen)
+    A = (G @ G.T / n + torch.eye(n, device="cuda")).contiguous()
+    del G
+    B = (torch.rand(nrhs, n, device="cuda", generator=gen) * 2 - 1).contiguous()
+    X = torch.zeros_like(B)
+    s = torch.cuda.current_stream().cuda_stream
+
+    def step():
+        rc = lib.mat_gpu_posv_f32(s, n, nrhs, ptr(A), n, ptr(B), n, ptr(X), n)
+        g.check(rc, "posv")
+        assert rc == 0, "matrix not positive definite"
+
+    for w in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    g.check(lib.mat_gpu_timing_enable(1), "timing")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    g.check(lib.mat_gpu_timing_enable(0), "timing")
+    table = kernel_table(lib)
+    Ad = A.double()
+    R = Ad @ X.double().T - B.double().T
+    berr = float((R.abs().amax(dim=0) / (Ad.abs().sum(dim=1).max() * X.double().abs().amax(dim=1)
+                                         + B.double().abs().amax(dim=1))).max())
+    flops = n ** 3 / 3.0 + 2.0 * n ** 2 * nrhs
+    value = flops * args.steps / elapsed / 1e9
+    out = {"metric": "fp32 Cholesky solve GFLOP/s", "value": round(value, 2), "unit": "GFLOP/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": round(elapsed / args.steps * 1e3, 3), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic SPD G G^T / N + I, G ~ N(0,1)",
+           "config": {"workload": "fp32 Cholesky solve (mat_chol + mat_solve_spd)", "n": n,
+                      "nrhs": nrhs, "parallelism": "single GPU"},
+           "backward_error": berr, "kernels": breakdown(table, args.steps),
+           "roofline": roofline(table, dominant(table), "fp32", args.steps)}
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_chol(min(args.cpu_n, 1024), nrhs, "f32")
+    return out
+
+
+# ------------------------------------------------------------ batched
+def make_batch(count, seed=5, lo=3, hi=64):
+    rng = np.random.default_rng(seed)
+    ns = np.exp(rng.uniform(np.log(lo), np.log(hi + 1), count)).astype(np.int32).clip(lo, hi)
+    kinds = (np.arange(count) % 2).astype(np.int8)
+    aoff = np.zeros(count, np.int64)
+    voff = np.zeros(count, np.int64)
+    aoff[1:] = np.cumsum(ns.astype(np.int64) ** 2)[:-1]
+    voff[1:] = np.cumsum(ns.astype(np.int64))[:-1]
+    A = np.empty(int(aoff[-1] + ns[-1] ** 2), np.float32)
+    b = rng.uniform(-1, 1, int(voff[-1] + ns[-1])).astype(np.float32)
+    for n in range(lo, hi + 1):
+        for k in (0, 1):
+            idx = np.nonzero((ns == n) & (kinds == k))[0]
+            if idx.size == 0:
+                continue
+            if k == 0:
+                M = rng.uniform(-1, 1, (idx.size, n, n)) + n * np.eye(n)
+            else:
+                G = rng.standard_normal((idx.size, n, n))
+                M = G @ np.transpose(G, (0, 2, 1)) + np.eye(n)
+            Mf = np.transpose(M, (0, 2, 1)).reshape(idx.size, n * n).astype(np.float32)  # column-major
+            for j, i in enumerate(idx):
+                A[aoff[i]:aoff[i] + n * n] = Mf[j]
+    return ns, kinds, A, b, aoff, voff
+
+
+def run_batched(torch, libmat_amd, args):
+    g = libmat_amd.load("f32")
+    lib = g.lib
+    count = args.count
+    log(f"generating {count} systems")
+    ns, kinds, A, b, aoff, voff = make_batch(count)
+    d = {k: torch.from_numpy(v).cuda() for k, v in (("A", A), ("b", b), ("aoff", aoff),
+                                                      ("voff", voff))}
+    x = torch.zeros(b.size, dtype=torch.float32, device="cuda")
+    info = torch.zeros(count, dtype=torch.int32, device="cuda")
+    s = torch.cuda.current_stream().cuda_stream
+    plan = C.c_void_p()
+    g.check(lib.mat_gpu_batch_create_f32(s, count, ns.ctypes.data_as(C.c_void_p),
+                                         kinds.ctypes.data_as(C.c_void_p), ptr(d["A"]),
+                                         ptr(d["aoff"]), ptr(d["b"]), ptr(x), ptr(d["voff"]),
+                                         ptr(info), 1, C.byref(plan)), "batch plan")
+    wg, order, tuned = C.c_int(), C.c_int(), C.c_double()
+    g.check(lib.mat_gpu_batch_tuning(plan, C.byref(wg), C.byref(order), C.byref(tuned)), "tuning")
+
+    def step():
+        g.check(lib.mat_gpu_batch_run(s, plan), "batch run")
+
+    for w in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record()
+    for _ in range(args.steps):
+        step()
+    e1.record()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    batch_ms = e0.elapsed_time(e1) / args.steps
+    assert int((info != 0).sum()) == 0
+    # residual check on a sample
+    xh = x.cpu().numpy()
+    worst = 0.0
+    for i in range(0, count, max(1, count // 500)):
+        n = ns[i]
+        M = A[aoff[i]:aoff[i] + n * n].reshape(n, n, order="F").astype(np.float64)
+        xi = xh[voff[i]:voff[i] + n].astype(np.float64)
+        bi = b[voff[i]:voff[i] + n].astype(np.float64)
+        r = np.abs(M @ xi - bi).max() / (np.abs(M).sum(1).max() * np.abs(xi).max() + np.abs(bi).max())
+        worst = max(worst, r)
+
+    # per-matrix baselines on a sample: one plan (a one-matrix persistent launch,
+    # replayed from its graph) per matrix, and the single-system entry points
+    sample = min(args.loop_sample, count)
+    plans = []
+    for i in range(sample):
+        p = C.c_void_p()
+        nn = ns[i:i + 1].copy()
+        kk = kinds[i:i + 1].copy()
+        ao = torch.tensor([0], dtype=torch.int64, device="cuda")
+        sub_a = d["A"][aoff[i]:aoff[i] + nn[0] ** 2]
+        sub_b = d["b"][voff[i]:voff[i] + nn[0]]
+        sub_x = x[voff[i]:voff[i] + nn[0]]
+        g.check(lib.mat_gpu_batch_create_f32(s, 1, nn.ctypes.data_as(C.c_void_p),
+                                             kk.ctypes.data_as(C.c_void_p), ptr(sub_a), ptr(ao),
+                                             ptr(sub_b), ptr(sub_x), ptr(ao), ptr(info[i:i + 1]), 0,
+                                             C.byref(p)), "plan")
+        plans.append((p, ao))
+    for p, _ in plans[:10]:
+        lib.mat_gpu_batch_run(s, p)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for p, _ in plans:
+        lib.mat_gpu_batch_run(s, p)
+    torch.cuda.synchronize()
+    per_launch_s = (time.perf_counter() - t0) / sample
+    for p, _ in plans:
+        lib.mat_gpu_batch_destroy(p)
+    sample2 = min(200, count)
+    xs = torch.zeros(64, dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(sample2):
+        n = int(ns[i])
+        sa = d["A"][aoff[i]:aoff[i] + n * n]
+        sb = d["b"][voff[i]:voff[i] + n]
+        if kinds[i] == 0:
+            lib.mat_gpu_gesv_f32(s, n, 1, ptr(sa), n, ptr(sb), n, ptr(xs), n)
+        else:
+            lib.mat_gpu_posv_f32(s, n, 1, ptr(sa), n, ptr(sb), n, ptr(xs), n)
+    torch.cuda.synchronize()
+    per_call_s = (time.perf_counter() - t0) / sample2
+    lib.mat_gpu_batch_destroy(plan)
+
+    n64 = ns.astype(np.float64)
+    gen_mask = kinds == 0
+    flops = float(np.sum(np.where(gen_mask, 2.0 / 3.0 * n64 ** 3 + 2 * n64 ** 2,
+                                  n64 ** 3 / 3.0 + 2 * n64 ** 2)))
+    bytes_min = float(np.sum(n64 ** 2 + 2 * n64) * 4)
+    value = flops / (batch_ms * 1e-3) / 1e9
+    achieved = bytes_min / (batch_ms * 1e-3) / 1e9
+    loop_ms = per_launch_s * count * 1e3
+    out = {"metric": "batched small solves GFLOP/s", "value": round(value, 2), "unit": "GFLOP/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": round(elapsed / args.steps * 1e3, 3), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic: n log-uniform on [3,64]; general U[-1,1]+nI / SPD G G^T + I",
+           "config": {"workload": "batched uneven small solves", "count": count,
+                      "parallelism": "single GPU"},
+           "matrices_per_s": round(count / (batch_ms * 1e-3), 1),
+           "batch_ms": round(batch_ms, 4), "tuning": {"wg_per_cu": wg.value,
+                                                      "order_mode": order.value,
+                                                      "tuned_ms": round(tuned.value, 4)},
+           "max_backward_error_sampled": worst,
+           "per_matrix_launch_loop": {"ms_for_all": round(loop_ms, 2),
+                                      "sample": f"{sample} one-matrix plans, one graph launch each",
+                                      "speedup_of_batched": round(loop_ms / batch_ms, 1)},
+           "per_matrix_entry_point_loop": {
+               "ms_for_all": round(per_call_s * count * 1e3, 2),
+               "sample": f"{sample2} mat_gpu_gesv_f32 / mat_gpu_posv_f32 calls",
+               "speedup_of_batched": round(per_call_s * count * 1e3 / batch_ms, 1)},
+           "roofline": {"kernel": "batched persistent solve (group_lu / group_chol)",
+                        "bound": "hbm", "achieved": round(achieved, 3), "peak": PEAK["hbm"],
+                        "unit": "GB/s", "frac": round(achieved / PEAK["hbm"], 5),
+                        "traffic": None, "avg_launch_us": round(batch_ms * 1e3, 3)}}
+    if not args.no_cpu:
+        o, kind = _oracle("f32")
+        csample = min(args.cpu_count, count)
+        mats = [A[aoff[i]:aoff[i] + ns[i] ** 2].reshape(ns[i], ns[i], order="F")
+                for i in range(csample)]
+        vecs = [b[voff[i]:voff[i] + ns[i]] for i in range(csample)]
+        t0 = time.perf_counter()
+        for i in range(csample):
+            if kinds[i] == 0:
+                o.solve(mats[i], vecs[i])
+            else:
+                o.solve_spd(mats[i], vecs[i])
+        t = time.perf_counter() - t0
+        sflops = float(np.sum(np.where(gen_mask[:csample], 2.0 / 3.0 * n64[:csample] ** 3
+                                       + 2 * n64[:csample] ** 2,
+                                       n64[:csample] ** 3 / 3.0 + 2 * n64[:csample] ** 2)))
+        out["cpu_baseline"] = {"value": round(sflops / t / 1e9, 4), "unit": "GFLOP/s", "cores": 1,
+                               "kind": kind,
+                               "sample": f"first {csample} of the {count} systems, mat_solve / "
+                                         f"mat_solve_spd per matrix through ctypes, "
+                                         f"{t:.2f} s, single thread"}
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="mixed-lu",
+                    choices=["mixed-lu", "mixed-spd", "gemm", "chol", "batched"])
+    ap.add_argument("--n", type=int, default=None)
+    ap.add_argument("--nrhs", type=int, default=None)
+    ap.add_argument("--kappas", default=None)
+    ap.add_argument("--passes", type=int, default=None)
+    ap.add_argument("--count", type=int, default=200000)
+    ap.add_argument("--loop-sample", type=int, default=2000)
+    ap.add_argument("--cpu-count", type=int, default=20000)
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-n", type=int, default=1024)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); the "
+              "multi-GPU layout is a design note, not a measured path", file=sys.stderr)
+        sys.exit(2)
+    defaults = {"mixed-lu": (16384, 16, "1e2,1e4,1e6", 2), "mixed-spd": (32768, 4, "1e2,1e3", 1),
+                "gemm": (8192, 0, None, 10), "chol": (8192, 8, None, 5), "batched": (0, 1, None, 5)}
+    dn, dr, dk, ds = defaults[args.workload]
+    args.n = args.n or dn
+    args.nrhs = args.nrhs or dr
+    args.kappas = args.kappas or dk
+    args.steps = args.steps or ds
+
+    import torch
+    import libmat_amd
+
+    torch.cuda.set_device(0)
+    if args.workload == "mixed-lu":
+        out = run_mixed(torch, libmat_amd, args, spd=False)
+    elif args.workload == "mixed-spd":
+        out = run_mixed(torch, libmat_amd, args, spd=True)
+    elif args.workload == "gemm":
+        out = run_gemm(torch, libmat_amd, args)
+    elif args.workload == "chol":
+        out = run_chol(torch, libmat_amd, args)
+    else:
+        out = run_batched(torch, libmat_amd, args)
     print(json.dumps(out), flush=True)
 
 

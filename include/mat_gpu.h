@@ -123,6 +123,22 @@ int mat_gpu_elem_size(void);
  * potrs, gesv, posv on a stream) are checked by the next synchronous call
  * on the same stream or by this one. */
 int mat_gpu_stream_check(void *stream);
+/* Opt-in kernel timing.  While enabled, the library records a HIP event
+ * pair on the launch stream around each launch of these families and
+ * files it with the launch's algorithmic work:
+ *   "panel"   GEPP panel (bytes: the m x w panel read and written once)
+ *   "rowswap" row interchanges outside the panel (work 0)
+ *   "gemm"    native fp32/fp64 MFMA GEMM (flops 2mnk)
+ *   "gemm_bf16x" split-bf16 GEMM (MFMA flops: passes * 2mnk)
+ *   "gemm_fp8"   fp8 GEMM (flops 2mnk)
+ *   "pack"    operand packing for the low-precision GEMMs (bytes moved)
+ *   "trsm"    triangular-solve leaves and block inverses (work 0)
+ *   "tri_chain" chained triangular solve (bytes: triangle + solutions)
+ * Enabling clears earlier records (and synchronizes the device). */
+int mat_gpu_timing_enable(int on);
+/* Launch count, summed event time (ms) and summed work of one family;
+ * synchronizes on the recorded events. */
+int mat_gpu_timing_query(const char *family, int64_t *launches, double *total_ms, double *work);
 
 /* C = alpha * op(A) * op(B) + beta * C; trans_* = 0 (N) or 1 (T).
  * MFMA fp32 / fp64.  beta == 0 never reads C.  Asynchronous on stream. */
@@ -178,11 +194,15 @@ int mat_gpu_posv_f64(void *stream, size_t n, size_t nrhs, const double *a, size_
  *   passes: trailing-update multiply of the fp32 factorization (default
  *           6 for gesv, 8 for posv):
  *           0 = fp32 MFMA, 1/3/6 = BF16 MFMA with 1/3/6 split products,
- *           8 = FP8 e4m3 with per-128x128-tile scales (posv only).
+ *           8 = FP8 e4m3 with per-128x128-tile scales (posv only; the
+ *           trailing updates only, the panel solves use 3-pass BF16).
+ *           Products below 2e9 multiply-adds run as fp32 MFMA.
  *   The residual b - A x (A in fp64) and x += d are the only fp64 steps.
  *   Stops at backward error max_c ||b_c - A x_c|| / (||A|| ||x_c|| + ||b_c||)
- *   <= tol (infinity norms); falls back to the plain fp64 factorization
- *   when refinement stalls.  Synchronous. */
+ *   <= tol (infinity norms).  posv with fp8: when the factor loses positive
+ *   definiteness or refinement stalls, it refactors with 3-pass BF16 first.
+ *   Falls back to the plain fp64 factorization when refinement stalls.
+ *   Synchronous. */
 typedef struct {
   int passes;   /* default 6 (gesv) / 8 (posv) */
   int max_iter; /* default 30 */
@@ -199,7 +219,9 @@ typedef struct {
   double backward_error; /* final normwise backward error */
   double factor_ms;      /* low-precision factorization wall time */
   double total_ms;       /* whole call wall time */
-  int precision;         /* passes used */
+  int precision;         /* passes of the factorization that produced x
+                            (posv: 8 = fp8, or 3 after the fp8 factor lost
+                            definiteness or stalled; see fell_back) */
 } mat_gpu_ir_status;
 /* Fill opts with the defaults. */
 void mat_gpu_ir_defaults(mat_gpu_ir_opts *opts, int spd);

@@ -33,6 +33,7 @@ DROPIN_SYMBOLS = [
 ]
 EXT_SYMBOLS = [
     "mat_gpu_last_error", "mat_gpu_elem_size", "mat_gpu_stream_check",
+    "mat_gpu_timing_enable", "mat_gpu_timing_query",
     "mat_gpu_gemm_f32", "mat_gpu_gemm_f64", "mat_gpu_gemm_lowp", "mat_gpu_getrf_f32", "mat_gpu_getrf_f64",
     "mat_gpu_getrs_f32", "mat_gpu_getrs_f64", "mat_gpu_potrf_f32", "mat_gpu_potrf_f64",
     "mat_gpu_potrs_f32", "mat_gpu_potrs_f64", "mat_gpu_gesv_f32", "mat_gpu_gesv_f64",
@@ -106,6 +107,9 @@ class GpuLib:
         s("mat_gpu_last_error", C.c_char_p, [])
         s("mat_gpu_elem_size", i32, [])
         s("mat_gpu_stream_check", i32, [vp])
+        s("mat_gpu_timing_enable", i32, [i32])
+        s("mat_gpu_timing_query", i32, [C.c_char_p, C.POINTER(C.c_int64), C.POINTER(dbl),
+                                        C.POINTER(dbl)])
         s("mat_gpu_gemm_f32", i32, [vp, i32, i32, sz, sz, sz, f32, vp, sz, vp, sz, f32, vp, sz])
         s("mat_gpu_gemm_f64", i32, [vp, i32, i32, sz, sz, sz, dbl, vp, sz, vp, sz, dbl, vp, sz])
         s("mat_gpu_gemm_lowp", i32, [vp, i32, sz, sz, sz, f32, vp, sz, vp, sz, vp, sz])

@@ -183,6 +183,23 @@ extern "C" {
 
 const char *mat_gpu_last_error(void) { return last_error(); }
 
+int mat_gpu_timing_enable(int on) {
+  return guarded([&] {
+    timing_enable(on != 0);
+    return 0;
+  });
+}
+
+int mat_gpu_timing_query(const char *family, int64_t *launches, double *total_ms, double *work) {
+  return guarded([&] {
+    MG_CHECK(family != nullptr, "timing_query: family is NULL");
+    long c = 0;
+    timing_query(family, &c, total_ms, work);
+    if (launches) *launches = c;
+    return 0;
+  });
+}
+
 int mat_gpu_stream_check(void *s) {
   return guarded([&] {
     sync_and_check(as_stream(s));

@@ -67,6 +67,7 @@ __global__ __launch_bounds__(256) void chol_leaf_kernel(int n, T *A, long lda, i
 template <typename T>
 static void potrf_rec(hipStream_t s, int n, T *A, long lda, int j0, int *info, Mul mul) {
   if (n <= CHOL_NB) {
+    KScope ks(s, "chol_leaf", (double)n * n * sizeof(T));
     hipLaunchKernelGGL((chol_leaf_kernel<T>), dim3(1), dim3(256), 0, s, n, A, lda, j0, info);
     MG_HIP(hipGetLastError());
     return;
@@ -77,12 +78,18 @@ static void potrf_rec(hipStream_t s, int n, T *A, long lda, int j0, int *info, M
   potrf_rec<T>(s, h, A, lda, j0, info, mul);
   T *A21 = A + h;
   T *A22 = A + h + (long)h * lda;
-  // A21 := A21 L11^-T  <=>  L11 * A21^T = A21^T (A21^T is row-major in A21's storage)
-  trsm_left<T>(s, true, false, h, m, View<const T>{A, 1, lda}, View<T>{A21, lda, 1}, mul);
+  // A21 := A21 L11^-T  <=>  L11 * A21^T = A21^T (A21^T is row-major in A21's storage).
+  // The panel stays near fp32 accuracy (fp8 only ever enters the trailing
+  // update A22), so an fp8 request runs the solve with 3-pass bf16 splits.
+  const Mul tmul = mul == Mul::Fp8 ? Mul::Bf16x3 : mul;
+  trsm_left<T>(s, true, false, h, m, View<const T>{A, 1, lda}, View<T>{A21, lda, 1}, tmul);
   // A22 -= A21 A21^T on the lower triangle
   View<const T> a21{A21, 1, lda};
+  const bool big = (double)m * m * h >= kLowpMinWork;
   if constexpr (std::is_same<T, float>::value) {
-    if (mul == Mul::Fp8) {
+    if (!big) {
+      gemm<T>(s, m, m, h, T(-1), a21, a21.t(), T(1), View<T>{A22, 1, lda}, true);
+    } else if (mul == Mul::Fp8) {
       gemm_fp8(s, m, m, h, -1.0f, a21, a21.t(), View<float>{A22, 1, lda}, true);
     } else if (mul != Mul::Native) {
       gemm_bf16x(s, m, m, h, -1.0f, a21, a21.t(), View<float>{A22, 1, lda}, (int)mul);

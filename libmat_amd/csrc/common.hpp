@@ -59,8 +59,36 @@ Workspace &workspace(hipStream_t s);
 int *sync_error_word(hipStream_t s);
 void sync_and_check(hipStream_t s);
 
+// ------------------------------------------------------------ launch timing --
+// Opt-in (mat_gpu_timing_enable): a KScope placed around a launch records a
+// HIP event pair on the launch stream and files the launch under a kernel
+// family together with its algorithmic work (bytes or flops, the family's
+// own unit).  Disabled, a KScope costs one branch.
+struct KScope {
+  KScope(hipStream_t s, const char *family, double work = 0.0);
+  ~KScope();
+  KScope(const KScope &) = delete;
+  KScope &operator=(const KScope &) = delete;
+  hipStream_t s;
+  long slot;
+};
+void timing_enable(bool on);
+// launches, summed milliseconds and summed work of one family (synchronizes
+// on the recorded events)
+void timing_query(const char *family, long *launches, double *ms, double *work);
+
 // Library-owned default stream for the drop-in (synchronous) entry points.
 hipStream_t default_stream();
+
+// Second library-owned stream per device for the look-ahead LU's trailing
+// updates.  Where the runtime supports CU masks it leaves MATGPU_PANEL_CUS
+// compute units (default 32) to the caller's stream, whose pivoting panels
+// are latency bound and must not queue behind GEMM workgroups.
+hipStream_t side_stream();
+// Reusable events (timing disabled) of the current device, by index.
+hipEvent_t pooled_event(int i);
+// Look-ahead block width of the LU (MATGPU_LU_NB, default 256; 0 = none).
+int lookahead_block();
 
 // Number of compute units of the current device.
 int num_cus();
@@ -91,6 +119,10 @@ inline unsigned ceil_div(size_t a, size_t b) { return (unsigned)((a + b - 1) / b
 template <typename T>
 void gemm(hipStream_t s, long M, long N, long K, T alpha, View<const T> A,
           View<const T> B, T beta, View<T> C, bool lower_only = false);
+
+// Below this many multiply-adds (M*N*K) the factorizations run their
+// bf16-split updates as native fp32 MFMA GEMMs instead.
+constexpr double kLowpMinWork = 2.0e9;
 
 // Low-precision GEMM used by the mixed solvers: C(fp32) += alpha * A * B with
 // A, B given in fp32 and multiplied on BF16 MFMA after an exact split into

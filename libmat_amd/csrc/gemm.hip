@@ -246,6 +246,7 @@ static void launch_mode(hipStream_t s, GemmP<T> &p, bool vec) {
   p.tiles = p.tiles_m * p.tiles_n;
   int grid = p.tiles;
   if (grid >= 8) grid = (grid + 7) / 8 * 8;
+  KScope ks(s, "gemm", (p.lower ? 1.0 : 2.0) * p.M * p.N * p.K);
   if (vec)
     hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, A_MN, B_MN, true>), dim3(grid), dim3(256), 0, s, p);
   else

@@ -415,11 +415,14 @@ void gemm_bf16x(hipStream_t s, long M, long N, long K, float alpha, View<const f
   unsigned short *Ap = (unsigned short *)ws.get("lp.A", (size_t)np * Mp * Kp * 2);
   unsigned short *Bp = (unsigned short *)ws.get("lp.B", (size_t)np * Np * Kp * 2);
   // A(i,k) = A.p[i*rs + k*cs] -> [mn=i][k];  B(k,j) = B.p[k*rs + j*cs] -> [mn=j][k]
+  {
+  KScope ks(s, "pack", (double)(M + N) * K * 4 + (double)(Mp + Np) * Kp * 2 * np);
   hipLaunchKernelGGL(pack_bf16_kernel, dim3(Kp / 32, Mp / 32), dim3(256), 0, s, A.p, A.rs, A.cs,
                      (int)M, (int)K, (int)Kp, np, Mp * Kp, Ap);
   hipLaunchKernelGGL(pack_bf16_kernel, dim3(Kp / 32, Np / 32), dim3(256), 0, s, B.p, B.cs, B.rs,
                      (int)N, (int)K, (int)Kp, np, Np * Kp, Bp);
   MG_HIP(hipGetLastError());
+  }
   LpP p{};
   p.A = Ap;
   p.B = Bp;
@@ -437,6 +440,7 @@ void gemm_bf16x(hipStream_t s, long M, long N, long K, float alpha, View<const f
   p.lower = 0;
   int grid = p.tiles;
   if (grid >= 8) grid = (grid + 7) / 8 * 8;
+  KScope ks(s, "gemm_bf16x", 2.0 * passes * M * N * K);
   if (passes == 1) hipLaunchKernelGGL((gemm_bf16x_kernel<1, 1>), dim3(grid), dim3(256), 0, s, p);
   else if (passes == 3) hipLaunchKernelGGL((gemm_bf16x_kernel<2, 3>), dim3(grid), dim3(256), 0, s, p);
   else hipLaunchKernelGGL((gemm_bf16x_kernel<3, 6>), dim3(grid), dim3(256), 0, s, p);
@@ -458,6 +462,8 @@ void gemm_fp8(hipStream_t s, long M, long N, long K, float alpha, View<const flo
   unsigned char *Bp = (unsigned char *)ws.get("f8.B", (size_t)Np * Kp);
   float *sA = (float *)ws.get("f8.sA", (size_t)(Mp / 128) * ktiles * 4);
   float *sB = (float *)ws.get("f8.sB", (size_t)(Np / 128) * ktiles * 4);
+  {
+  KScope ks(s, "pack", (double)(M + N) * K * 8 + (double)(Mp + Np) * Kp);
   hipLaunchKernelGGL(fp8_scale_kernel, dim3(ktiles, Mp / 128), dim3(256), 0, s, A.p, A.rs, A.cs,
                      (int)M, (int)K, ktiles, sA);
   hipLaunchKernelGGL(fp8_scale_kernel, dim3(ktiles, Np / 128), dim3(256), 0, s, B.p, B.cs, B.rs,
@@ -467,6 +473,7 @@ void gemm_fp8(hipStream_t s, long M, long N, long K, float alpha, View<const flo
   hipLaunchKernelGGL(pack_fp8_kernel, dim3(Kp / 32, Np / 32), dim3(256), 0, s, B.p, B.cs, B.rs,
                      (int)N, (int)K, (int)Kp, ktiles, sB, Bp);
   MG_HIP(hipGetLastError());
+  }
   F8P p{};
   p.A = Ap;
   p.B = Bp;
@@ -485,6 +492,7 @@ void gemm_fp8(hipStream_t s, long M, long N, long K, float alpha, View<const flo
   p.lower = lower_only ? 1 : 0;
   int grid = p.tiles;
   if (grid >= 8) grid = (grid + 7) / 8 * 8;
+  KScope ks(s, "gemm_fp8", 2.0 * M * N * K);
   hipLaunchKernelGGL(gemm_fp8_kernel, dim3(grid), dim3(256), 0, s, p);
   MG_HIP(hipGetLastError());
 }

@@ -12,18 +12,29 @@
 // getrf_full: full-pivoting LU PAQ = LU (mat_lu, x86 branch
 // mat.h:4502-4557), one pivot search per step over the trailing
 // submatrix, reproducing the reference's row-major scan order.
+#include <algorithm>
+#include <mutex>
+#include <vector>
+
 #include "ops.hpp"
 
 namespace mg {
 
 template <typename T>
-void lu_leaf(hipStream_t s, int n, T *W, long ld, int c0, int w, int *ipiv, int *perm, T eps);
+void lu_leaf(hipStream_t s, int n, T *W, long ld, int c0, int w, int *ipiv, int *perm, T eps, int cl,
+             int ch);
+template <typename T>
+void lu_rowswap(hipStream_t s, T *W, long ld, const int *ipiv, int *perm, int k0, int w, int cl,
+                int ch);
 
 template <typename T>
 static void lu_gemm_sub(hipStream_t s, long M, long N, long K, View<const T> A, View<const T> B,
                         View<T> C, Mul mul) {
   if constexpr (std::is_same<T, float>::value) {
-    if (mul == Mul::Bf16x1 || mul == Mul::Bf16x3 || mul == Mul::Bf16x6) {
+    // small products: native fp32 MFMA is as fast as packing for the split
+    // passes (and exact to fp32); the split pays off on large updates only
+    if ((mul == Mul::Bf16x1 || mul == Mul::Bf16x3 || mul == Mul::Bf16x6) &&
+        (double)M * N * K >= kLowpMinWork) {
       gemm_bf16x(s, M, N, K, -1.0f, A, B, C, (int)mul);
       return;
     }
@@ -31,17 +42,19 @@ static void lu_gemm_sub(hipStream_t s, long M, long N, long K, View<const T> A, 
   gemm<T>(s, M, N, K, T(-1), A, B, T(1), C, false);
 }
 
+// Factor columns [c0, c0+w) (rows c0..n-1, all earlier updates applied);
+// the leaves' row interchanges reach columns [cl, ch).
 template <typename T>
 static void getrf_rec(hipStream_t s, int n, T *W, long ld, int c0, int w, int *ipiv, int *perm,
-                      T eps, const LuOpts &o) {
+                      T eps, const LuOpts &o, int cl, int ch) {
   if (w <= o.leaf) {
-    lu_leaf<T>(s, n, W, ld, c0, w, ipiv, perm, eps);
+    lu_leaf<T>(s, n, W, ld, c0, w, ipiv, perm, eps, cl, ch);
     return;
   }
   int w1 = ((w / 2 + o.leaf - 1) / o.leaf) * o.leaf;
   if (w1 >= w) w1 = w - o.leaf;
   const int w2 = w - w1;
-  getrf_rec<T>(s, n, W, ld, c0, w1, ipiv, perm, eps, o);
+  getrf_rec<T>(s, n, W, ld, c0, w1, ipiv, perm, eps, o, cl, ch);
   // U12 = L11^-1 A12 (unit lower)
   View<const T> L11{W + (long)c0 * ld + c0, ld, 1};
   View<T> A12{W + (long)c0 * ld + c0 + w1, ld, 1};
@@ -51,7 +64,88 @@ static void getrf_rec(hipStream_t s, int n, T *W, long ld, int c0, int w, int *i
   View<const T> L21{W + (long)(c0 + w1) * ld + c0, ld, 1};
   View<T> A22{W + (long)(c0 + w1) * ld + c0 + w1, ld, 1};
   lu_gemm_sub<T>(s, m2, w2, w1, L21, cview(A12), A22, o.mul);
-  getrf_rec<T>(s, n, W, ld, c0 + w1, w2, ipiv, perm, eps, o);
+  getrf_rec<T>(s, n, W, ld, c0 + w1, w2, ipiv, perm, eps, o, cl, ch);
+}
+
+// Row interchanges of the factored block [b0, b1) (its leaves in order)
+// applied to columns [cl, ch).
+template <typename T>
+static void block_swaps(hipStream_t s, T *W, long ld, const int *ipiv, int b0, int b1, int leaf, int cl,
+                        int ch) {
+  if (ch <= cl) return;
+  for (int k = b0; k < b1; k += leaf) lu_rowswap<T>(s, W, ld, ipiv, nullptr, k, std::min(leaf, b1 - k), cl, ch);
+}
+
+// U12 = L11^-1 A12 and A22 -= L21 U12 for the factored block [b0, b1) and
+// the columns [c0, c1).
+template <typename T>
+static void block_update(hipStream_t s, int n, T *W, long ld, int b0, int b1, int c0, int c1,
+                         const LuOpts &o) {
+  if (c1 <= c0) return;
+  const int nb = b1 - b0;
+  View<const T> L11{W + (long)b0 * ld + b0, ld, 1};
+  View<T> A12{W + (long)b0 * ld + c0, ld, 1};
+  trsm_left<T>(s, true, true, nb, c1 - c0, L11, A12, o.mul);
+  View<const T> L21{W + (long)b1 * ld + b0, ld, 1};
+  View<T> A22{W + (long)b1 * ld + c0, ld, 1};
+  lu_gemm_sub<T>(s, n - b1, c1 - c0, nb, L21, cview(A12), A22, o.mul);
+}
+
+// Right-looking blocked LU with look-ahead over NB-wide column blocks.
+// The caller's stream (the critical path) factors block k+1 right after
+// applying block k to block k+1's columns; a second stream meanwhile
+// applies block k to everything right of block k+1 (interchanges, TRSM,
+// the large GEMM) and then brings block k's interchanges to the columns
+// left of block k.  Within a block the recursion's leaves swap only the
+// block's own columns.  The two streams never touch the same columns at
+// the same time:
+//   caller:  panel(k+1) on [b1, c2)   after the side stream has applied
+//            block k-1 to those columns (the first part of its update)
+//   side:    block k on [c2, c3), then on [c3, n), then the interchanges on
+//            [0, b0)   after panel(k) is done
+// and interchanges of different blocks reach any column in pivot order.
+template <typename T>
+static void getrf_lookahead(hipStream_t s, int n, T *W, long ld, int *ipiv, int *perm, T eps,
+                            const LuOpts &o, int NB) {
+  // the side stream and the event pool are per device: one enqueue at a time
+  static std::mutex mu;
+  std::lock_guard<std::mutex> lk(mu);
+  hipStream_t u = side_stream();
+  const int nblk = (n + NB - 1) / NB;
+  std::vector<hipEvent_t> evP(nblk), evU(nblk);
+  for (int k = 0; k < nblk; ++k) {
+    evP[k] = pooled_event(2 * k);
+    evU[k] = pooled_event(2 * k + 1);
+  }
+  // the side stream starts after everything queued on the caller's stream
+  hipEvent_t start = pooled_event(2 * nblk);
+  MG_HIP(hipEventRecord(start, s));
+  MG_HIP(hipStreamWaitEvent(u, start, 0));
+  getrf_rec<T>(s, n, W, ld, 0, std::min(NB, n), ipiv, perm, eps, o, 0, std::min(NB, n));
+  for (int k = 0; k < nblk; ++k) {
+    const int b0 = k * NB, b1 = std::min(n, b0 + NB);
+    const int c2 = std::min(n, b1 + NB);
+    MG_HIP(hipEventRecord(evP[k], s));
+    if (b1 < n) {
+      // caller: block k+1's columns, then its panel
+      if (k >= 1) MG_HIP(hipStreamWaitEvent(s, evU[k - 1], 0));
+      block_swaps<T>(s, W, ld, ipiv, b0, b1, o.leaf, b1, c2);
+      block_update<T>(s, n, W, ld, b0, b1, b1, c2, o);
+      getrf_rec<T>(s, n, W, ld, b1, c2 - b1, ipiv, perm, eps, o, b1, c2);
+    }
+    // side: the rest of the trailing matrix, block k+2's columns first (the
+    // only part the caller's next step waits for), then the columns on the left
+    MG_HIP(hipStreamWaitEvent(u, evP[k], 0));
+    const int c3 = std::min(n, c2 + NB);
+    block_swaps<T>(u, W, ld, ipiv, b0, b1, o.leaf, c2, n);
+    block_update<T>(u, n, W, ld, b0, b1, c2, c3, o);
+    MG_HIP(hipEventRecord(evU[k], u));
+    block_update<T>(u, n, W, ld, b0, b1, c3, n, o);
+    block_swaps<T>(u, W, ld, ipiv, b0, b1, o.leaf, 0, b0);
+  }
+  hipEvent_t done = pooled_event(2 * nblk + 1);
+  MG_HIP(hipEventRecord(done, u));
+  MG_HIP(hipStreamWaitEvent(s, done, 0));
 }
 
 __global__ void iota_kernel(int *p, int n) {
@@ -68,7 +162,11 @@ void getrf_rowmajor_perm(hipStream_t s, int n, T *W, long ld, int *ipiv, int *pe
   hipLaunchKernelGGL(iota_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, s, ipiv, n);
   hipLaunchKernelGGL(iota_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, s, perm, n);
   MG_HIP(hipGetLastError());
-  getrf_rec<T>(s, n, W, ld, 0, n, ipiv, perm, eps, o);
+  const int NB = lookahead_block();
+  if (NB > 0 && n >= 4 * NB && NB % o.leaf == 0)
+    getrf_lookahead<T>(s, n, W, ld, ipiv, perm, eps, o, NB);
+  else
+    getrf_rec<T>(s, n, W, ld, 0, n, ipiv, perm, eps, o, 0, n);
 }
 
 template <typename T>

@@ -95,6 +95,27 @@ __device__ inline uint64_t wave_max_u64(uint64_t k) {
   return r;
 }
 
+// Wavefront max of a float (no NaN inputs): DPP within each 16-lane row,
+// readlane across the four rows; every lane gets the result.
+template <int CTRL>
+__device__ inline float dpp_maxf(float v) {
+  const float o = __int_as_float(
+      __builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), CTRL, 0xf, 0xf, false));
+  return fmaxf(v, o);
+}
+
+__device__ inline float wave_max_f32(float v) {
+  v = dpp_maxf<0xB1>(v);
+  v = dpp_maxf<0x4E>(v);
+  v = dpp_maxf<0x141>(v);
+  v = dpp_maxf<0x140>(v);
+  const float r0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0));
+  const float r1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 16));
+  const float r2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 32));
+  const float r3 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 48));
+  return fmaxf(fmaxf(r0, r1), fmaxf(r2, r3));
+}
+
 // Pivot candidates as keys: larger key = larger |value|, then lower row.
 // Key 0 means "no candidate" (row not eligible, or NaN, which the
 // reference's strict '>' scan never selects).  f32: (|v| bits, ~row) in one
@@ -209,7 +230,7 @@ __global__ __launch_bounds__(PANEL_NT) void panel_gepp_kernel(PanelP<T> p) {
   __shared__ __align__(16) PivKey<T> wkey[2][PANEL_NW];
   __shared__ __align__(16) T krow[2][WMAX];            // row k (if this workgroup owns it)
   __shared__ __align__(16) uint32_t stage[2][2 * WMAX * NWD];  // winner row, row k (payload words)
-  __shared__ int s_piv[2];
+  __shared__ int s_piv[2], s_gw[2];
   __shared__ int s_abort;
 
   // Register rotation: at step j, slot 0 holds column j, slots 1..w-1-j the
@@ -235,12 +256,37 @@ __global__ __launch_bounds__(PANEL_NT) void panel_gepp_kernel(PanelP<T> p) {
       // 1. wavefront candidate (first row of max |a|); park it and row k
       PivKey<T> key;
       key.clear();
+      int wbest = -1;
+      if constexpr (sizeof(T) == 4) {
+        // float max over the wavefront, then the first (slot, lane) holding it:
+        // rows increase with the slot q, then with the lane
+        float tv = -1.0f;
+        int tq = -1;
 #pragma unroll
-      for (int q = 0; q < RPT; ++q)
-        if (row[q] < rend && row[q] >= k) key.offer(a[q][0], row[q]);
-      key.wave_reduce();
+        for (int q = 0; q < RPT; ++q) {
+          const float av = fabsf(a[q][0]);
+          if (row[q] < rend && row[q] >= k && av > tv) {  // NaN never wins
+            tv = av;
+            tq = q;
+          }
+        }
+        const float mx = wave_max_f32(tv);
+        if (mx >= 0.0f) {
+#pragma unroll
+          for (int q = RPT - 1; q >= 0; --q) {
+            const unsigned long long msk = __ballot(tq == q && tv == mx);
+            if (msk) wbest = p.r0 + b * RPW + q * PANEL_NT + wave * 64 + __builtin_ctzll(msk);
+          }
+          key.offer(mx, wbest);
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < RPT; ++q)
+          if (row[q] < rend && row[q] >= k) key.offer(a[q][0], row[q]);
+        key.wave_reduce();
+        wbest = key.valid() ? key.row() : -1;
+      }
       if (lane == 0) wkey[par][wave] = key;
-      const int wbest = key.valid() ? key.row() : -1;
 #pragma unroll
       for (int q = 0; q < RPT; ++q) {
         if (row[q] == wbest)
@@ -251,28 +297,42 @@ __global__ __launch_bounds__(PANEL_NT) void panel_gepp_kernel(PanelP<T> p) {
           for (int c = 0; c < WMAX; ++c) krow[par][c] = a[q][c];
       }
       __syncthreads();
-      // 2. workgroup candidate
-      PivKey<T> gkey = wkey[par][0];
-      int gw = 0;
-#pragma unroll
-      for (int x = 1; x < PANEL_NW; ++x) {
-        const PivKey<T> o = wkey[par][x];
-        PivKey<T> m = gkey;
-        m.merge(o);
-        if (m.row() != gkey.row() || (m.valid() && !gkey.valid())) gw = x;
-        gkey = m;
-      }
+      // 2. workgroup candidate, lane-parallel in wavefront 0: lane x < 8
+      //    holds wavefront x's key; the pivot index (and for P == 1 the
+      //    wavefront that parked the pivot row) goes to LDS
       int piv;
       const T *win_row = nullptr;
       const T *k_row = nullptr;
       if (P == 1) {
-        piv = gkey.valid() ? gkey.row() : k;
+        if (wave == 0) {
+          PivKey<T> mk;
+          mk.clear();
+          if (lane < PANEL_NW) mk = wkey[par][lane];
+          PivKey<T> gkey = mk;
+          gkey.wave_reduce();
+          const unsigned long long gm =
+              __ballot(lane < PANEL_NW && mk.valid() && gkey.valid() && mk.row() == gkey.row());
+          if (lane == 0) {
+            s_piv[par] = gkey.valid() ? gkey.row() : k;
+            s_gw[par] = gm ? __builtin_ctzll(gm) : 0;
+          }
+        }
+        __syncthreads();
+        piv = s_piv[par];
       } else {
         // 3. exchange between workgroups (wavefront 0 of every workgroup):
         //    publish this workgroup's key and row (and row k by its owner),
         //    poll the P keys, then fetch the winner's row and row k
         const unsigned tag = p.epoch * 64u + (unsigned)step + 1u;
         if (wave == 0) {
+          PivKey<T> mk;
+          mk.clear();
+          if (lane < PANEL_NW) mk = wkey[par][lane];
+          PivKey<T> gkey = mk;
+          gkey.wave_reduce();
+          const unsigned long long gm =
+              __ballot(lane < PANEL_NW && mk.valid() && gkey.valid() && mk.row() == gkey.row());
+          const int gw = gm ? __builtin_ctzll(gm) : 0;
           uint64_t *mine = p.rec + ((long)par * P + b) * REC;
           if (lane < KW)
             __hip_atomic_store(mine + lane, tagged(tag, gkey.word(lane)), __ATOMIC_RELAXED,
@@ -358,7 +418,7 @@ __global__ __launch_bounds__(PANEL_NT) void panel_gepp_kernel(PanelP<T> p) {
       const T *pr;
       const T *dr;
       if (P == 1) {
-        pr = (piv == k) ? krow[par] : wrow[par][gw];
+        pr = (piv == k) ? krow[par] : wrow[par][s_gw[par]];
         dr = krow[par];
       } else {
         pr = (piv == k) ? k_row : win_row;
@@ -377,33 +437,90 @@ __global__ __launch_bounds__(PANEL_NT) void panel_gepp_kernel(PanelP<T> p) {
           }
         }
       }
-      // 5. scale and rank-1 update of the columns still to factor
-      const T pv = pr[0];
+      // 5. scale (1/pivot, then multiply) and rank-1 update of the columns
+      //    still to factor, fused with the rotation: column c lands in slot
+      //    c-1 and column 0 (now L) in the last slot
+      // the pivot row into registers with 16-byte LDS reads (pr is 16-byte aligned)
+      T pra[WMAX];
+      if constexpr (sizeof(T) == 4) {
+        const float4 *p4 = reinterpret_cast<const float4 *>(pr);
+#pragma unroll
+        for (int i = 0; i < WMAX / 4; ++i) {
+          const float4 v = p4[i];
+          pra[4 * i] = v.x;
+          pra[4 * i + 1] = v.y;
+          pra[4 * i + 2] = v.z;
+          pra[4 * i + 3] = v.w;
+        }
+      } else {
+        const double2 *p2 = reinterpret_cast<const double2 *>(pr);
+#pragma unroll
+        for (int i = 0; i < WMAX / 2; ++i) {
+          const double2 v = p2[i];
+          pra[2 * i] = v.x;
+          pra[2 * i + 1] = v.y;
+        }
+      }
+      const T pv = pra[0];
       if (!((T)fabs(pv) < p.eps)) {
         const T inv = T(1) / pv;
-        const int rem = w - j;  // slots 1..rem-1 hold unfinished columns
+        const int rem = w - j;  // columns 1..rem-1 are unfinished
+        T lq[RPT], a0[RPT];
+        bool below[RPT], all_below = true;
 #pragma unroll
         for (int q = 0; q < RPT; ++q) {
-          if (row[q] > k && row[q] < rend) {
-            const T l = a[q][0] * inv;
-            a[q][0] = l;
+          below[q] = row[q] > k && row[q] < rend;
+          all_below = all_below && below[q];
+          const T l = a[q][0] * inv;
+          a0[q] = below[q] ? l : a[q][0];
+          lq[q] = l;
+        }
+        // finished columns (c >= rem) and rows at or above k (and padding
+        // rows) keep their values exactly: selects, not branches
+        if (__all(all_below)) {
 #pragma unroll
-            for (int c = 1; c < WMAX; ++c) {
-              const T lm = (c < rem) ? l : T(0);  // finished slots keep their L values
-              a[q][c] = fma(-pr[c], lm, a[q][c]);
+          for (int c = 1; c < WMAX; ++c) {
+            const bool live = c < rem;
+            const T prc = pra[c];
+#pragma unroll
+            for (int q = 0; q < RPT; ++q) {
+              const T u = fma(-prc, lq[q], a[q][c]);
+              a[q][c - 1] = live ? u : a[q][c];
+            }
+          }
+        } else {
+#pragma unroll
+          for (int c = 1; c < WMAX; ++c) {
+            const bool live = c < rem;
+            const T prc = pra[c];
+#pragma unroll
+            for (int q = 0; q < RPT; ++q) {
+              const T u = fma(-prc, lq[q], a[q][c]);
+              a[q][c - 1] = (live && below[q]) ? u : a[q][c];
             }
           }
         }
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) a[q][WMAX - 1] = a0[q];
+      } else {
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+          const T f = a[q][0];
+#pragma unroll
+          for (int c = 0; c + 1 < WMAX; ++c) a[q][c] = a[q][c + 1];
+          a[q][WMAX - 1] = f;
+        }
       }
       ++step;
-    }
-    // 6. rotate: the finished column moves to the tail
+    } else {
+      // 6. column not pivoted (k >= n-1): rotate only
 #pragma unroll
-    for (int q = 0; q < RPT; ++q) {
-      const T f = a[q][0];
+      for (int q = 0; q < RPT; ++q) {
+        const T f = a[q][0];
 #pragma unroll
-      for (int c = 0; c + 1 < WMAX; ++c) a[q][c] = a[q][c + 1];
-      a[q][WMAX - 1] = f;
+        for (int c = 0; c + 1 < WMAX; ++c) a[q][c] = a[q][c + 1];
+        a[q][WMAX - 1] = f;
+      }
     }
   }
 #pragma unroll
@@ -427,8 +544,8 @@ __global__ __launch_bounds__(PANEL_NT) void panel_gepp_kernel(PanelP<T> p) {
 // rows; then each thread moves its column's touched entries: all reads
 // first, then all writes, so the permutation is applied in one pass.
 template <typename T, int WMAX>
-__global__ __launch_bounds__(256) void rowswap_kernel(T *W, long ld, int ncols, const int *ipiv,
-                                                      int *perm, int k0, int w, int c0) {
+__global__ __launch_bounds__(256) void rowswap_kernel(T *W, long ld, int cl, int ch, const int *ipiv,
+                                                      int *perm, int k0, int w, int skip0) {
   static_assert(WMAX <= 32, "window and out-of-window rows each use <= 32 lanes");
   __shared__ int dst[2 * WMAX], src[2 * WMAX];
   __shared__ int npair;
@@ -483,16 +600,16 @@ __global__ __launch_bounds__(256) void rowswap_kernel(T *W, long ld, int ncols, 
   __syncthreads();
   const int np = npair;
   if (np == 0) return;
-  if (blockIdx.x == 0 && t < 64) {
+  if (perm && blockIdx.x == 0 && t < 64) {
     // the row-index vector moves with the rows (libmat's Perm)
     int pv = 0;
     if (t < np) pv = perm[src[t]];
     __builtin_amdgcn_wave_barrier();
     if (t < np) perm[dst[t]] = pv;
   }
-  int c = blockIdx.x * 256 + t;
-  if (c >= c0) c += w;  // skip the panel's own columns
-  if (c >= ncols) return;
+  int c = cl + blockIdx.x * 256 + t;
+  if (skip0 >= 0 && c >= skip0) c += w;  // skip the leaf's own columns
+  if (c >= ch) return;
   T v[2 * WMAX];
 #pragma unroll
   for (int x = 0; x < 2 * WMAX; ++x)
@@ -510,14 +627,34 @@ static void launch_panel(hipStream_t s, PanelP<T> &p) {
   // At most 32 workgroups of 512 threads: all resident at once on an idle
   // device (one per CU), which the exchange needs; a launch that could not
   // be resident would end in the bounded poll's timeout, not in a hang.
+  KScope ks(s, "panel", 2.0 * p.m * p.w * sizeof(T));
   hipLaunchKernelGGL((panel_gepp_kernel<T, WMAX, RPT>), dim3(p.nwg), dim3(PANEL_NT), 0, s, p);
   MG_HIP(hipGetLastError());
 }
 
-// Factor the leaf block W[c0:n, c0:c0+w] (w <= 32) and apply its row swaps
-// to all other columns of the n x n row-major matrix (and to perm).
+// Apply the row interchanges ipiv[k0 .. k0+w) to columns [cl, ch) of the
+// row-major matrix, leaving out the leaf's own columns [k0, k0+w) when they
+// lie in the range, and to perm (if not null).
 template <typename T>
-void lu_leaf(hipStream_t s, int n, T *W, long ld, int c0, int w, int *ipiv, int *perm, T eps) {
+void lu_rowswap(hipStream_t s, T *W, long ld, const int *ipiv, int *perm, int k0, int w, int cl,
+                int ch) {
+  constexpr int WMAX = 32;
+  MG_CHECK(w <= WMAX, "lu_rowswap: at most 32 interchanges per call");
+  const int skip0 = (k0 >= cl && k0 + w <= ch) ? k0 : -1;
+  const int count = (ch - cl) - (skip0 >= 0 ? w : 0);
+  if (count <= 0 && !perm) return;
+  KScope ks(s, "rowswap");
+  hipLaunchKernelGGL((rowswap_kernel<T, WMAX>), dim3(count > 0 ? ceil_div(count, 256) : 1),
+                     dim3(256), 0, s, W, ld, cl, ch, ipiv, perm, k0, w, skip0);
+  MG_HIP(hipGetLastError());
+}
+
+// Factor the leaf block W[c0:n, c0:c0+w] (w <= 32) and apply its row swaps
+// to the columns [cl, ch) of the n x n row-major matrix outside the leaf
+// (and to perm).
+template <typename T>
+void lu_leaf(hipStream_t s, int n, T *W, long ld, int c0, int w, int *ipiv, int *perm, T eps, int cl,
+             int ch) {
   constexpr int WMAX = 32;
   constexpr int REC = 4 + WMAX * Words<T>::N;
   MG_CHECK(w <= WMAX, "lu_leaf: leaf too wide");
@@ -550,13 +687,13 @@ void lu_leaf(hipStream_t s, int n, T *W, long ld, int c0, int w, int *ipiv, int 
   } else {
     launch_panel<T, WMAX, 1>(s, p);
   }
-  const int other = n - w;
-  hipLaunchKernelGGL((rowswap_kernel<T, WMAX>), dim3(other > 0 ? ceil_div(other, 256) : 1),
-                     dim3(256), 0, s, W, ld, n, ipiv, perm, c0, w, c0);
-  MG_HIP(hipGetLastError());
+  lu_rowswap<T>(s, W, ld, ipiv, perm, c0, w, cl, ch);
 }
 
-template void lu_leaf<float>(hipStream_t, int, float *, long, int, int, int *, int *, float);
-template void lu_leaf<double>(hipStream_t, int, double *, long, int, int, int *, int *, double);
+template void lu_leaf<float>(hipStream_t, int, float *, long, int, int, int *, int *, float, int, int);
+template void lu_leaf<double>(hipStream_t, int, double *, long, int, int, int *, int *, double, int,
+                              int);
+template void lu_rowswap<float>(hipStream_t, float *, long, const int *, int *, int, int, int, int);
+template void lu_rowswap<double>(hipStream_t, double *, long, const int *, int *, int, int, int, int);
 
 }  // namespace mg

@@ -1,6 +1,10 @@
 // Host runtime: error strings, per-stream device workspaces, the library's
 // own stream for the synchronous drop-in entry points.
+#include <atomic>
+#include <cstdlib>
 #include <map>
+#include <mutex>
+#include <vector>
 
 #include "common.hpp"
 
@@ -73,6 +77,124 @@ void sync_and_check(hipStream_t s) {
     MG_HIP(hipStreamSynchronize(s));
     throw Error{-5, "a multi-workgroup kernel timed out waiting for another workgroup"};
   }
+}
+
+namespace {
+struct TimeRec {
+  std::string family;
+  hipEvent_t e0, e1;
+  double work;
+};
+std::mutex g_tm_mutex;
+std::atomic<bool> g_tm_on{false};
+std::vector<TimeRec> g_tm_recs;
+std::vector<hipEvent_t> g_tm_pool;
+
+hipEvent_t tm_event() {
+  if (!g_tm_pool.empty()) {
+    hipEvent_t e = g_tm_pool.back();
+    g_tm_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  MG_HIP(hipEventCreate(&e));
+  return e;
+}
+}  // namespace
+
+KScope::KScope(hipStream_t s_, const char *family, double work) : s(s_), slot(-1) {
+  if (!g_tm_on.load(std::memory_order_relaxed)) return;
+  std::lock_guard<std::mutex> lk(g_tm_mutex);
+  TimeRec r{family, tm_event(), tm_event(), work};
+  MG_HIP(hipEventRecord(r.e0, s));
+  slot = (long)g_tm_recs.size();
+  g_tm_recs.push_back(r);
+}
+
+KScope::~KScope() {
+  if (slot < 0) return;
+  std::lock_guard<std::mutex> lk(g_tm_mutex);
+  if (slot < (long)g_tm_recs.size()) (void)hipEventRecord(g_tm_recs[slot].e1, s);
+}
+
+void timing_enable(bool on) {
+  std::lock_guard<std::mutex> lk(g_tm_mutex);
+  if (on) {
+    MG_HIP(hipDeviceSynchronize());  // recorded events may still be pending
+    for (auto &r : g_tm_recs) {
+      g_tm_pool.push_back(r.e0);
+      g_tm_pool.push_back(r.e1);
+    }
+    g_tm_recs.clear();
+  }
+  g_tm_on.store(on);
+}
+
+void timing_query(const char *family, long *launches, double *ms, double *work) {
+  std::lock_guard<std::mutex> lk(g_tm_mutex);
+  long c = 0;
+  double t = 0.0, w = 0.0;
+  for (auto &r : g_tm_recs) {
+    if (r.family != family) continue;
+    MG_HIP(hipEventSynchronize(r.e1));
+    float e = 0.0f;
+    MG_HIP(hipEventElapsedTime(&e, r.e0, r.e1));
+    ++c;
+    t += e;
+    w += r.work;
+  }
+  if (launches) *launches = c;
+  if (ms) *ms = t;
+  if (work) *work = w;
+}
+
+hipStream_t side_stream() {
+  static std::mutex m;
+  static std::map<int, hipStream_t> streams;
+  int dev = 0;
+  MG_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(m);
+  auto it = streams.find(dev);
+  if (it != streams.end()) return it->second;
+  hipStream_t st = nullptr;
+  int keep = 32;
+  if (const char *e = getenv("MATGPU_PANEL_CUS")) keep = atoi(e);
+  const int cus = num_cus();
+  if (keep > 0 && keep < cus) {
+    // mask off the first `keep` CUs of the side stream
+    std::vector<uint32_t> mask((cus + 31) / 32, 0u);
+    for (int c = keep; c < cus; ++c) mask[c / 32] |= 1u << (c % 32);
+    if (hipExtStreamCreateWithCUMask(&st, (uint32_t)mask.size(), mask.data()) != hipSuccess) {
+      (void)hipGetLastError();
+      st = nullptr;
+    }
+  }
+  if (!st) MG_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  streams[dev] = st;
+  return st;
+}
+
+hipEvent_t pooled_event(int i) {
+  static std::mutex m;
+  static std::map<int, std::vector<hipEvent_t>> pools;
+  int dev = 0;
+  MG_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(m);
+  auto &v = pools[dev];
+  while ((int)v.size() <= i) {
+    hipEvent_t e;
+    MG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    v.push_back(e);
+  }
+  return v[i];
+}
+
+int lookahead_block() {
+  static int nb = [] {
+    const char *e = getenv("MATGPU_LU_NB");
+    return e ? atoi(e) : 256;
+  }();
+  return nb;
 }
 
 hipStream_t default_stream() {

@@ -504,23 +504,30 @@ int posv_mixed(hipStream_t s, int n, int nrhs, const double *A, long lda, const 
   float *D = (float *)ws.get("posvm.D", (size_t)n * nrhs * 4);
 
   const double normA = norm_inf_f64(s, n, A, lda);
-  convert2d<double, float>(s, n, n, A, lda, L, n);
-  potrf_lower<float>(s, n, L, n, info, o.passes == 0 ? Mul::Native : (Mul)o.passes);
-  int h = 0;
-  MG_HIP(hipMemcpyAsync(&h, info, 4, hipMemcpyDeviceToHost, s));
-  sync_and_check(s);
-  const double t_factor = ms_since(t0);
-  int it = 0;
-  double berr = INFINITY;
-  bool ok = false;
   BerrEval be(s, n, nrhs, normA, B, ldb);
   TriPair<float> tp = chol_pair<float>(n, L, n);
-  if (h == 0) {
+  // precision ladder: the requested updates, then (after fp8) 3-pass bf16,
+  // then the plain fp64 solver below
+  const int levels[2] = {o.passes, o.passes == 8 ? 3 : -1};
+  int it = 0, used = o.passes;
+  double berr = INFINITY, t_factor = 0.0;
+  bool ok = false;
+  for (int li = 0; li < 2 && !ok && levels[li] >= 0; ++li) {
+    const int ps = levels[li];
+    convert2d<double, float>(s, n, n, A, lda, L, n);
+    potrf_lower<float>(s, n, L, n, info, ps == 0 ? Mul::Native : (Mul)ps);
+    int h = 0;
+    MG_HIP(hipMemcpyAsync(&h, info, 4, hipMemcpyDeviceToHost, s));
+    sync_and_check(s);
+    if (li == 0) t_factor = ms_since(t0);
+    if (h != 0) continue;  // not positive definite at this precision
+    used = ps;
     tp.prepare(s, "posvm");
     convert2d<double, float>(s, n, nrhs, B, ldb, D, n);
     tp.solve(s, nrhs, D, n);
     convert2d<float, double>(s, n, nrhs, D, n, X, ldx);
     double prev = INFINITY, prev2 = INFINITY;
+    int lit = 0;
     while (true) {
       residual_f64(s, n, nrhs, A, lda, X, ldx, B, ldb, R, n);
       berr = be.eval(R, n, X, ldx);
@@ -528,8 +535,8 @@ int posv_mixed(hipStream_t s, int n, int nrhs, const double *A, long lda, const 
         ok = true;
         break;
       }
-      const bool stalled = !(berr == berr) || it >= o.max_iter ||
-                           (it >= 2 && berr > 0.5 * prev2) || (it >= 1 && berr > 2.0 * prev);
+      const bool stalled = !(berr == berr) || lit >= o.max_iter ||
+                           (lit >= 2 && berr > 0.5 * prev2) || (lit >= 1 && berr > 2.0 * prev);
       if (stalled) break;
       prev2 = prev;
       prev = berr;
@@ -538,6 +545,7 @@ int posv_mixed(hipStream_t s, int n, int nrhs, const double *A, long lda, const 
       hipLaunchKernelGGL(axpy_f32_to_f64_kernel, dim3(ceil_div((size_t)n * nrhs, 256)), dim3(256), 0,
                          s, n, nrhs, D, n, X, ldx);
       MG_HIP(hipGetLastError());
+      ++lit;
       ++it;
     }
   }
@@ -546,7 +554,7 @@ int posv_mixed(hipStream_t s, int n, int nrhs, const double *A, long lda, const 
     st->fell_back = 0;
     st->backward_error = berr;
     st->factor_ms = t_factor;
-    st->precision = o.passes;
+    st->precision = used;
   }
   int rc = 0;
   if (!ok) {

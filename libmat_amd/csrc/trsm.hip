@@ -93,11 +93,14 @@ template <typename T>
 static void gemm_sub(hipStream_t s, long M, long N, long K, View<const T> A, View<const T> B,
                      View<T> C, Mul mul) {
   if constexpr (std::is_same<T, float>::value) {
-    if (mul == Mul::Bf16x1 || mul == Mul::Bf16x3 || mul == Mul::Bf16x6) {
+    // small products: native fp32 MFMA is as fast as packing for the split
+    // passes (and exact to fp32); the split pays off on large updates only
+    if ((mul == Mul::Bf16x1 || mul == Mul::Bf16x3 || mul == Mul::Bf16x6) &&
+        (double)M * N * K >= kLowpMinWork) {
       gemm_bf16x(s, M, N, K, -1.0f, A, B, C, (int)mul);
       return;
     }
-    if (mul == Mul::Fp8) {
+    if (mul == Mul::Fp8 && (double)M * N * K >= kLowpMinWork) {
       gemm_fp8(s, M, N, K, -1.0f, A, B, C, false);
       return;
     }
@@ -144,6 +147,7 @@ void trsm_left(hipStream_t s, bool lower, bool unit, int n, int nrhs, View<const
   if (n <= TRSM_TB && nrhs <= 4096) {
     // small: one launch of the register-resident leaf
     dim3 grid(ceil_div(nrhs, TRSM_NT)), blk(TRSM_NT);
+    KScope ks(s, "trsm");
     if (lower && unit) hipLaunchKernelGGL((trsm_leaf_kernel<T, true, true>), grid, blk, 0, s, n, nrhs, Tm, B);
     else if (lower) hipLaunchKernelGGL((trsm_leaf_kernel<T, true, false>), grid, blk, 0, s, n, nrhs, Tm, B);
     else if (unit) hipLaunchKernelGGL((trsm_leaf_kernel<T, false, true>), grid, blk, 0, s, n, nrhs, Tm, B);

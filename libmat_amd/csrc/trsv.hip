@@ -390,6 +390,7 @@ void tri_invert_blocks(hipStream_t s, bool lower, bool unit, int n, View<const T
   constexpr int TB = TriTB<T>::TB;
   const int nb = (n + TB - 1) / TB;
   if (nb == 0) return;
+  KScope ks(s, "trsm");
   if (lower && unit) hipLaunchKernelGGL((tri_inv_kernel<T, true, true>), dim3(nb), dim3(256), 0, s, n, Tm, D);
   else if (lower) hipLaunchKernelGGL((tri_inv_kernel<T, true, false>), dim3(nb), dim3(256), 0, s, n, Tm, D);
   else if (unit) hipLaunchKernelGGL((tri_inv_kernel<T, false, true>), dim3(nb), dim3(256), 0, s, n, Tm, D);
@@ -422,6 +423,7 @@ void tri_solve_blocked(hipStream_t s, bool lower, int n, int nrhs, View<const T>
     unsigned tag = ++g_chain_launches;
     if (tag == 0) tag = ++g_chain_launches;
     fill_u32(s, claim, 1, 0u);
+    KScope ks(s, "tri_chain", ((double)n * n / 2 + 2.0 * n * nc) * sizeof(T));
     if (lower)
       hipLaunchKernelGGL((tri_chain_kernel<T, true>), dim3(grid), dim3(ChainNT<T>::NT), 0, s, n, nc, Tm,
                          D, X + (long)cb * ldx, ldx, claim, pub, tag, err);

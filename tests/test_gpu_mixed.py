@@ -69,6 +69,78 @@ def test_gesv_mixed(kappa, passes, pivoting):
         assert st.fell_back == 0
 
 
+def conditioned_dev(n, kappa, seed, spd=False):
+    """As conditioned(), generated on the GPU (column-major fp64 buffer as a
+    flat tensor; the matrix is the transpose of the row-major product, which
+    is a member of the same family)."""
+    gen = torch.Generator(device="cuda").manual_seed(seed)
+    U, _ = torch.linalg.qr(torch.randn(n, n, dtype=torch.float64, device="cuda", generator=gen))
+    if spd:
+        lam = torch.exp(torch.rand(n, dtype=torch.float64, device="cuda", generator=gen)
+                        * np.log(kappa))
+        A = (U * lam) @ U.T
+        A = (A + A.T) / 2
+    else:
+        V, _ = torch.linalg.qr(torch.randn(n, n, dtype=torch.float64, device="cuda", generator=gen))
+        sig = torch.logspace(0, -np.log10(kappa), n, dtype=torch.float64, device="cuda")
+        A = (U * sig) @ V.T
+    return A.contiguous().reshape(-1)
+
+
+def run_dev(kind, dA, n, B, passes):
+    g = libmat_amd.load("f64")
+    nr = B.shape[1]
+    opts = libmat_amd.IrOpts()
+    g.lib.mat_gpu_ir_defaults(C.byref(opts), 1 if kind == "posv" else 0)
+    opts.passes = passes
+    st = libmat_amd.IrStatus()
+    dB = to_dev(B, np.float64)
+    dX = torch.zeros(n * nr, dtype=torch.float64, device="cuda")
+    rc = getattr(g.lib, f"mat_gpu_{kind}_mixed")(stream(), n, nr, ptr(dA), n, ptr(dB), n, ptr(dX),
+                                                 n, C.byref(opts), C.byref(st))
+    g.check(rc, kind)
+    torch.cuda.synchronize()
+    A = dA.reshape(n, n).T  # logical column-major matrix
+    X = dX.reshape(nr, n).T
+    Bm = dB.reshape(nr, n).T
+    R = A @ X - Bm
+    be = float((R.abs().amax(0) / (A.abs().sum(1).max() * X.abs().amax(0) + Bm.abs().amax(0))).max())
+    Xref = torch.linalg.solve(A, Bm)
+    fe = float((X - Xref).abs().max() / Xref.abs().max())
+    return rc, st, be, fe
+
+
+@pytest.mark.parametrize("kappa", [1e2, 1e6])
+@pytest.mark.parametrize("passes", [3, 6])
+def test_gesv_mixed_split_updates(kappa, passes):
+    """n = 4096: the large trailing updates run as split-BF16 MFMA GEMMs."""
+    n = 4096
+    dA = conditioned_dev(n, kappa, seed=11)
+    B = np.random.default_rng(5).uniform(-1, 1, (n, 16))
+    rc, st, be, fe = run_dev("gesv", dA, n, B, passes)
+    print(f"n={n} kappa={kappa:g} passes={passes} iters={st.iterations} fell_back={st.fell_back} "
+          f"berr={be:.2e} ferr={fe:.2e}")
+    assert rc == 0 and be <= 1e-14 and fe <= kappa * 1e-13
+    if passes == 6:
+        assert st.fell_back == 0
+
+
+@pytest.mark.parametrize("kappa", [1e2, 1e3])
+def test_posv_mixed_fp8_updates(kappa):
+    """n = 4096: the top-level trailing update runs in FP8 (tile-scaled); the
+    ladder may refactor with 3-pass BF16 before the fp64 fallback."""
+    n = 4096
+    dA = conditioned_dev(n, kappa, seed=13, spd=True)
+    B = np.random.default_rng(6).standard_normal((n, 4))
+    rc, st, be, fe = run_dev("posv", dA, n, B, 8)
+    print(f"fp8 n={n} kappa={kappa:g} iters={st.iterations} precision={st.precision} "
+          f"fell_back={st.fell_back} berr={be:.2e} ferr={fe:.2e}")
+    assert rc == 0 and be <= 1e-14 and fe <= kappa * 1e-13
+    assert st.precision in (8, 3)
+    if kappa <= 1e2:
+        assert st.fell_back == 0
+
+
 def test_gesv_mixed_forced_stall_falls_back():
     n = 512
     A = conditioned(n, 1e6, seed=3)
