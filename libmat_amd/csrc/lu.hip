@@ -21,11 +21,12 @@
 namespace mg {
 
 template <typename T>
-void lu_leaf(hipStream_t s, int n, T *W, long ld, int c0, int w, int *ipiv, int *perm, T eps, int cl,
-             int ch);
+void lu_leaf(hipStream_t s, int n, T *W, long ld, int c0, int w, int *ipiv, int *perm, int *pairs,
+             T eps, int cl, int ch);
 template <typename T>
-void lu_rowswap(hipStream_t s, T *W, long ld, const int *ipiv, int *perm, int k0, int w, int cl,
+void lu_rowswap(hipStream_t s, T *W, long ld, const int *pairs, int *perm, int k0, int w, int cl,
                 int ch);
+constexpr int kPairStride = 129;  // ints per leaf in the row-pair buffer (panel.hip)
 
 template <typename T>
 static void lu_gemm_sub(hipStream_t s, long M, long N, long K, View<const T> A, View<const T> B,
@@ -46,15 +47,15 @@ static void lu_gemm_sub(hipStream_t s, long M, long N, long K, View<const T> A, 
 // the leaves' row interchanges reach columns [cl, ch).
 template <typename T>
 static void getrf_rec(hipStream_t s, int n, T *W, long ld, int c0, int w, int *ipiv, int *perm,
-                      T eps, const LuOpts &o, int cl, int ch) {
+                      int *pairs, T eps, const LuOpts &o, int cl, int ch) {
   if (w <= o.leaf) {
-    lu_leaf<T>(s, n, W, ld, c0, w, ipiv, perm, eps, cl, ch);
+    lu_leaf<T>(s, n, W, ld, c0, w, ipiv, perm, pairs, eps, cl, ch);
     return;
   }
   int w1 = ((w / 2 + o.leaf - 1) / o.leaf) * o.leaf;
   if (w1 >= w) w1 = w - o.leaf;
   const int w2 = w - w1;
-  getrf_rec<T>(s, n, W, ld, c0, w1, ipiv, perm, eps, o, cl, ch);
+  getrf_rec<T>(s, n, W, ld, c0, w1, ipiv, perm, pairs, eps, o, cl, ch);
   // U12 = L11^-1 A12 (unit lower)
   View<const T> L11{W + (long)c0 * ld + c0, ld, 1};
   View<T> A12{W + (long)c0 * ld + c0 + w1, ld, 1};
@@ -64,16 +65,17 @@ static void getrf_rec(hipStream_t s, int n, T *W, long ld, int c0, int w, int *i
   View<const T> L21{W + (long)(c0 + w1) * ld + c0, ld, 1};
   View<T> A22{W + (long)(c0 + w1) * ld + c0 + w1, ld, 1};
   lu_gemm_sub<T>(s, m2, w2, w1, L21, cview(A12), A22, o.mul);
-  getrf_rec<T>(s, n, W, ld, c0 + w1, w2, ipiv, perm, eps, o, cl, ch);
+  getrf_rec<T>(s, n, W, ld, c0 + w1, w2, ipiv, perm, pairs, eps, o, cl, ch);
 }
 
 // Row interchanges of the factored block [b0, b1) (its leaves in order)
 // applied to columns [cl, ch).
 template <typename T>
-static void block_swaps(hipStream_t s, T *W, long ld, const int *ipiv, int b0, int b1, int leaf, int cl,
-                        int ch) {
+static void block_swaps(hipStream_t s, T *W, long ld, const int *pairs, int b0, int b1, int leaf,
+                        int cl, int ch) {
   if (ch <= cl) return;
-  for (int k = b0; k < b1; k += leaf) lu_rowswap<T>(s, W, ld, ipiv, nullptr, k, std::min(leaf, b1 - k), cl, ch);
+  for (int k = b0; k < b1; k += leaf)
+    lu_rowswap<T>(s, W, ld, pairs, nullptr, k, std::min(leaf, b1 - k), cl, ch);
 }
 
 // U12 = L11^-1 A12 and A22 -= L21 U12 for the factored block [b0, b1) and
@@ -105,8 +107,8 @@ static void block_update(hipStream_t s, int n, T *W, long ld, int b0, int b1, in
 //            [0, b0)   after panel(k) is done
 // and interchanges of different blocks reach any column in pivot order.
 template <typename T>
-static void getrf_lookahead(hipStream_t s, int n, T *W, long ld, int *ipiv, int *perm, T eps,
-                            const LuOpts &o, int NB) {
+static void getrf_lookahead(hipStream_t s, int n, T *W, long ld, int *ipiv, int *perm, int *pairs,
+                            T eps, const LuOpts &o, int NB) {
   // the side stream and the event pool are per device: one enqueue at a time
   static std::mutex mu;
   std::lock_guard<std::mutex> lk(mu);
@@ -121,7 +123,7 @@ static void getrf_lookahead(hipStream_t s, int n, T *W, long ld, int *ipiv, int 
   hipEvent_t start = pooled_event(2 * nblk);
   MG_HIP(hipEventRecord(start, s));
   MG_HIP(hipStreamWaitEvent(u, start, 0));
-  getrf_rec<T>(s, n, W, ld, 0, std::min(NB, n), ipiv, perm, eps, o, 0, std::min(NB, n));
+  getrf_rec<T>(s, n, W, ld, 0, std::min(NB, n), ipiv, perm, pairs, eps, o, 0, std::min(NB, n));
   for (int k = 0; k < nblk; ++k) {
     const int b0 = k * NB, b1 = std::min(n, b0 + NB);
     const int c2 = std::min(n, b1 + NB);
@@ -129,19 +131,19 @@ static void getrf_lookahead(hipStream_t s, int n, T *W, long ld, int *ipiv, int 
     if (b1 < n) {
       // caller: block k+1's columns, then its panel
       if (k >= 1) MG_HIP(hipStreamWaitEvent(s, evU[k - 1], 0));
-      block_swaps<T>(s, W, ld, ipiv, b0, b1, o.leaf, b1, c2);
+      block_swaps<T>(s, W, ld, pairs, b0, b1, o.leaf, b1, c2);
       block_update<T>(s, n, W, ld, b0, b1, b1, c2, o);
-      getrf_rec<T>(s, n, W, ld, b1, c2 - b1, ipiv, perm, eps, o, b1, c2);
+      getrf_rec<T>(s, n, W, ld, b1, c2 - b1, ipiv, perm, pairs, eps, o, b1, c2);
     }
     // side: the rest of the trailing matrix, block k+2's columns first (the
     // only part the caller's next step waits for), then the columns on the left
     MG_HIP(hipStreamWaitEvent(u, evP[k], 0));
     const int c3 = std::min(n, c2 + NB);
-    block_swaps<T>(u, W, ld, ipiv, b0, b1, o.leaf, c2, n);
+    block_swaps<T>(u, W, ld, pairs, b0, b1, o.leaf, c2, n);
     block_update<T>(u, n, W, ld, b0, b1, c2, c3, o);
     MG_HIP(hipEventRecord(evU[k], u));
     block_update<T>(u, n, W, ld, b0, b1, c3, n, o);
-    block_swaps<T>(u, W, ld, ipiv, b0, b1, o.leaf, 0, b0);
+    block_swaps<T>(u, W, ld, pairs, b0, b1, o.leaf, 0, b0);
   }
   hipEvent_t done = pooled_event(2 * nblk + 1);
   MG_HIP(hipEventRecord(done, u));
@@ -162,11 +164,13 @@ void getrf_rowmajor_perm(hipStream_t s, int n, T *W, long ld, int *ipiv, int *pe
   hipLaunchKernelGGL(iota_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, s, ipiv, n);
   hipLaunchKernelGGL(iota_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, s, perm, n);
   MG_HIP(hipGetLastError());
+  // every leaf's interchanges as row pairs, written by its panel kernel
+  int *pairs = (int *)workspace(s).get("getrf.pairs", (size_t)n * kPairStride * sizeof(int));
   const int NB = lookahead_block();
   if (NB > 0 && n >= 4 * NB && NB % o.leaf == 0)
-    getrf_lookahead<T>(s, n, W, ld, ipiv, perm, eps, o, NB);
+    getrf_lookahead<T>(s, n, W, ld, ipiv, perm, pairs, eps, o, NB);
   else
-    getrf_rec<T>(s, n, W, ld, 0, n, ipiv, perm, eps, o, 0, n);
+    getrf_rec<T>(s, n, W, ld, 0, n, ipiv, perm, pairs, eps, o, 0, n);
 }
 
 template <typename T>

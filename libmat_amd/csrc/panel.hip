@@ -52,7 +52,62 @@ struct PanelP {
   uint64_t *rec;         // [2][nwg][REC] tagged words: value, row, row values
   uint64_t *krec;        // [2][KREC] tagged words: row k
   int *err;              // set to 1 on a poll timeout
+  int *pairs;            // this leaf's interchanges as row pairs (see swap_pairs)
 };
+
+// A leaf's interchanges as one permutation of the touched rows, stored as
+// pairs[0] = count, pairs[1 ..] = destination rows, pairs[1 + PAIR_MAX ..]
+// = source rows: row dst[i] receives the original content of row src[i].
+constexpr int PAIR_MAX = 64;                  // 2 * widest leaf
+constexpr int PAIR_STRIDE = 1 + 2 * PAIR_MAX;  // ints per leaf
+
+// One full wavefront turns the swap sequence piv[0 .. w) (step kk swaps
+// rows k0+kk and piv[kk]) into pairs.  Lane i tracks window row k0+i (the
+// original row now there); lane x also tracks the x-th distinct row below
+// the window.  Every index is wave-uniform, so lane reads are v_readlane.
+__device__ inline void swap_pairs(int my_piv, int k0, int w, int *out) {
+  const int lane = threadIdx.x & 63;
+  int my_win = k0 + lane, my_outr = -1, my_outc = -1, nout = 0;
+  for (int kk = 0; kk < w; ++kk) {
+    const int k = k0 + kk;
+    const int pr = __builtin_amdgcn_readlane(my_piv, kk);
+    if (pr == k) continue;
+    const int a = __builtin_amdgcn_readlane(my_win, kk);
+    int bsrc;
+    if (pr < k0 + w) {
+      bsrc = __builtin_amdgcn_readlane(my_win, pr - k0);
+      if (lane == pr - k0) my_win = a;
+    } else {
+      const unsigned long long m = __ballot(my_outr == pr);
+      int slot;
+      if (m) {
+        slot = __builtin_ctzll(m);
+      } else {
+        slot = nout++;
+        if (lane == slot) my_outr = my_outc = pr;
+      }
+      bsrc = __builtin_amdgcn_readlane(my_outc, slot);
+      if (lane == slot) my_outc = a;
+    }
+    if (lane == kk) my_win = bsrc;
+  }
+  const bool wpair = lane < w && my_win != k0 + lane;
+  const bool opair = lane < nout && my_outc != my_outr;
+  const unsigned long long mw = __ballot(wpair), mo = __ballot(opair);
+  const unsigned long long below = (1ull << lane) - 1ull;
+  const int nw = __popcll(mw);
+  if (wpair) {
+    const int at = __popcll(mw & below);
+    out[1 + at] = k0 + lane;
+    out[1 + PAIR_MAX + at] = my_win;
+  }
+  if (opair) {
+    const int at = nw + __popcll(mo & below);
+    out[1 + at] = my_outr;
+    out[1 + PAIR_MAX + at] = my_outc;
+  }
+  if (lane == 0) out[0] = nw + __popcll(mo);
+}
 
 template <typename T>
 __device__ inline void better(T &bv, int &bi, T v, int i) {
@@ -231,6 +286,7 @@ __global__ __launch_bounds__(PANEL_NT) void panel_gepp_kernel(PanelP<T> p) {
   __shared__ __align__(16) T krow[2][WMAX];            // row k (if this workgroup owns it)
   __shared__ __align__(16) uint32_t stage[2][2 * WMAX * NWD];  // winner row, row k (payload words)
   __shared__ int s_piv[2], s_gw[2];
+  __shared__ int s_hist[WMAX];  // pivot row of every step (k itself when not pivoted)
   __shared__ int s_abort;
 
   // Register rotation: at step j, slot 0 holds column j, slots 1..w-1-j the
@@ -425,6 +481,7 @@ __global__ __launch_bounds__(PANEL_NT) void panel_gepp_kernel(PanelP<T> p) {
         dr = k_row;
       }
       if (b == 0 && t == 0) p.ipiv[k] = piv;
+      if (t == 0) s_hist[j] = piv;
       if (piv != k) {
 #pragma unroll
         for (int q = 0; q < RPT; ++q) {
@@ -514,6 +571,7 @@ __global__ __launch_bounds__(PANEL_NT) void panel_gepp_kernel(PanelP<T> p) {
       ++step;
     } else {
       // 6. column not pivoted (k >= n-1): rotate only
+      if (t == 0) s_hist[j] = k;
 #pragma unroll
       for (int q = 0; q < RPT; ++q) {
         const T f = a[q][0];
@@ -535,67 +593,32 @@ __global__ __launch_bounds__(PANEL_NT) void panel_gepp_kernel(PanelP<T> p) {
       }
     }
   }
+  // the leaf's interchanges as row pairs, once, for every later row move
+  if (b == 0) {
+    __syncthreads();
+    if (wave == 0) swap_pairs(lane < w ? s_hist[lane] : 0, p.c0, w, p.pairs);
+  }
 }
 
-// Apply the row interchanges ipiv[k0 .. k0+w) (sequential swaps, global
-// row indices) to every column of the row-major matrix outside the panel
-// columns [c0, c0+w).  Wavefront 0 of each workgroup turns the swap
-// sequence into an explicit (destination, source) list over the touched
-// rows; then each thread moves its column's touched entries: all reads
-// first, then all writes, so the permutation is applied in one pass.
+// Apply a leaf's interchanges, given as row pairs by the panel kernel, to
+// the columns [cl, ch) outside the leaf's own columns: each thread moves
+// one column's touched entries, all reads first, then all writes, so the
+// permutation is applied in one pass.
 template <typename T, int WMAX>
-__global__ __launch_bounds__(256) void rowswap_kernel(T *W, long ld, int cl, int ch, const int *ipiv,
-                                                      int *perm, int k0, int w, int skip0) {
-  static_assert(WMAX <= 32, "window and out-of-window rows each use <= 32 lanes");
-  __shared__ int dst[2 * WMAX], src[2 * WMAX];
+__global__ __launch_bounds__(256) void rowswap_kernel(T *W, long ld, int cl, int ch, const int *pairs,
+                                                      int *perm, int w, int skip0) {
+  static_assert(2 * WMAX <= PAIR_MAX, "pairs hold at most PAIR_MAX rows");
+  __shared__ int dst[PAIR_MAX], src[PAIR_MAX];
   __shared__ int npair;
   const int t = threadIdx.x;
-  if (t < 64) {
-    // Lane i tracks window row k0+i (my_win = original row now there);
-    // lane x also tracks the x-th distinct row below the window.
-    const int lane = t;
-    // every index below is wave-uniform, so lane reads are v_readlane
-    int my_win = k0 + lane, my_outr = -1, my_outc = -1, nout = 0;
-    const int my_piv = lane < w ? ipiv[k0 + lane] : 0;
-    for (int kk = 0; kk < w; ++kk) {
-      const int k = k0 + kk;
-      const int pr = __builtin_amdgcn_readlane(my_piv, kk);
-      if (pr == k) continue;
-      const int a = __builtin_amdgcn_readlane(my_win, kk);
-      int bsrc;
-      if (pr < k0 + w) {
-        bsrc = __builtin_amdgcn_readlane(my_win, pr - k0);
-        if (lane == pr - k0) my_win = a;
-      } else {
-        const unsigned long long m = __ballot(my_outr == pr);
-        int slot;
-        if (m) {
-          slot = __builtin_ctzll(m);
-        } else {
-          slot = nout++;
-          if (lane == slot) my_outr = my_outc = pr;
-        }
-        bsrc = __builtin_amdgcn_readlane(my_outc, slot);
-        if (lane == slot) my_outc = a;
-      }
-      if (lane == kk) my_win = bsrc;
-    }
-    const bool wpair = lane < w && my_win != k0 + lane;
-    const bool opair = lane < nout && my_outc != my_outr;
-    const unsigned long long mw = __ballot(wpair), mo = __ballot(opair);
-    const unsigned long long below = (1ull << lane) - 1ull;
-    const int nw = __popcll(mw);
-    if (wpair) {
-      const int at = __popcll(mw & below);
-      dst[at] = k0 + lane;
-      src[at] = my_win;
-    }
-    if (opair) {
-      const int at = nw + __popcll(mo & below);
-      dst[at] = my_outr;
-      src[at] = my_outc;
-    }
-    if (lane == 0) npair = nw + __popcll(mo);
+  if (t < PAIR_MAX) {
+    // unused slots become (src[0] -> src[0]): the copy loops below then run
+    // without branches; see the store order there
+    const int np0 = pairs[0];
+    const int s0 = pairs[1 + PAIR_MAX];
+    dst[t] = t < np0 ? pairs[1 + t] : s0;
+    src[t] = t < np0 ? pairs[1 + PAIR_MAX + t] : s0;
+    if (t == 0) npair = np0;
   }
   __syncthreads();
   const int np = npair;
@@ -610,13 +633,16 @@ __global__ __launch_bounds__(256) void rowswap_kernel(T *W, long ld, int cl, int
   int c = cl + blockIdx.x * 256 + t;
   if (skip0 >= 0 && c >= skip0) c += w;  // skip the leaf's own columns
   if (c >= ch) return;
-  T v[2 * WMAX];
+  // all reads, then all writes.  Slots past np copy row src[0] onto itself;
+  // they are stored first (descending order), so every real destination is
+  // written after them and keeps its new value.
+  T *Wc = W + c;
+  const unsigned uld = (unsigned)ld;
+  T v[PAIR_MAX];
 #pragma unroll
-  for (int x = 0; x < 2 * WMAX; ++x)
-    if (x < np) v[x] = W[(long)src[x] * ld + c];
+  for (int x = 0; x < PAIR_MAX; ++x) v[x] = Wc[(unsigned long)(unsigned)src[x] * uld];
 #pragma unroll
-  for (int x = 0; x < 2 * WMAX; ++x)
-    if (x < np) W[(long)dst[x] * ld + c] = v[x];
+  for (int x = PAIR_MAX - 1; x >= 0; --x) Wc[(unsigned long)(unsigned)dst[x] * uld] = v[x];
 }
 
 
@@ -632,11 +658,12 @@ static void launch_panel(hipStream_t s, PanelP<T> &p) {
   MG_HIP(hipGetLastError());
 }
 
-// Apply the row interchanges ipiv[k0 .. k0+w) to columns [cl, ch) of the
-// row-major matrix, leaving out the leaf's own columns [k0, k0+w) when they
-// lie in the range, and to perm (if not null).
+// Apply the interchanges of the leaf starting at column k0 (its row pairs,
+// pairs + k0 * PAIR_STRIDE) to columns [cl, ch) of the row-major matrix,
+// leaving out the leaf's own columns [k0, k0+w) when they lie in the range,
+// and to perm (if not null).
 template <typename T>
-void lu_rowswap(hipStream_t s, T *W, long ld, const int *ipiv, int *perm, int k0, int w, int cl,
+void lu_rowswap(hipStream_t s, T *W, long ld, const int *pairs, int *perm, int k0, int w, int cl,
                 int ch) {
   constexpr int WMAX = 32;
   MG_CHECK(w <= WMAX, "lu_rowswap: at most 32 interchanges per call");
@@ -645,7 +672,7 @@ void lu_rowswap(hipStream_t s, T *W, long ld, const int *ipiv, int *perm, int k0
   if (count <= 0 && !perm) return;
   KScope ks(s, "rowswap");
   hipLaunchKernelGGL((rowswap_kernel<T, WMAX>), dim3(count > 0 ? ceil_div(count, 256) : 1),
-                     dim3(256), 0, s, W, ld, cl, ch, ipiv, perm, k0, w, skip0);
+                     dim3(256), 0, s, W, ld, cl, ch, pairs + (long)k0 * PAIR_STRIDE, perm, w, skip0);
   MG_HIP(hipGetLastError());
 }
 
@@ -653,8 +680,8 @@ void lu_rowswap(hipStream_t s, T *W, long ld, const int *ipiv, int *perm, int k0
 // to the columns [cl, ch) of the n x n row-major matrix outside the leaf
 // (and to perm).
 template <typename T>
-void lu_leaf(hipStream_t s, int n, T *W, long ld, int c0, int w, int *ipiv, int *perm, T eps, int cl,
-             int ch) {
+void lu_leaf(hipStream_t s, int n, T *W, long ld, int c0, int w, int *ipiv, int *perm, int *pairs,
+             T eps, int cl, int ch) {
   constexpr int WMAX = 32;
   constexpr int REC = 4 + WMAX * Words<T>::N;
   MG_CHECK(w <= WMAX, "lu_leaf: leaf too wide");
@@ -671,6 +698,7 @@ void lu_leaf(hipStream_t s, int n, T *W, long ld, int c0, int w, int *ipiv, int 
   p.kmax = n - 1;
   p.eps = eps;
   p.ipiv = ipiv;
+  p.pairs = pairs + (long)c0 * PAIR_STRIDE;
   const int maxwg = 32;
   p.rec = (uint64_t *)ws.get("panel.rec", 2L * maxwg * REC * 8);
   p.krec = (uint64_t *)ws.get("panel.krec", 2L * WMAX * Words<T>::N * 8);
@@ -687,12 +715,13 @@ void lu_leaf(hipStream_t s, int n, T *W, long ld, int c0, int w, int *ipiv, int 
   } else {
     launch_panel<T, WMAX, 1>(s, p);
   }
-  lu_rowswap<T>(s, W, ld, ipiv, perm, c0, w, cl, ch);
+  lu_rowswap<T>(s, W, ld, pairs, perm, c0, w, cl, ch);
 }
 
-template void lu_leaf<float>(hipStream_t, int, float *, long, int, int, int *, int *, float, int, int);
-template void lu_leaf<double>(hipStream_t, int, double *, long, int, int, int *, int *, double, int,
-                              int);
+template void lu_leaf<float>(hipStream_t, int, float *, long, int, int, int *, int *, int *, float, int,
+                             int);
+template void lu_leaf<double>(hipStream_t, int, double *, long, int, int, int *, int *, int *, double,
+                              int, int);
 template void lu_rowswap<float>(hipStream_t, float *, long, const int *, int *, int, int, int, int);
 template void lu_rowswap<double>(hipStream_t, double *, long, const int *, int *, int, int, int, int);
 

@@ -157,7 +157,10 @@ hipStream_t side_stream() {
   auto it = streams.find(dev);
   if (it != streams.end()) return it->second;
   hipStream_t st = nullptr;
-  int keep = 32;
+  // measured on MI355X: masking CUs off the trailing-update stream slows
+  // the fp32 factorization (its GEMMs lose the CUs) and gains nothing for
+  // the others, so the mask is opt-in
+  int keep = 0;
   if (const char *e = getenv("MATGPU_PANEL_CUS")) keep = atoi(e);
   const int cus = num_cus();
   if (keep > 0 && keep < cus) {
