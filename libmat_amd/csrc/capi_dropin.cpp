@@ -94,7 +94,7 @@ int tri_block_size();
 template <typename U>
 void tri_invert_blocks(hipStream_t, bool, bool, int, View<const U>, U *);
 template <typename U>
-void tri_solve_blocked(hipStream_t, bool, int, int, View<const U>, const U *, U *, long);
+void tri_solve_blocked(hipStream_t, bool, int, int, View<const U>, const U *, U *, long, const U *);
 template <typename U>
 void getrs_rowmajor(hipStream_t, int, int, const U *, long, const int *, const U *, long, U *, long);
 template <typename U>
@@ -299,7 +299,7 @@ static void tri_solve(const char *fn, Vec *x, const Mat *Tm, const Vec *b, bool 
     const int tb = mg::tri_block_size<T>();
     T *D = (T *)mg::workspace(s).get("dropin.D", (size_t)((n + tb - 1) / tb) * tb * tb * sizeof(T));
     mg::tri_invert_blocks<T>(s, lower, unit, n, v, D);
-    mg::tri_solve_blocked<T>(s, lower, n, 1, v, D, dx, n);
+    mg::tri_solve_blocked<T>(s, lower, n, 1, v, D, dx, n, nullptr);
     MG_HIP(hipMemcpyAsync(x->data, dx, (size_t)n * sizeof(T), hipMemcpyDeviceToHost, s));
     finish(s);
   });

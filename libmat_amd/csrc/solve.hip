@@ -31,8 +31,12 @@ int tri_block_size();
 template <typename T>
 void tri_invert_blocks(hipStream_t s, bool lower, bool unit, int n, View<const T> Tm, T *D);
 template <typename T>
+size_t tri_fold_size(bool lower, int n);
+template <typename T>
+void tri_fold_tiles(hipStream_t s, bool lower, int n, View<const T> Tm, const T *D, T *Tp);
+template <typename T>
 void tri_solve_blocked(hipStream_t s, bool lower, int n, int nrhs, View<const T> Tm, const T *D,
-                       T *X, long ldx);
+                       T *X, long ldx, const T *Tp);
 
 // A pair of triangular factors with their inverted diagonal blocks, ready
 // for repeated solves (one preparation per factorization).
@@ -42,8 +46,12 @@ struct TriPair {
   View<const T> lo{}, up{};
   bool lo_unit = false, up_unit = false;
   T *dlo = nullptr, *dup = nullptr;
+  T *flo = nullptr, *fup = nullptr;  // folded tiles (repeated solves only)
 
-  void prepare(hipStream_t s, const char *tag) {
+  // fold: also precompute D_r T(r,k) for every off-diagonal tile (one GEMM
+  // per block row and triangle), which takes the diagonal-block product off
+  // the critical path of every later solve; pays off from a few solves on
+  void prepare(hipStream_t s, const char *tag, bool fold = false) {
     const int tb = tri_block_size<T>();
     const size_t blocks = (size_t)((n + tb - 1) / tb) * tb * tb;
     Workspace &ws = workspace(s);
@@ -51,12 +59,19 @@ struct TriPair {
     dup = (T *)ws.get(std::string(tag) + ".dup", blocks * sizeof(T));
     tri_invert_blocks<T>(s, true, lo_unit, n, lo, dlo);
     tri_invert_blocks<T>(s, false, up_unit, n, up, dup);
+    flo = fup = nullptr;
+    if (fold) {
+      flo = (T *)ws.get(std::string(tag) + ".flo", tri_fold_size<T>(true, n) * sizeof(T));
+      fup = (T *)ws.get(std::string(tag) + ".fup", tri_fold_size<T>(false, n) * sizeof(T));
+      tri_fold_tiles<T>(s, true, n, lo, dlo, flo);
+      tri_fold_tiles<T>(s, false, n, up, dup, fup);
+    }
   }
   // X := (lo * up)^-1 X
   void solve(hipStream_t s, int nrhs, T *X, long ldx) const {
     if (nrhs <= 64) {
-      tri_solve_blocked<T>(s, true, n, nrhs, lo, dlo, X, ldx);
-      tri_solve_blocked<T>(s, false, n, nrhs, up, dup, X, ldx);
+      tri_solve_blocked<T>(s, true, n, nrhs, lo, dlo, X, ldx, flo);
+      tri_solve_blocked<T>(s, false, n, nrhs, up, dup, X, ldx, fup);
     } else {
       trsm_left<T>(s, true, lo_unit, n, nrhs, lo, View<T>{X, 1, ldx});
       trsm_left<T>(s, false, up_unit, n, nrhs, up, View<T>{X, 1, ldx});
@@ -194,16 +209,20 @@ void residual_f64(hipStream_t s, int n, int nrhs, const double *A, long lda, con
 
 // ------------------------------------------------------------- norms -----
 // out[c] = max_i |X(i,c)| (non-negative doubles order like their bits).
+// max that keeps NaN (fmax drops it, which would hide a NaN solution); as a
+// 64-bit pattern a NaN also orders above +inf for the atomicMax below
+__device__ inline double nan_max(double a, double b) { return (b > a || b != b) ? b : a; }
+
 __global__ void col_amax_kernel(int n, const double *X, long ldx, unsigned long long *out) {
   const int c = blockIdx.y;
   double m = 0.0;
   for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256)
-    m = fmax(m, fabs(X[i + (long)c * ldx]));
+    m = nan_max(m, fabs(X[i + (long)c * ldx]));
   __shared__ double red[256];
   red[threadIdx.x] = m;
   __syncthreads();
   for (int off = 128; off > 0; off >>= 1) {
-    if (threadIdx.x < off) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + off]);
+    if (threadIdx.x < off) red[threadIdx.x] = nan_max(red[threadIdx.x], red[threadIdx.x + off]);
     __syncthreads();
   }
   if (threadIdx.x == 0) atomicMax(out + c, (unsigned long long)__double_as_longlong(red[0]));
@@ -230,7 +249,7 @@ __global__ void row_abs_max_kernel(int n, int nchunks, const double *part,
   red[threadIdx.x] = s;
   __syncthreads();
   for (int off = 128; off > 0; off >>= 1) {
-    if (threadIdx.x < off) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + off]);
+    if (threadIdx.x < off) red[threadIdx.x] = nan_max(red[threadIdx.x], red[threadIdx.x + off]);
     __syncthreads();
   }
   if (threadIdx.x == 0) atomicMax(out, (unsigned long long)__double_as_longlong(red[0]));
@@ -325,7 +344,10 @@ struct BerrEval {
       memcpy(&x, &host[nrhs + c], 8);
       memcpy(&b, &host[2 * nrhs + c], 8);
       const double den = normA * x + b;
-      const double e = den > 0 ? r / den : (r > 0 ? INFINITY : 0.0);
+      double e;
+      if (!(den == den) || !(r == r)) e = NAN;  // NaN anywhere: never converged
+      else if (den > 0) e = r / den;
+      else e = r > 0 ? INFINITY : 0.0;
       if (!(e <= worst)) worst = e;  // NaN propagates
     }
     return worst;
@@ -404,7 +426,9 @@ int gesv_mixed(hipStream_t s, int n, int nrhs, const double *A, long lda, const 
     getrf_rowmajor_perm<float>(s, n, W, n, ipiv, perm, 1e-6f, lo);
     tp = lu_pair<float>(n, W, n);
   }
-  tp.prepare(s, "gesvm");
+  // folded tiles need bounded diagonal-block inverses: partial pivoting
+  // gives them, the pivot-free butterfly path may not
+  tp.prepare(s, "gesvm", !rbt);
   sync_and_check(s);
   const double t_factor = ms_since(t0);
 
@@ -522,7 +546,7 @@ int posv_mixed(hipStream_t s, int n, int nrhs, const double *A, long lda, const 
     if (li == 0) t_factor = ms_since(t0);
     if (h != 0) continue;  // not positive definite at this precision
     used = ps;
-    tp.prepare(s, "posvm");
+    tp.prepare(s, "posvm", true);
     convert2d<double, float>(s, n, nrhs, B, ldb, D, n);
     tp.solve(s, nrhs, D, n);
     convert2d<float, double>(s, n, nrhs, D, n, X, ldx);

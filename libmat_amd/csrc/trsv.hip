@@ -248,11 +248,25 @@ __device__ __forceinline__ uint32_t chain_word(T v, int h) {
   }
 }
 
-template <typename T, bool LOWER>
+// Folded tiles T'(r,k) = D_r T(r,k): block row r of the lower triangle is a
+// row-major TB x (r TB) matrix holding T'(r, 0..r-1); of the upper triangle
+// a row-major TB x (n - (r+1) TB) matrix holding T'(r, r+1..nb-1).
+__host__ __device__ inline long fold_off(bool lower, int r, int n, int TB) {
+  if (lower) return (long)TB * TB * ((long)r * (r - 1) / 2);
+  return (long)TB * ((long)r * n - (long)TB * r * (r + 1) / 2);
+}
+__host__ __device__ inline long fold_ld(bool lower, int r, int n, int TB) {
+  return lower ? (long)r * TB : (long)n - (long)(r + 1) * TB;
+}
+
+// FOLD: tiles are the folded T'(r,k) and the workgroup starts from D_r b_r,
+// so the product with the diagonal block leaves the chain's critical path.
+template <typename T, bool LOWER, bool FOLD>
 __global__ __launch_bounds__(ChainNT<T>::NT) void tri_chain_kernel(int n, int nrhs, View<const T> Tm,
-                                                                    const T *D, T *X, long ldx,
-                                                                    unsigned *claim, uint64_t *pub,
-                                                                    unsigned tag, int *err) {
+                                                                    const T *D, const T *Tp, T *X,
+                                                                    long ldx, unsigned *claim,
+                                                                    uint64_t *pub, unsigned tag,
+                                                                    int *err) {
   constexpr int TB = TriTB<T>::TB, NT = ChainNT<T>::NT, NC = CHAIN_NC;
   constexpr int NG = NT / TB;   // column groups
   constexpr int CPT = NC / NG;  // right-hand sides per thread
@@ -278,27 +292,59 @@ __global__ __launch_bounds__(ChainNT<T>::NT) void tri_chain_kernel(int n, int nr
     const int ro = r * TB, rrows = min(TB, n - ro);
     const int cnt = rb;  // blocks solved before this one
     View<const T> Dv{D + (long)r * TB * TB, TB, 1};
+    // tiles: rows tr0.., columns tcol(k).. of Tv
+    const View<const T> Tv =
+        FOLD ? View<const T>{Tp + fold_off(LOWER, r, n, TB), fold_ld(LOWER, r, n, TB), 1} : Tm;
+    const bool trm = FOLD ? true : rm;
+    const int tr0 = FOLD ? 0 : ro;
+    auto tcol = [&](int k) { return FOLD ? (LOWER ? k : k - r - 1) * TB : k * TB; };
     T v[PER];
-    if (cnt > 0) {
-      const int ko = (LOWER ? 0 : nb - 1) * TB;
-      chain_load<T, TB, NT>(Tm, ro, rrows, ko, min(TB, n - ko), v);
-    } else {
-      chain_load<T, TB, NT>(Dv, 0, TB, 0, TB, v);
-    }
     T acc[CPT];
+    if constexpr (FOLD) {
+      // acc = D_r b_r while the earlier blocks are still being solved
+      chain_load<T, TB, NT>(Dv, 0, TB, 0, TB, v);
+      chain_store<T, TB, NT>(true, v, S[1]);
 #pragma unroll
-    for (int q = 0; q < CPT; ++q) {
-      const int c = g * CPT + q;
-      acc[q] = (i < rrows && c < nrhs) ? X[(ro + i) + (long)c * ldx] : T(0);
+      for (int u = 0; u < TB * NC / NT; ++u) {
+        const int e = t + u * NT;
+        const int j = e % TB, c = e / TB;
+        Xs[j * NC + c] = (j < rrows && c < nrhs) ? X[(ro + j) + (long)c * ldx] : T(0);
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < CPT; ++q) acc[q] = T(0);
+#pragma unroll 8
+      for (int j = 0; j < TB; ++j) {
+        const T a = S[1][i * LDS + j];
+#pragma unroll
+        for (int q = 0; q < CPT; ++q) acc[q] = fma(a, Xs[j * NC + g * CPT + q], acc[q]);
+      }
+      __syncthreads();
+      if (cnt > 0) {
+        const int k0 = LOWER ? 0 : nb - 1;
+        chain_load<T, TB, NT>(Tv, tr0, rrows, tcol(k0), min(TB, n - k0 * TB), v);
+      }
+    } else {
+      if (cnt > 0) {
+        const int k0 = LOWER ? 0 : nb - 1;
+        chain_load<T, TB, NT>(Tv, tr0, rrows, tcol(k0), min(TB, n - k0 * TB), v);
+      } else {
+        chain_load<T, TB, NT>(Dv, 0, TB, 0, TB, v);
+      }
+#pragma unroll
+      for (int q = 0; q < CPT; ++q) {
+        const int c = g * CPT + q;
+        acc[q] = (i < rrows && c < nrhs) ? X[(ro + i) + (long)c * ldx] : T(0);
+      }
     }
     for (int idx = 0; idx < cnt; ++idx) {
       const int k = LOWER ? idx : nb - 1 - idx;
       T *Sb = S[idx & 1];
-      chain_store<T, TB, NT>(rm, v, Sb);
+      chain_store<T, TB, NT>(trm, v, Sb);
       if (idx + 1 < cnt) {
         const int k2 = LOWER ? idx + 1 : nb - 2 - idx;
-        chain_load<T, TB, NT>(Tm, ro, rrows, k2 * TB, min(TB, n - k2 * TB), v);
-      } else {
+        chain_load<T, TB, NT>(Tv, tr0, rrows, tcol(k2), min(TB, n - k2 * TB), v);
+      } else if (!FOLD) {
         chain_load<T, TB, NT>(Dv, 0, TB, 0, TB, v);
       }
       // X_k from its published words
@@ -344,20 +390,25 @@ __global__ __launch_bounds__(ChainNT<T>::NT) void tri_chain_kernel(int n, int nr
       }
       __syncthreads();
     }
-    // X_r = D_r^-1 acc
-    T *Sd = S[cnt & 1];
-    chain_store<T, TB, NT>(true, v, Sd);
-#pragma unroll
-    for (int q = 0; q < CPT; ++q) Xs[i * NC + g * CPT + q] = acc[q];
-    __syncthreads();
     T out[CPT];
+    if constexpr (FOLD) {
 #pragma unroll
-    for (int q = 0; q < CPT; ++q) out[q] = T(0);
+      for (int q = 0; q < CPT; ++q) out[q] = acc[q];
+    } else {
+      // X_r = D_r^-1 acc
+      T *Sd = S[cnt & 1];
+      chain_store<T, TB, NT>(true, v, Sd);
+#pragma unroll
+      for (int q = 0; q < CPT; ++q) Xs[i * NC + g * CPT + q] = acc[q];
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < CPT; ++q) out[q] = T(0);
 #pragma unroll 8
-    for (int j = 0; j < TB; ++j) {
-      const T a = Sd[i * LDS + j];
+      for (int j = 0; j < TB; ++j) {
+        const T a = Sd[i * LDS + j];
 #pragma unroll
-      for (int q = 0; q < CPT; ++q) out[q] = fma(a, Xs[j * NC + g * CPT + q], out[q]);
+        for (int q = 0; q < CPT; ++q) out[q] = fma(a, Xs[j * NC + g * CPT + q], out[q]);
+      }
     }
     // publish first (the chain waits on it), then the result itself
     uint64_t *dst = pub + (long)r * BW;
@@ -402,10 +453,36 @@ void tri_invert_blocks(hipStream_t s, bool lower, bool unit, int n, View<const T
 // f32 and f64 solves on a stream share the publication buffer.
 static std::atomic<unsigned> g_chain_launches{0};
 
-// X := op(T)^-1 X, X column-major n x nrhs; D from tri_invert_blocks.
+// Elements of the folded-tile buffer of one triangle.
+template <typename T>
+size_t tri_fold_size(bool lower, int n) {
+  constexpr int TB = TriTB<T>::TB;
+  const int nb = (n + TB - 1) / TB;
+  if (nb <= 1) return 1;
+  return (size_t)(lower ? fold_off(true, nb, n, TB) : fold_off(false, nb - 1, n, TB));
+}
+
+// Tp := D_r T(r, k) for every off-diagonal tile, one MFMA GEMM per block row.
+template <typename T>
+void tri_fold_tiles(hipStream_t s, bool lower, int n, View<const T> Tm, const T *D, T *Tp) {
+  constexpr int TB = TriTB<T>::TB;
+  const int nb = (n + TB - 1) / TB;
+  for (int r = 0; r < nb; ++r) {
+    const int ro = r * TB, rrows = std::min(TB, n - ro);
+    const int c0 = lower ? 0 : ro + TB;
+    const int cols = lower ? ro : n - c0;
+    if (cols <= 0) continue;
+    View<const T> Dr{D + (long)r * TB * TB, TB, 1};
+    View<T> C{Tp + fold_off(lower, r, n, TB), fold_ld(lower, r, n, TB), 1};
+    gemm<T>(s, rrows, cols, rrows, T(1), Dr, Tm.sub(ro, c0), T(0), C, false);
+  }
+}
+
+// X := op(T)^-1 X, X column-major n x nrhs; D from tri_invert_blocks and,
+// if not null, Tp from tri_fold_tiles (then D is only used for D_r b_r).
 template <typename T>
 void tri_solve_blocked(hipStream_t s, bool lower, int n, int nrhs, View<const T> Tm, const T *D,
-                       T *X, long ldx) {
+                       T *X, long ldx, const T *Tp) {
   constexpr int TB = TriTB<T>::TB;
   const int nb = (n + TB - 1) / TB;
   if (nb == 0 || nrhs <= 0) return;
@@ -424,12 +501,23 @@ void tri_solve_blocked(hipStream_t s, bool lower, int n, int nrhs, View<const T>
     if (tag == 0) tag = ++g_chain_launches;
     fill_u32(s, claim, 1, 0u);
     KScope ks(s, "tri_chain", ((double)n * n / 2 + 2.0 * n * nc) * sizeof(T));
-    if (lower)
-      hipLaunchKernelGGL((tri_chain_kernel<T, true>), dim3(grid), dim3(ChainNT<T>::NT), 0, s, n, nc, Tm,
-                         D, X + (long)cb * ldx, ldx, claim, pub, tag, err);
-    else
-      hipLaunchKernelGGL((tri_chain_kernel<T, false>), dim3(grid), dim3(ChainNT<T>::NT), 0, s, n, nc, Tm,
-                         D, X + (long)cb * ldx, ldx, claim, pub, tag, err);
+    T *Xc = X + (long)cb * ldx;
+    const dim3 gr(grid), bl(ChainNT<T>::NT);
+    if (Tp) {
+      if (lower)
+        hipLaunchKernelGGL((tri_chain_kernel<T, true, true>), gr, bl, 0, s, n, nc, Tm, D, Tp, Xc, ldx,
+                           claim, pub, tag, err);
+      else
+        hipLaunchKernelGGL((tri_chain_kernel<T, false, true>), gr, bl, 0, s, n, nc, Tm, D, Tp, Xc, ldx,
+                           claim, pub, tag, err);
+    } else {
+      if (lower)
+        hipLaunchKernelGGL((tri_chain_kernel<T, true, false>), gr, bl, 0, s, n, nc, Tm, D, Tp, Xc, ldx,
+                           claim, pub, tag, err);
+      else
+        hipLaunchKernelGGL((tri_chain_kernel<T, false, false>), gr, bl, 0, s, n, nc, Tm, D, Tp, Xc,
+                           ldx, claim, pub, tag, err);
+    }
     MG_HIP(hipGetLastError());
   }
 }
@@ -437,8 +525,10 @@ void tri_solve_blocked(hipStream_t s, bool lower, int n, int nrhs, View<const T>
 #define MG_TRI_INST(T)                                                                           \
   template int tri_block_size<T>();                                                              \
   template void tri_invert_blocks<T>(hipStream_t, bool, bool, int, View<const T>, T *);          \
+  template size_t tri_fold_size<T>(bool, int);                                                   \
+  template void tri_fold_tiles<T>(hipStream_t, bool, int, View<const T>, const T *, T *);        \
   template void tri_solve_blocked<T>(hipStream_t, bool, int, int, View<const T>, const T *, T *, \
-                                     long);
+                                     long, const T *);
 MG_TRI_INST(float)
 MG_TRI_INST(double)
 

@@ -141,6 +141,28 @@ def test_posv_mixed_fp8_updates(kappa):
         assert st.fell_back == 0
 
 
+@pytest.mark.parametrize("kind", ["gesv", "posv"])
+def test_mixed_nan_is_not_converged(kind):
+    """A NaN in the data must never pass the backward-error test: the norms
+    propagate NaN, refinement stops and the fp64 solver's result is reported."""
+    n = 300
+    A = conditioned(n, 1e2, seed=4, spd=(kind == "posv"))
+    A[5, 7] = np.nan
+    B = np.random.default_rng(8).uniform(-1, 1, (n, 2))
+    g = libmat_amd.load("f64")
+    opts = libmat_amd.IrOpts()
+    g.lib.mat_gpu_ir_defaults(C.byref(opts), 1 if kind == "posv" else 0)
+    st = libmat_amd.IrStatus()
+    dA, dB = to_dev(A, np.float64), to_dev(B, np.float64)
+    dX = torch.zeros(n * 2, dtype=torch.float64, device="cuda")
+    getattr(g.lib, f"mat_gpu_{kind}_mixed")(stream(), n, 2, ptr(dA), n, ptr(dB), n, ptr(dX), n,
+                                            C.byref(opts), C.byref(st))
+    torch.cuda.synchronize()
+    assert not (st.backward_error <= 1e-14)
+    if kind == "gesv":
+        assert st.fell_back == 1
+
+
 def test_gesv_mixed_forced_stall_falls_back():
     n = 512
     A = conditioned(n, 1e6, seed=3)
