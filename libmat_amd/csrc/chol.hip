@@ -6,8 +6,8 @@
 // d <= 0, l(j,j) = sqrt(d) and the column below is multiplied by 1/l(j,j).
 //
 // MI355X design: recursive blocking.  A 64-column diagonal block is
-// factored by one workgroup in LDS (right-looking: scale the column, rank-1
-// update of the trailing lower triangle); the panel below it is solved by
+// factored by one wavefront in registers (right-looking: scale the column,
+// rank-1 update of the trailing lower triangle); the panel below it is solved by
 // the triangular-solve kernels (A21 := A21 L11^-T), and the trailing lower
 // triangle is updated by one MFMA GEMM restricted to tiles on or below the
 // diagonal (the reference's SYRK, mat.h:5471-5487).  The first failing
@@ -20,55 +20,74 @@ namespace mg {
 constexpr int CHOL_NB = 64;
 
 template <typename T>
-__global__ __launch_bounds__(256) void chol_leaf_kernel(int n, T *A, long lda, int j0, int *info) {
+__device__ __forceinline__ T lane_value(T v, int src) {
+  if constexpr (sizeof(T) == 4) {
+    return __uint_as_float(__builtin_amdgcn_readlane(__float_as_uint(v), src));
+  } else {
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = __builtin_amdgcn_readlane((unsigned)u, src);
+    const unsigned hi = __builtin_amdgcn_readlane((unsigned)(u >> 32), src);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+  }
+}
+
+// Column j of the one-wavefront leaf, then the columns after it; compile-time
+// recursion so that every register index is static.
+template <typename T, int J, int NB>
+__device__ __forceinline__ void chol_steps(T (&a)[NB], int i, int n, int j0, int *info, bool &failed) {
+  if constexpr (J < NB) {
+    if (J >= n) return;
+    const T d = lane_value(a[J], J);
+    if (d <= T(0)) {  // the reference's test; NaN passes it
+      if (i == 0) *info = j0 + J + 1;
+      failed = true;
+      return;
+    }
+    const T l = sqrt(d);
+    const T inv = T(1) / l;
+    a[J] = i == J ? l : (i > J ? a[J] * inv : a[J]);
+    const T lij = a[J];
+#pragma unroll
+    for (int c = J + 1; c < NB; ++c) {
+      const T lcj = lane_value(lij, c);
+      a[c] = i >= c ? fma(-lij, lcj, a[c]) : a[c];
+    }
+    chol_steps<T, J + 1, NB>(a, i, n, j0, info, failed);
+  }
+}
+
+// One wavefront factors the NB x NB diagonal block: lane i holds row i in
+// registers; column j's entries are broadcast with v_readlane (static lane
+// and register indices, no barrier); right-looking rank-1 updates.  The
+// block goes through LDS only to be read and written column-coalesced.
+template <typename T>
+__global__ __launch_bounds__(64) void chol_leaf_kernel(int n, T *A, long lda, int j0, int *info) {
   constexpr int NB = CHOL_NB;
   __shared__ T S[NB * (NB + 1)];  // S[c*(NB+1) + r] = A(r, c)
-  __shared__ T s_inv;
-  __shared__ int s_fail;
   if (*(volatile int *)info != 0) return;  // an earlier column already failed
-  const int t = threadIdx.x;
-  for (int idx = t; idx < NB * NB; idx += 256) {
-    const int r = idx % NB, c = idx / NB;
-    S[c * (NB + 1) + r] = (r < n && c < n && r >= c) ? A[r + (long)c * lda] : T(0);
-  }
-  if (t == 0) s_fail = 0;
-  __syncthreads();
-  for (int j = 0; j < n; ++j) {
-    if (t == 0) {
-      const T d = S[j * (NB + 1) + j];
-      if (d <= T(0)) {
-        s_fail = 1;
-        *info = j0 + j + 1;
-      } else {
-        const T l = sqrt(d);
-        S[j * (NB + 1) + j] = l;
-        s_inv = T(1) / l;
-      }
-    }
-    __syncthreads();
-    if (s_fail) return;
-    const T inv = s_inv;
-    for (int i = j + 1 + t; i < n; i += 256) S[j * (NB + 1) + i] *= inv;
-    __syncthreads();
-    // trailing lower triangle: a(i,c) -= l(i,j) l(c,j), j < c <= i
-    const int m = n - j - 1;
-    for (int idx = t; idx < m * m; idx += 256) {
-      const int i = j + 1 + idx % m, c = j + 1 + idx / m;
-      if (i >= c) S[c * (NB + 1) + i] = fma(-S[j * (NB + 1) + i], S[j * (NB + 1) + c], S[c * (NB + 1) + i]);
-    }
-    __syncthreads();
-  }
-  for (int idx = t; idx < NB * NB; idx += 256) {
-    const int r = idx % NB, c = idx / NB;
-    if (r < n && c < n && r >= c) A[r + (long)c * lda] = S[c * (NB + 1) + r];
-  }
+  const int i = threadIdx.x;
+  for (int c = 0; c < NB; ++c) S[c * (NB + 1) + i] = (i < n && c < n && i >= c) ? A[i + (long)c * lda] : T(0);
+  __builtin_amdgcn_s_waitcnt(0);
+  __builtin_amdgcn_wave_barrier();
+  T a[NB];
+#pragma unroll
+  for (int c = 0; c < NB; ++c) a[c] = S[c * (NB + 1) + i];
+  bool failed = false;
+  chol_steps<T, 0, NB>(a, i, n, j0, info, failed);
+  if (failed) return;
+#pragma unroll
+  for (int c = 0; c < NB; ++c) S[c * (NB + 1) + i] = a[c];
+  __builtin_amdgcn_s_waitcnt(0);
+  __builtin_amdgcn_wave_barrier();
+  for (int c = 0; c < NB; ++c)
+    if (i < n && c < n && i >= c) A[i + (long)c * lda] = S[c * (NB + 1) + i];
 }
 
 template <typename T>
 static void potrf_rec(hipStream_t s, int n, T *A, long lda, int j0, int *info, Mul mul) {
   if (n <= CHOL_NB) {
     KScope ks(s, "chol_leaf", (double)n * n * sizeof(T));
-    hipLaunchKernelGGL((chol_leaf_kernel<T>), dim3(1), dim3(256), 0, s, n, A, lda, j0, info);
+    hipLaunchKernelGGL((chol_leaf_kernel<T>), dim3(1), dim3(64), 0, s, n, A, lda, j0, info);
     MG_HIP(hipGetLastError());
     return;
   }

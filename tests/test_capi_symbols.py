@@ -72,3 +72,32 @@ def test_host_helpers(prec):
     L.mat_free_perm(p)
     L.mat_free_mat(m)
     L.mat_free_mat(e)
+
+
+@pytest.mark.parametrize("fn", ["mat_plu", "mat_chol", "mat_det"])
+def test_empty_matrix_is_a_contract_violation(fn):
+    """libmat's MAT_ASSERT_MAT requires rows > 0 and cols > 0 (mat.h:334-340);
+    the drop-in aborts with a message on the same violation, before any GPU
+    call (so this runs without a GPU), instead of returning a value."""
+    import subprocess
+    import sys
+    code = f"""
+import ctypes as C, sys
+sys.path.insert(0, {libmat_amd.ROOT!r})
+import libmat_amd
+g = libmat_amd.load("f32")
+buf = (C.c_float * 1)()
+z = g.Mat(0, 0, C.cast(buf, C.POINTER(C.c_float)))
+pbuf = (C.c_size_t * 1)()
+p = libmat_amd.Perm(C.cast(pbuf, C.POINTER(C.c_size_t)), 0)
+if "{fn}" == "mat_plu":
+    g.lib.mat_plu(C.byref(z), C.byref(z), C.byref(z), C.byref(p))
+elif "{fn}" == "mat_chol":
+    g.lib.mat_chol(C.byref(z), C.byref(z))
+else:
+    g.lib.mat_det(C.byref(z))
+print("returned")
+"""
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
+    assert r.returncode != 0 and "returned" not in r.stdout
+    assert "assertion failed" in r.stderr and fn in r.stderr

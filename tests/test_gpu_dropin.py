@@ -188,3 +188,28 @@ def test_det_inv(gpu, prec):
     A = rng.uniform(-1, 1, (n, n)) + n * np.eye(n)
     Ai = g.mat_inv(A.astype(DT[prec]))
     assert np.abs(A @ Ai - np.eye(n)).max() < 100 * n * EPS[prec]
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_one_by_one(gpu, prec):
+    """1 x 1 systems against the oracle: a regular pivot, a zero pivot (the
+    eps skip) and a non-positive-definite Cholesky."""
+    from oracle.oracle import Oracle
+    g, o = gpu[prec], Oracle(prec)
+    for a in (2.5, -0.75, 0.0):
+        A = np.array([[a]])
+        L, U, p, s = g.mat_plu(A)
+        Lo, Uo, po, so = o.plu(A)
+        assert np.array_equal(p, po) and s == so
+        assert np.array_equal(L, Lo) and np.array_equal(U, Uo)
+        assert g.mat_det(A) == o.det(A)
+        Lc, rc = g.mat_chol(A)
+        Lco, rco = o.chol(A)
+        assert rc == rco
+        if rc == 0:
+            assert np.allclose(Lc, Lco, rtol=4 * EPS[prec])
+        if a != 0.0:
+            x = g.mat_solve(A, [1.5])
+            assert np.allclose(x, o.solve(A, [1.5]), rtol=4 * EPS[prec])
+    C = g.mat_mul([[3.0]], [[-2.0]])
+    assert C[0, 0] == -6.0
