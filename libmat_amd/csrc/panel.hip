@@ -8,14 +8,15 @@
 // column below by 1/pivot and apply the rank-1 update to the block's
 // remaining columns.  Columns k >= n-1 are not pivoted (mat.h:4214).
 //
-// MI355X design.  The panel's rows are spread over P workgroups of 512
-// threads, each thread holding RPT whole rows of the leaf in registers for
-// the whole leaf (register rotation keeps the current column in slot 0, so
-// every register index is static while the column loop stays a runtime
-// loop).  One column step:
-//   * each wavefront finds its best row with shuffles and parks that row,
+// MI355X design.  The panel's rows are spread over P <= 64 workgroups of
+// 512 threads, each thread holding one whole row of
+// the leaf in registers for the whole leaf (register rotation keeps the
+// current column in slot 0, so every register index is static while the
+// column loop stays a runtime loop).  One column step:
+//   * each wavefront finds its best row (f32: a DPP float max, then ballots
+//     for the first row holding it; f64: 64-bit keys) and parks that row,
 //     plus row k if it owns it, in LDS (double-buffered by step parity);
-//   * one barrier; every thread reduces the eight wavefront candidates;
+//   * one barrier; wavefront 0 merges the wavefront candidates lane-parallel;
 //   * with P > 1, wavefront 0 publishes the workgroup's candidate (value,
 //     row index, row) to global memory as self-validating 64-bit words
 //     (payload in the low half, a tag unique to this launch and step in the
@@ -24,8 +25,11 @@
 //     workgroup) and parks the pivot row and row k in LDS; one more barrier.
 //     No fences, atomics or counters are needed and nothing is reset between
 //     launches: stale words carry an older tag.
-//   * swap, scale and rank-1 update of the thread's own rows.
-// Every poll is bounded and reports a timeout instead of hanging.
+//   * swap; scale and rank-1 update of the unfinished columns, fused with
+//     the rotation and free of branches.
+// At the end, workgroup 0 turns the leaf's interchanges into row pairs that
+// every later row move of this leaf reuses.  Every poll is bounded and
+// reports a timeout instead of hanging.
 #include <algorithm>
 #include <cstdint>
 
@@ -33,8 +37,14 @@
 
 namespace mg {
 
-constexpr int PANEL_NT = 512;
-constexpr int PANEL_NW = PANEL_NT / 64;
+// threads per panel workgroup, one leaf row per thread.  f32 rows would fit
+// 1024-thread workgroups (half as many workgroups per exchange), but the
+// 16-wavefront barrier and merge cost more than that saves: measured at
+// m = 16384, 4.64 us per column against 4.03 us with 512 threads
+template <typename T>
+struct PanelNT {
+  static constexpr int value = 512;
+};
 constexpr long SPIN_LIMIT = 1L << 24;
 
 template <typename T>
@@ -267,7 +277,9 @@ __device__ inline T value_of(const uint32_t *w) {
 }
 
 template <typename T, int WMAX, int RPT>
-__global__ __launch_bounds__(PANEL_NT) void panel_gepp_kernel(PanelP<T> p) {
+__global__ __launch_bounds__(PanelNT<T>::value) void panel_gepp_kernel(PanelP<T> p) {
+  constexpr int PANEL_NT = PanelNT<T>::value;
+  constexpr int PANEL_NW = PANEL_NT / 64;
   constexpr int RPW = PANEL_NT * RPT;
   constexpr int NWD = Words<T>::N;
   // record words: [0..KW) key, [ROW0..) row
@@ -648,6 +660,7 @@ __global__ __launch_bounds__(256) void rowswap_kernel(T *W, long ld, int cl, int
 
 template <typename T, int WMAX, int RPT>
 static void launch_panel(hipStream_t s, PanelP<T> &p) {
+  constexpr int PANEL_NT = PanelNT<T>::value;
   constexpr int RPW = PANEL_NT * RPT;
   p.nwg = ceil_div(p.m, RPW);
   // At most 64 workgroups of 512 threads: all resident at once on an idle
@@ -706,18 +719,11 @@ void lu_leaf(hipStream_t s, int n, T *W, long ld, int c0, int w, int *ipiv, int 
   p.err = sync_error_word(s);
   p.epoch = ++epoch;
   if (p.epoch >= (1u << 25)) epoch = 0, p.epoch = ++epoch;
-  // rows per thread: one keeps the per-step update short (measured: a
-  // column step costs ~40% less than with two), so two only when one row
-  // per thread would need more than 64 workgroups (an f32 row takes 32
-  // registers, an f64 row 64)
-  const int rpt = (sizeof(T) == 4 && ceil_div(m, PANEL_NT) > maxwg) ? 2 : 1;
-  const int nwg = ceil_div(m, PANEL_NT * rpt);
+  // one row per thread (measured: a column step costs ~40% less than with
+  // two rows per thread); an f32 row takes 32 registers, an f64 row 64
+  const int nwg = ceil_div(m, PanelNT<T>::value);
   MG_CHECK(nwg <= maxwg, "lu_leaf: matrix too tall");
-  if (rpt == 2) {
-    if constexpr (sizeof(T) == 4) launch_panel<T, WMAX, 2>(s, p);
-  } else {
-    launch_panel<T, WMAX, 1>(s, p);
-  }
+  launch_panel<T, WMAX, 1>(s, p);
   lu_rowswap<T>(s, W, ld, pairs, perm, c0, w, cl, ch);
 }
 
