@@ -9,7 +9,7 @@
 // remaining columns.  Columns k >= n-1 are not pivoted (mat.h:4214).
 //
 // MI355X design.  The panel's rows are spread over P <= 64 workgroups of
-// 512 threads, each thread holding one whole row of
+// 256 threads, each thread holding one (or, for m > 16384, two) whole rows of
 // the leaf in registers for the whole leaf (register rotation keeps the
 // current column in slot 0, so every register index is static while the
 // column loop stays a runtime loop).  One column step:
@@ -37,13 +37,12 @@
 
 namespace mg {
 
-// threads per panel workgroup, one leaf row per thread.  f32 rows would fit
-// 1024-thread workgroups (half as many workgroups per exchange), but the
-// 16-wavefront barrier and merge cost more than that saves: measured at
-// m = 16384, 4.64 us per column against 4.03 us with 512 threads
+// threads per panel workgroup.  Fewer wavefronts per barrier beat fewer
+// workgroups per exchange; measured per column at m = 16384 (f32): 1024
+// threads 4.64 us, 512 threads 4.03 us, 256 threads 3.72 us.
 template <typename T>
 struct PanelNT {
-  static constexpr int value = 512;
+  static constexpr int value = 256;
 };
 constexpr long SPIN_LIMIT = 1L << 24;
 
@@ -663,7 +662,7 @@ static void launch_panel(hipStream_t s, PanelP<T> &p) {
   constexpr int PANEL_NT = PanelNT<T>::value;
   constexpr int RPW = PANEL_NT * RPT;
   p.nwg = ceil_div(p.m, RPW);
-  // At most 64 workgroups of 512 threads: all resident at once on an idle
+  // At most 64 workgroups of 256 threads: all resident at once on an idle
   // device (one per CU), which the exchange needs; a launch that could not
   // be resident would end in the bounded poll's timeout, not in a hang.
   KScope ks(s, "panel", 2.0 * p.m * p.w * sizeof(T));
@@ -719,11 +718,14 @@ void lu_leaf(hipStream_t s, int n, T *W, long ld, int c0, int w, int *ipiv, int 
   p.err = sync_error_word(s);
   p.epoch = ++epoch;
   if (p.epoch >= (1u << 25)) epoch = 0, p.epoch = ++epoch;
-  // one row per thread (measured: a column step costs ~40% less than with
-  // two rows per thread); an f32 row takes 32 registers, an f64 row 64
-  const int nwg = ceil_div(m, PanelNT<T>::value);
+  // one row per thread where 64 workgroups suffice (a column step then
+  // costs ~40% less than with two rows per thread), else two; an f32 row
+  // takes 32 registers, an f64 row 64
+  const int rpt = ceil_div(m, PanelNT<T>::value) > maxwg ? 2 : 1;
+  const int nwg = ceil_div(m, PanelNT<T>::value * rpt);
   MG_CHECK(nwg <= maxwg, "lu_leaf: matrix too tall");
-  launch_panel<T, WMAX, 1>(s, p);
+  if (rpt == 1) launch_panel<T, WMAX, 1>(s, p);
+  else launch_panel<T, WMAX, 2>(s, p);
   lu_rowswap<T>(s, W, ld, pairs, perm, c0, w, cl, ch);
 }
 

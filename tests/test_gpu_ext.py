@@ -137,3 +137,30 @@ def test_potrf_not_spd(gpu, prec):
     L = from_dev(dA, n, n)
     assert np.all(np.triu(L, 1) == 0)
     assert rel_fro(L.astype(np.float64) @ L.T, G @ G.T / n + np.eye(n)) <= 8 * n * EPS[prec]
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_getrf_tall_panel(gpu, prec):
+    """n = 16500 > 16384: the first leaves hold two rows per thread (64
+    workgroups of 256 threads cover 16384 rows with one).  PA = LU checked
+    in fp64 on the GPU."""
+    g = gpu[prec]
+    n = 16500
+    dt = torch.float32 if prec == "f32" else torch.float64
+    gen = torch.Generator(device="cuda").manual_seed(77)
+    A = (torch.rand(n * n, device="cuda", dtype=dt, generator=gen) * 2 - 1)
+    A0 = A.clone()
+    perm = torch.zeros(n, dtype=torch.int32, device="cuda")
+    sw = C.c_int(0)
+    g.check(getattr(g.lib, f"mat_gpu_getrf_{prec}")(stream(), n, ptr(A), n, ptr(perm), 32,
+                                                     C.byref(sw)), "getrf")
+    torch.cuda.synchronize()
+    LU = A.view(n, n).T.double()  # column-major buffer -> logical matrix
+    p = perm.long()
+    assert torch.equal(torch.sort(p).values, torch.arange(n, device="cuda"))
+    L = torch.tril(LU, -1) + torch.eye(n, device="cuda", dtype=torch.float64)
+    U = torch.triu(LU)
+    del LU
+    PA = A0.view(n, n).T.double()[p]
+    err = float(torch.linalg.matrix_norm(L @ U - PA) / torch.linalg.matrix_norm(PA))
+    assert err <= 8 * n * EPS[prec], err
