@@ -159,8 +159,8 @@ int mat_gpu_gemm_lowp(void *stream, int passes, size_t m, size_t n, size_t k, fl
 /* In-place partial-pivot LU of the n x n column-major a (packed: unit-lower
  * L below the diagonal, U on and above), libmat's pivot rules and eps-skip.
  * perm (device, n int32) gets libmat's Perm; *swaps (host) the swap count.
- * leaf: pivoting block width in [1, 32]; 24 reproduces libmat's block
- * boundaries for near-zero pivots.  Synchronous. */
+ * leaf: pivoting block width in [1, 64] (0: 32); 24 reproduces libmat's
+ * block boundaries for near-zero pivots.  Synchronous. */
 int mat_gpu_getrf_f32(void *stream, size_t n, float *a, size_t lda, int32_t *perm, int leaf,
                       int *swaps);
 int mat_gpu_getrf_f64(void *stream, size_t n, double *a, size_t lda, int32_t *perm, int leaf,
@@ -207,7 +207,7 @@ typedef struct {
   int passes;   /* default 6 (gesv) / 8 (posv) */
   int max_iter; /* default 30 */
   double tol;   /* default 1e-14 */
-  int leaf;     /* LU pivoting block width, default 32 */
+  int leaf;     /* LU pivoting block width in [1, 64], default 32 */
   int pivoting; /* gesv only: 0 = partial pivoting, libmat's rule
                    (default); 1 = recursive random butterfly transform +
                    no pivoting (faster factorization, but its growth can

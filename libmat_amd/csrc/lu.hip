@@ -26,7 +26,8 @@ void lu_leaf(hipStream_t s, int n, T *W, long ld, int c0, int w, int *ipiv, int 
 template <typename T>
 void lu_rowswap(hipStream_t s, T *W, long ld, const int *pairs, int *perm, int k0, int w, int cl,
                 int ch);
-constexpr int kPairStride = 129;  // ints per leaf in the row-pair buffer (panel.hip)
+constexpr int kPairStride = 257;  // ints per leaf in the row-pair buffer (panel.hip)
+int lu_leaf_max(int m, int elem_size);  // widest leaf the panel kernel takes for m rows
 
 template <typename T>
 static void lu_gemm_sub(hipStream_t s, long M, long N, long K, View<const T> A, View<const T> B,
@@ -43,19 +44,20 @@ static void lu_gemm_sub(hipStream_t s, long M, long N, long K, View<const T> A, 
   gemm<T>(s, M, N, K, T(-1), A, B, T(1), C, false);
 }
 
-// Factor columns [c0, c0+w) (rows c0..n-1, all earlier updates applied);
-// the leaves' row interchanges reach columns [cl, ch).
+// Factor columns [c0, c0+w) (rows c0..n-1, all earlier updates applied)
+// with leaves of at most `leaf` columns (every leaf starts at c0 + a
+// multiple of `leaf`); the leaves' row interchanges reach columns [cl, ch).
 template <typename T>
 static void getrf_rec(hipStream_t s, int n, T *W, long ld, int c0, int w, int *ipiv, int *perm,
-                      int *pairs, T eps, const LuOpts &o, int cl, int ch) {
-  if (w <= o.leaf) {
+                      int *pairs, T eps, const LuOpts &o, int leaf, int cl, int ch) {
+  if (w <= leaf) {
     lu_leaf<T>(s, n, W, ld, c0, w, ipiv, perm, pairs, eps, cl, ch);
     return;
   }
-  int w1 = ((w / 2 + o.leaf - 1) / o.leaf) * o.leaf;
-  if (w1 >= w) w1 = w - o.leaf;
+  int w1 = ((w / 2 + leaf - 1) / leaf) * leaf;
+  if (w1 >= w) w1 = w - leaf;
   const int w2 = w - w1;
-  getrf_rec<T>(s, n, W, ld, c0, w1, ipiv, perm, pairs, eps, o, cl, ch);
+  getrf_rec<T>(s, n, W, ld, c0, w1, ipiv, perm, pairs, eps, o, leaf, cl, ch);
   // U12 = L11^-1 A12 (unit lower)
   View<const T> L11{W + (long)c0 * ld + c0, ld, 1};
   View<T> A12{W + (long)c0 * ld + c0 + w1, ld, 1};
@@ -65,7 +67,7 @@ static void getrf_rec(hipStream_t s, int n, T *W, long ld, int c0, int w, int *i
   View<const T> L21{W + (long)(c0 + w1) * ld + c0, ld, 1};
   View<T> A22{W + (long)(c0 + w1) * ld + c0 + w1, ld, 1};
   lu_gemm_sub<T>(s, m2, w2, w1, L21, cview(A12), A22, o.mul);
-  getrf_rec<T>(s, n, W, ld, c0 + w1, w2, ipiv, perm, pairs, eps, o, cl, ch);
+  getrf_rec<T>(s, n, W, ld, c0 + w1, w2, ipiv, perm, pairs, eps, o, leaf, cl, ch);
 }
 
 // Row interchanges of the factored block [b0, b1) (its leaves in order)
@@ -123,7 +125,10 @@ static void getrf_lookahead(hipStream_t s, int n, T *W, long ld, int *ipiv, int 
   hipEvent_t start = pooled_event(2 * nblk);
   MG_HIP(hipEventRecord(start, s));
   MG_HIP(hipStreamWaitEvent(u, start, 0));
-  getrf_rec<T>(s, n, W, ld, 0, std::min(NB, n), ipiv, perm, pairs, eps, o, 0, std::min(NB, n));
+  // leaf width of a block: the panel kernel's limit for the block's height
+  auto leaf_of = [&](int b0) { return std::min(o.leaf, lu_leaf_max(n - b0, (int)sizeof(T))); };
+  getrf_rec<T>(s, n, W, ld, 0, std::min(NB, n), ipiv, perm, pairs, eps, o, leaf_of(0), 0,
+               std::min(NB, n));
   for (int k = 0; k < nblk; ++k) {
     const int b0 = k * NB, b1 = std::min(n, b0 + NB);
     const int c2 = std::min(n, b1 + NB);
@@ -131,19 +136,19 @@ static void getrf_lookahead(hipStream_t s, int n, T *W, long ld, int *ipiv, int 
     if (b1 < n) {
       // caller: block k+1's columns, then its panel
       if (k >= 1) MG_HIP(hipStreamWaitEvent(s, evU[k - 1], 0));
-      block_swaps<T>(s, W, ld, pairs, b0, b1, o.leaf, b1, c2);
+      block_swaps<T>(s, W, ld, pairs, b0, b1, leaf_of(b0), b1, c2);
       block_update<T>(s, n, W, ld, b0, b1, b1, c2, o);
-      getrf_rec<T>(s, n, W, ld, b1, c2 - b1, ipiv, perm, pairs, eps, o, b1, c2);
+      getrf_rec<T>(s, n, W, ld, b1, c2 - b1, ipiv, perm, pairs, eps, o, leaf_of(b1), b1, c2);
     }
     // side: the rest of the trailing matrix, block k+2's columns first (the
     // only part the caller's next step waits for), then the columns on the left
     MG_HIP(hipStreamWaitEvent(u, evP[k], 0));
     const int c3 = std::min(n, c2 + NB);
-    block_swaps<T>(u, W, ld, pairs, b0, b1, o.leaf, c2, n);
+    block_swaps<T>(u, W, ld, pairs, b0, b1, leaf_of(b0), c2, n);
     block_update<T>(u, n, W, ld, b0, b1, c2, c3, o);
     MG_HIP(hipEventRecord(evU[k], u));
     block_update<T>(u, n, W, ld, b0, b1, c3, n, o);
-    block_swaps<T>(u, W, ld, pairs, b0, b1, o.leaf, 0, b0);
+    block_swaps<T>(u, W, ld, pairs, b0, b1, leaf_of(b0), 0, b0);
   }
   hipEvent_t done = pooled_event(2 * nblk + 1);
   MG_HIP(hipEventRecord(done, u));
@@ -160,7 +165,7 @@ __global__ void iota_kernel(int *p, int n) {
 template <typename T>
 void getrf_rowmajor_perm(hipStream_t s, int n, T *W, long ld, int *ipiv, int *perm, T eps,
                          const LuOpts &o) {
-  MG_CHECK(o.leaf >= 1 && o.leaf <= 32, "getrf: leaf width must be in [1, 32]");
+  MG_CHECK(o.leaf >= 1 && o.leaf <= 64, "getrf: leaf width must be in [1, 64]");
   hipLaunchKernelGGL(iota_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, s, ipiv, n);
   hipLaunchKernelGGL(iota_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, s, perm, n);
   MG_HIP(hipGetLastError());
@@ -170,7 +175,8 @@ void getrf_rowmajor_perm(hipStream_t s, int n, T *W, long ld, int *ipiv, int *pe
   if (NB > 0 && n >= 4 * NB && NB % o.leaf == 0)
     getrf_lookahead<T>(s, n, W, ld, ipiv, perm, pairs, eps, o, NB);
   else
-    getrf_rec<T>(s, n, W, ld, 0, n, ipiv, perm, pairs, eps, o, 0, n);
+    getrf_rec<T>(s, n, W, ld, 0, n, ipiv, perm, pairs, eps, o,
+                 std::min(o.leaf, lu_leaf_max(n, (int)sizeof(T))), 0, n);
 }
 
 template <typename T>

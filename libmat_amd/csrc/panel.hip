@@ -67,7 +67,7 @@ struct PanelP {
 // A leaf's interchanges as one permutation of the touched rows, stored as
 // pairs[0] = count, pairs[1 ..] = destination rows, pairs[1 + PAIR_MAX ..]
 // = source rows: row dst[i] receives the original content of row src[i].
-constexpr int PAIR_MAX = 64;                  // 2 * widest leaf
+constexpr int PAIR_MAX = 128;                 // 2 * widest leaf (64)
 constexpr int PAIR_STRIDE = 1 + 2 * PAIR_MAX;  // ints per leaf
 
 // One full wavefront turns the swap sequence piv[0 .. w) (step kk swaps
@@ -615,10 +615,10 @@ __global__ __launch_bounds__(PanelNT<T>::value) void panel_gepp_kernel(PanelP<T>
 // the columns [cl, ch) outside the leaf's own columns: each thread moves
 // one column's touched entries, all reads first, then all writes, so the
 // permutation is applied in one pass.
-template <typename T, int WMAX>
+template <typename T, int NP>
 __global__ __launch_bounds__(256) void rowswap_kernel(T *W, long ld, int cl, int ch, const int *pairs,
                                                       int *perm, int w, int skip0) {
-  static_assert(2 * WMAX <= PAIR_MAX, "pairs hold at most PAIR_MAX rows");
+  static_assert(NP <= PAIR_MAX, "pairs hold at most PAIR_MAX rows");
   __shared__ int dst[PAIR_MAX], src[PAIR_MAX];
   __shared__ int npair;
   const int t = threadIdx.x;
@@ -634,11 +634,12 @@ __global__ __launch_bounds__(256) void rowswap_kernel(T *W, long ld, int cl, int
   __syncthreads();
   const int np = npair;
   if (np == 0) return;
-  if (perm && blockIdx.x == 0 && t < 64) {
-    // the row-index vector moves with the rows (libmat's Perm)
+  if (perm && blockIdx.x == 0) {
+    // the row-index vector moves with the rows (libmat's Perm); up to
+    // PAIR_MAX pairs span several wavefronts: all reads, barrier, all writes
     int pv = 0;
     if (t < np) pv = perm[src[t]];
-    __builtin_amdgcn_wave_barrier();
+    __syncthreads();
     if (t < np) perm[dst[t]] = pv;
   }
   int c = cl + blockIdx.x * 256 + t;
@@ -649,11 +650,11 @@ __global__ __launch_bounds__(256) void rowswap_kernel(T *W, long ld, int cl, int
   // written after them and keeps its new value.
   T *Wc = W + c;
   const unsigned uld = (unsigned)ld;
-  T v[PAIR_MAX];
+  T v[NP];
 #pragma unroll
-  for (int x = 0; x < PAIR_MAX; ++x) v[x] = Wc[(unsigned long)(unsigned)src[x] * uld];
+  for (int x = 0; x < NP; ++x) v[x] = Wc[(unsigned long)(unsigned)src[x] * uld];
 #pragma unroll
-  for (int x = PAIR_MAX - 1; x >= 0; --x) Wc[(unsigned long)(unsigned)dst[x] * uld] = v[x];
+  for (int x = NP - 1; x >= 0; --x) Wc[(unsigned long)(unsigned)dst[x] * uld] = v[x];
 }
 
 
@@ -677,26 +678,27 @@ static void launch_panel(hipStream_t s, PanelP<T> &p) {
 template <typename T>
 void lu_rowswap(hipStream_t s, T *W, long ld, const int *pairs, int *perm, int k0, int w, int cl,
                 int ch) {
-  constexpr int WMAX = 32;
-  MG_CHECK(w <= WMAX, "lu_rowswap: at most 32 interchanges per call");
+  MG_CHECK(w <= 64, "lu_rowswap: at most 64 interchanges per call");
   const int skip0 = (k0 >= cl && k0 + w <= ch) ? k0 : -1;
   const int count = (ch - cl) - (skip0 >= 0 ? w : 0);
   if (count <= 0 && !perm) return;
   KScope ks(s, "rowswap");
-  hipLaunchKernelGGL((rowswap_kernel<T, WMAX>), dim3(count > 0 ? ceil_div(count, 256) : 1),
-                     dim3(256), 0, s, W, ld, cl, ch, pairs + (long)k0 * PAIR_STRIDE, perm, w, skip0);
+  const dim3 grid(count > 0 ? ceil_div(count, 256) : 1);
+  const int *pr = pairs + (long)k0 * PAIR_STRIDE;
+  if (w <= 32)
+    hipLaunchKernelGGL((rowswap_kernel<T, 64>), grid, dim3(256), 0, s, W, ld, cl, ch, pr, perm, w, skip0);
+  else
+    hipLaunchKernelGGL((rowswap_kernel<T, 128>), grid, dim3(256), 0, s, W, ld, cl, ch, pr, perm, w, skip0);
   MG_HIP(hipGetLastError());
 }
 
 // Factor the leaf block W[c0:n, c0:c0+w] (w <= 32) and apply its row swaps
 // to the columns [cl, ch) of the n x n row-major matrix outside the leaf
 // (and to perm).
-template <typename T>
-void lu_leaf(hipStream_t s, int n, T *W, long ld, int c0, int w, int *ipiv, int *perm, int *pairs,
-             T eps, int cl, int ch) {
-  constexpr int WMAX = 32;
+template <typename T, int WMAX>
+static void lu_leaf_w(hipStream_t s, int n, T *W, long ld, int c0, int w, int *ipiv, int *pairs,
+                      T eps) {
   constexpr int REC = 4 + WMAX * Words<T>::N;
-  MG_CHECK(w <= WMAX, "lu_leaf: leaf too wide");
   static unsigned epoch = 0;  // tags of the exchange words; never 0
   Workspace &ws = workspace(s);
   const int m = n - c0;
@@ -713,19 +715,39 @@ void lu_leaf(hipStream_t s, int n, T *W, long ld, int c0, int w, int *ipiv, int 
   p.pairs = pairs + (long)c0 * PAIR_STRIDE;
   // the exchange polls one workgroup per lane of a wavefront
   const int maxwg = 64;
-  p.rec = (uint64_t *)ws.get("panel.rec", 2L * maxwg * REC * 8);
-  p.krec = (uint64_t *)ws.get("panel.krec", 2L * WMAX * Words<T>::N * 8);
+  p.rec = (uint64_t *)ws.get("panel.rec", 2L * maxwg * (4 + 64 * Words<T>::N) * 8);
+  p.krec = (uint64_t *)ws.get("panel.krec", 2L * 64 * Words<T>::N * 8);
+  (void)REC;
   p.err = sync_error_word(s);
   p.epoch = ++epoch;
   if (p.epoch >= (1u << 25)) epoch = 0, p.epoch = ++epoch;
   // one row per thread where 64 workgroups suffice (a column step then
-  // costs ~40% less than with two rows per thread), else two; an f32 row
-  // takes 32 registers, an f64 row 64
+  // costs ~40% less than with two rows per thread), else two
   const int rpt = ceil_div(m, PanelNT<T>::value) > maxwg ? 2 : 1;
   const int nwg = ceil_div(m, PanelNT<T>::value * rpt);
   MG_CHECK(nwg <= maxwg, "lu_leaf: matrix too tall");
-  if (rpt == 1) launch_panel<T, WMAX, 1>(s, p);
-  else launch_panel<T, WMAX, 2>(s, p);
+  if (rpt == 1) {
+    launch_panel<T, WMAX, 1>(s, p);
+  } else {
+    // two f64 rows of 64 columns per thread exceed the register file
+    if constexpr (sizeof(T) == 8 && WMAX == 64) MG_CHECK(false, "lu_leaf: leaf too wide for this height");
+    else launch_panel<T, WMAX, 2>(s, p);
+  }
+}
+
+int lu_leaf_max(int m, int elem_size) {
+  return (elem_size == 8 && m > 64 * PanelNT<double>::value) ? 32 : 64;
+}
+
+// Factor the leaf block W[c0:n, c0:c0+w] (w <= lu_leaf_max) and apply its row swaps
+// to the columns [cl, ch) of the n x n row-major matrix outside the leaf
+// (and to perm).  Leaves up to 32 columns use the 32-column kernel.
+template <typename T>
+void lu_leaf(hipStream_t s, int n, T *W, long ld, int c0, int w, int *ipiv, int *perm, int *pairs,
+             T eps, int cl, int ch) {
+  MG_CHECK(w >= 1 && w <= 64, "lu_leaf: leaf width must be in [1, 64]");
+  if (w <= 32) lu_leaf_w<T, 32>(s, n, W, ld, c0, w, ipiv, pairs, eps);
+  else lu_leaf_w<T, 64>(s, n, W, ld, c0, w, ipiv, pairs, eps);
   lu_rowswap<T>(s, W, ld, pairs, perm, c0, w, cl, ch);
 }
 

@@ -66,7 +66,8 @@ def test_gemm_8192_sampled():
 
 
 @pytest.mark.parametrize("prec", PRECS)
-@pytest.mark.parametrize("n,leaf", [(300, 32), (1000, 24), (1000, 32), (1100, 32), (2500, 32)])
+@pytest.mark.parametrize("n,leaf", [(300, 32), (1000, 24), (1000, 32), (1100, 32), (2500, 32),
+                                    (100, 64), (300, 64), (1100, 64)])
 def test_getrf_vs_oracle(gpu, prec, n, leaf):
     """Multi-workgroup pivoting (m > 256 rows) against the oracle's PLU; from
     n = 1024 (leaf 32) the look-ahead driver with its second stream runs."""
@@ -140,10 +141,12 @@ def test_potrf_not_spd(gpu, prec):
 
 
 @pytest.mark.parametrize("prec", PRECS)
-def test_getrf_tall_panel(gpu, prec):
+@pytest.mark.parametrize("leaf", [32, 64])
+def test_getrf_tall_panel(gpu, prec, leaf):
     """n = 16500 > 16384: the first leaves hold two rows per thread (64
-    workgroups of 256 threads cover 16384 rows with one).  PA = LU checked
-    in fp64 on the GPU."""
+    workgroups of 256 threads cover 16384 rows with one); with 64-column
+    leaves f64 blocks taller than that use 32-column leaves.  PA = LU
+    checked in fp64 on the GPU."""
     g = gpu[prec]
     n = 16500
     dt = torch.float32 if prec == "f32" else torch.float64
@@ -152,7 +155,7 @@ def test_getrf_tall_panel(gpu, prec):
     A0 = A.clone()
     perm = torch.zeros(n, dtype=torch.int32, device="cuda")
     sw = C.c_int(0)
-    g.check(getattr(g.lib, f"mat_gpu_getrf_{prec}")(stream(), n, ptr(A), n, ptr(perm), 32,
+    g.check(getattr(g.lib, f"mat_gpu_getrf_{prec}")(stream(), n, ptr(A), n, ptr(perm), leaf,
                                                      C.byref(sw)), "getrf")
     torch.cuda.synchronize()
     LU = A.view(n, n).T.double()  # column-major buffer -> logical matrix
