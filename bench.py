@@ -204,6 +204,10 @@ def berr_fp64(torch, A, X, B):
     return float(e.max())
 
 
+PIVOTING = {0: "partial pivoting", 1: "random butterfly, no pivoting",
+            2: "tournament pivoting"}
+
+
 def run_mixed(torch, libmat_amd, args, spd):
     g = libmat_amd.load("f64")
     lib = g.lib
@@ -217,6 +221,8 @@ def run_mixed(torch, libmat_amd, args, spd):
     lib.mat_gpu_ir_defaults(C.byref(opts), 1 if spd else 0)
     if args.passes is not None:
         opts.passes = args.passes
+    if args.pivoting is not None and not spd:
+        opts.pivoting = args.pivoting
     status = [libmat_amd.IrStatus() for _ in kappas]
     solve = lib.mat_gpu_posv_mixed if spd else lib.mat_gpu_gesv_mixed
     plain = lib.mat_gpu_posv_f64 if spd else lib.mat_gpu_gesv_f64
@@ -299,8 +305,8 @@ def run_mixed(torch, libmat_amd, args, spd):
            "config": {"workload": name, "n": n, "nrhs": nrhs, "kappas": kappas,
                       "factorization": ("fp32 Cholesky, FP8 e4m3 per-128x128-tile-scaled MFMA "
                                         "trailing updates" if spd else
-                                        f"fp32 LU (partial pivoting), BF16 x{opts.passes} split "
-                                        "MFMA trailing updates"),
+                                        f"fp32 LU ({PIVOTING[opts.pivoting]}), BF16 x{opts.passes} "
+                                        "split MFMA trailing updates"),
                       "refinement": "fp64 residual + fp64 update, fallback to fp64 on stall",
                       "parallelism": "single GPU"},
            "per_kappa": per, "kernels": breakdown(table, args.steps)}
@@ -638,6 +644,8 @@ def main():
     ap.add_argument("--nrhs", type=int, default=None)
     ap.add_argument("--kappas", default=None)
     ap.add_argument("--passes", type=int, default=None)
+    ap.add_argument("--pivoting", type=int, default=None, choices=[0, 1, 2],
+                    help="mixed LU: 0 partial, 1 butterfly, 2 tournament (library default if unset)")
     ap.add_argument("--count", type=int, default=200000)
     ap.add_argument("--loop-sample", type=int, default=2000)
     ap.add_argument("--cpu-count", type=int, default=20000)

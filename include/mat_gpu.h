@@ -165,6 +165,13 @@ int mat_gpu_getrf_f32(void *stream, size_t n, float *a, size_t lda, int32_t *per
                       int *swaps);
 int mat_gpu_getrf_f64(void *stream, size_t n, double *a, size_t lda, int32_t *perm, int leaf,
                       int *swaps);
+/* Same packed result with tournament pivoting (CALU) in the leaves of at
+ * most 16384 rows: PA = LU with other pivots than libmat's partial
+ * pivoting, and one exchange between workgroups per 32-column leaf instead
+ * of one per column.  No counterpart in libmat (its LU is mat_plu,
+ * mat.h:4436-4474); the mixed solver's factorization with pivoting = 2. */
+int mat_gpu_getrf_tournament_f32(void *stream, size_t n, float *a, size_t lda, int32_t *perm,
+                                 int *swaps);
 /* X = A^-1 B from mat_gpu_getrf's packed factor and perm; nrhs columns. */
 int mat_gpu_getrs_f32(void *stream, size_t n, size_t nrhs, const float *lu, size_t ldlu,
                       const int32_t *perm, const float *b, size_t ldb, float *x, size_t ldx);
@@ -211,7 +218,11 @@ typedef struct {
   int pivoting; /* gesv only: 0 = partial pivoting, libmat's rule
                    (default); 1 = recursive random butterfly transform +
                    no pivoting (faster factorization, but its growth can
-                   stall refinement on ill-conditioned matrices) */
+                   stall refinement on ill-conditioned matrices);
+                   2 = tournament pivoting (CALU) for leaves of at most 32
+                   columns and 16384 rows (partial pivoting elsewhere):
+                   one exchange between workgroups per leaf instead of
+                   one per column */
 } mat_gpu_ir_opts;
 typedef struct {
   int iterations;        /* refinement steps taken */

@@ -35,6 +35,7 @@ EXT_SYMBOLS = [
     "mat_gpu_last_error", "mat_gpu_elem_size", "mat_gpu_stream_check",
     "mat_gpu_timing_enable", "mat_gpu_timing_query",
     "mat_gpu_gemm_f32", "mat_gpu_gemm_f64", "mat_gpu_gemm_lowp", "mat_gpu_getrf_f32", "mat_gpu_getrf_f64",
+    "mat_gpu_getrf_tournament_f32",
     "mat_gpu_getrs_f32", "mat_gpu_getrs_f64", "mat_gpu_potrf_f32", "mat_gpu_potrf_f64",
     "mat_gpu_potrs_f32", "mat_gpu_potrs_f64", "mat_gpu_gesv_f32", "mat_gpu_gesv_f64",
     "mat_gpu_posv_f32", "mat_gpu_posv_f64", "mat_gpu_ir_defaults", "mat_gpu_gesv_mixed",
@@ -120,6 +121,7 @@ class GpuLib:
             s(f"mat_gpu_potrs_{t}", i32, [vp, sz, sz, vp, sz, vp, sz, vp, sz])
             s(f"mat_gpu_gesv_{t}", i32, [vp, sz, sz, vp, sz, vp, sz, vp, sz])
             s(f"mat_gpu_posv_{t}", i32, [vp, sz, sz, vp, sz, vp, sz, vp, sz])
+        s("mat_gpu_getrf_tournament_f32", i32, [vp, sz, vp, sz, vp, C.POINTER(i32)])
         s("mat_gpu_ir_defaults", None, [C.POINTER(IrOpts), i32])
         for t in ("mat_gpu_gesv_mixed", "mat_gpu_posv_mixed"):
             s(t, i32, [vp, sz, sz, vp, sz, vp, sz, vp, sz, C.POINTER(IrOpts), C.POINTER(IrStatus)])

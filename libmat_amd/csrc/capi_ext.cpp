@@ -61,7 +61,8 @@ int gemm_ext(void *stream, int ta, int tb, size_t m, size_t n, size_t k, T alpha
 
 // Packed column-major LU through the row-major working copy.
 template <typename T>
-int getrf_ext(void *stream, size_t n_, T *a, size_t lda, int32_t *perm, int leaf, int *swaps) {
+int getrf_ext(void *stream, size_t n_, T *a, size_t lda, int32_t *perm, int leaf, int *swaps,
+              bool tournament = false) {
   return guarded([&] {
     check_dims(n_);
     const int n = (int)n_;
@@ -76,6 +77,7 @@ int getrf_ext(void *stream, size_t n_, T *a, size_t lda, int32_t *perm, int leaf
     transpose<T>(s, n, n, a, (long)lda, W, n);
     LuOpts o;
     o.leaf = leaf > 0 ? leaf : 32;
+    o.tournament = tournament;
     const T eps = sizeof(T) == 4 ? (T)1e-6f : (T)1e-9;
     getrf_rowmajor_perm<T>(s, n, W, n, ipiv, perm, eps, o);
     transpose<T>(s, n, n, W, n, a, (long)lda);
@@ -223,6 +225,10 @@ int mat_gpu_getrf_f32(void *s, size_t n, float *a, size_t lda, int32_t *perm, in
 int mat_gpu_getrf_f64(void *s, size_t n, double *a, size_t lda, int32_t *perm, int leaf,
                       int *swaps) {
   return getrf_ext<double>(s, n, a, lda, perm, leaf, swaps);
+}
+int mat_gpu_getrf_tournament_f32(void *s, size_t n, float *a, size_t lda, int32_t *perm,
+                                 int *swaps) {
+  return getrf_ext<float>(s, n, a, lda, perm, 32, swaps, true);
 }
 int mat_gpu_getrs_f32(void *s, size_t n, size_t nrhs, const float *lu, size_t ldlu,
                       const int32_t *perm, const float *b, size_t ldb, float *x, size_t ldx) {

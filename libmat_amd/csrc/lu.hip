@@ -22,7 +22,7 @@ namespace mg {
 
 template <typename T>
 void lu_leaf(hipStream_t s, int n, T *W, long ld, int c0, int w, int *ipiv, int *perm, int *pairs,
-             T eps, int cl, int ch);
+             T eps, int cl, int ch, bool tournament);
 template <typename T>
 void lu_rowswap(hipStream_t s, T *W, long ld, const int *pairs, int *perm, int k0, int w, int cl,
                 int ch);
@@ -51,7 +51,7 @@ template <typename T>
 static void getrf_rec(hipStream_t s, int n, T *W, long ld, int c0, int w, int *ipiv, int *perm,
                       int *pairs, T eps, const LuOpts &o, int leaf, int cl, int ch) {
   if (w <= leaf) {
-    lu_leaf<T>(s, n, W, ld, c0, w, ipiv, perm, pairs, eps, cl, ch);
+    lu_leaf<T>(s, n, W, ld, c0, w, ipiv, perm, pairs, eps, cl, ch, o.tournament);
     return;
   }
   int w1 = ((w / 2 + leaf - 1) / leaf) * leaf;

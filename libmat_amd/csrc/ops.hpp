@@ -19,6 +19,9 @@ struct LuOpts {
   // at n = 16384, 32-column leaves factor faster: 122 ms against 127 ms)
   int leaf = 32;
   Mul mul = Mul::Native;  // precision of the trailing updates (float only)
+  // float only: tournament pivoting for leaves of at most 32 columns and
+  // 16384 rows (tslu.hip) instead of libmat's partial pivoting
+  bool tournament = false;
 };
 template <typename T>
 void getrf_rowmajor(hipStream_t s, int n, T *W, long ld, int *ipiv, T eps, const LuOpts &o);

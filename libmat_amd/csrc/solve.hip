@@ -385,7 +385,8 @@ int gesv_mixed(hipStream_t s, int n, int nrhs, const double *A, long lda, const 
   if (opt) o = *opt;
   MG_CHECK(o.passes == 0 || o.passes == 1 || o.passes == 3 || o.passes == 6,
            "gesv_mixed: passes must be 0 (fp32 MFMA), 1, 3 or 6");
-  MG_CHECK(o.pivoting == 0 || o.pivoting == 1, "gesv_mixed: pivoting must be 0 (partial) or 1 (rbt)");
+  MG_CHECK(o.pivoting >= 0 && o.pivoting <= 2,
+           "gesv_mixed: pivoting must be 0 (partial), 1 (rbt) or 2 (tournament)");
   const bool rbt = o.pivoting == 1;
   const int n4 = rbt ? (n + 3) / 4 * 4 : n;
   Workspace &ws = workspace(s);
@@ -423,6 +424,7 @@ int gesv_mixed(hipStream_t s, int n, int nrhs, const double *A, long lda, const 
     LuOpts lo;
     lo.leaf = o.leaf;
     lo.mul = mul;
+    lo.tournament = o.pivoting == 2;
     getrf_rowmajor_perm<float>(s, n, W, n, ipiv, perm, 1e-6f, lo);
     tp = lu_pair<float>(n, W, n);
   }

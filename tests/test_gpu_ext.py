@@ -102,6 +102,75 @@ def test_getrf_vs_oracle(gpu, prec, n, leaf):
     assert backward_error(A, from_dev(dX, n, 5), B) <= 8 * n * EPS[prec]
 
 
+def _perm_parity(p):
+    """(-1)^(number of transpositions) of the permutation p."""
+    p = list(p)
+    seen, par = [False] * len(p), 0
+    for i in range(len(p)):
+        if not seen[i]:
+            j, c = i, 0
+            while not seen[j]:
+                seen[j] = True
+                j = p[j]
+                c += 1
+            par ^= (c - 1) & 1
+    return par
+
+
+@pytest.mark.parametrize("n", [32, 100, 300, 1100, 3000, 9000, 16500])
+def test_getrf_tournament(gpu, n):
+    """Tournament pivoting (fp32): one workgroup (n <= 256), the one-round
+    cross-workgroup playoff (n = 300: 2 workgroups), two rounds (1100: 5;
+    3000: 12 workgroups, look-ahead driver), two rows per lane (9000: 18
+    workgroups of 512 rows), and panels taller than 16384 rows, which keep
+    partial pivoting (16500).  Checked by what the mathematics fixes, since
+    the pivots are not libmat's: perm is a permutation whose parity the
+    swap count gives, PA = LU to fp32 rounding (fp64 on the GPU), the
+    multipliers stay small (the tournament's growth is close to partial
+    pivoting's), a solve with the factor, and bit-identical results on a
+    second run (every workgroup plays the last round itself; they must
+    agree)."""
+    g = gpu["f32"]
+    gen = torch.Generator(device="cuda").manual_seed(n + 7)
+    A0 = torch.rand(n * n, device="cuda", dtype=torch.float32, generator=gen) * 2 - 1
+    outs = []
+    for _ in range(2):
+        A = A0.clone()
+        perm = torch.zeros(n, dtype=torch.int32, device="cuda")
+        sw = C.c_int(0)
+        g.check(g.lib.mat_gpu_getrf_tournament_f32(stream(), n, ptr(A), n, ptr(perm), C.byref(sw)),
+                "getrf_tournament")
+        torch.cuda.synchronize()
+        outs.append((A, perm, sw.value))
+    (A, perm, sw), (A2, perm2, sw2) = outs
+    assert torch.equal(A, A2) and torch.equal(perm, perm2) and sw == sw2
+    del A2
+    p = perm.long()
+    assert torch.equal(torch.sort(p).values, torch.arange(n, device="cuda"))
+    assert _perm_parity(p.cpu().tolist()) == (sw & 1)
+    LU = A.view(n, n).T.double()  # column-major buffer -> logical matrix
+    L = torch.tril(LU, -1) + torch.eye(n, device="cuda", dtype=torch.float64)
+    U = torch.triu(LU)
+    del LU
+    lmax = float(L.abs().max())
+    PA = A0.view(n, n).T.double()[p]
+    err = float(torch.linalg.matrix_norm(L @ U - PA) / torch.linalg.matrix_norm(PA))
+    print(f"n={n} max|L|={lmax:.2f} swaps={sw} rel err {err:.2e}")
+    del L, U, PA
+    assert lmax <= 8.0
+    assert err <= 8 * n * EPS["f32"], err
+    B = torch.rand(3 * n, device="cuda", dtype=torch.float32, generator=gen) * 2 - 1
+    X = torch.zeros_like(B)
+    g.check(g.lib.mat_gpu_getrs_f32(stream(), n, 3, ptr(A), n, ptr(perm), ptr(B), n, ptr(X), n),
+            "getrs")
+    torch.cuda.synchronize()
+    Am = A0.view(n, n).T.double()
+    Xm, Bm = X.view(3, n).T.double(), B.view(3, n).T.double()
+    be = float(((Am @ Xm - Bm).abs().amax(0) /
+                (Am.abs().sum(1).max() * Xm.abs().amax(0) + Bm.abs().amax(0))).max())
+    assert be <= 8 * n * EPS["f32"], be
+
+
 @pytest.mark.parametrize("prec", PRECS)
 @pytest.mark.parametrize("n", [100, 700, 2049])
 def test_gesv_posv(gpu, prec, n):

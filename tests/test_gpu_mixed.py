@@ -50,9 +50,9 @@ def run(kind, A, B, passes, max_iter=30, pivoting=None):
 
 @pytest.mark.parametrize("kappa", [1e2, 1e4, 1e6])
 @pytest.mark.parametrize("passes", [3, 6])
-@pytest.mark.parametrize("pivoting", [0, 1])
+@pytest.mark.parametrize("pivoting", [0, 1, 2])
 def test_gesv_mixed(kappa, passes, pivoting):
-    n = 1538 if pivoting else 1536  # the butterfly path pads n to a multiple of 4
+    n = 1538 if pivoting == 1 else 1536  # the butterfly path pads n to a multiple of 4
     A = conditioned(n, kappa, seed=int(np.log10(kappa)))
     B = np.random.default_rng(1).uniform(-1, 1, (n, 16))
     rc, X, st = run("gesv", A, B, passes, pivoting=pivoting)
@@ -65,7 +65,7 @@ def test_gesv_mixed(kappa, passes, pivoting):
     assert rc == 0
     assert be <= 1e-14 and st.backward_error <= 1e-14
     assert fe <= kappa * 1e-13
-    if kappa <= 1e4 and passes == 6 and pivoting == 0:
+    if kappa <= 1e4 and passes == 6 and pivoting != 1:
         assert st.fell_back == 0
 
 
@@ -87,12 +87,14 @@ def conditioned_dev(n, kappa, seed, spd=False):
     return A.contiguous().reshape(-1)
 
 
-def run_dev(kind, dA, n, B, passes):
+def run_dev(kind, dA, n, B, passes, pivoting=None):
     g = libmat_amd.load("f64")
     nr = B.shape[1]
     opts = libmat_amd.IrOpts()
     g.lib.mat_gpu_ir_defaults(C.byref(opts), 1 if kind == "posv" else 0)
     opts.passes = passes
+    if pivoting is not None:
+        opts.pivoting = pivoting
     st = libmat_amd.IrStatus()
     dB = to_dev(B, np.float64)
     dX = torch.zeros(n * nr, dtype=torch.float64, device="cuda")
@@ -112,14 +114,16 @@ def run_dev(kind, dA, n, B, passes):
 
 @pytest.mark.parametrize("kappa", [1e2, 1e6])
 @pytest.mark.parametrize("passes", [3, 6])
-def test_gesv_mixed_split_updates(kappa, passes):
-    """n = 4096: the large trailing updates run as split-BF16 MFMA GEMMs."""
+@pytest.mark.parametrize("pivoting", [0, 2])
+def test_gesv_mixed_split_updates(kappa, passes, pivoting):
+    """n = 4096: the large trailing updates run as split-BF16 MFMA GEMMs;
+    partial and tournament pivoting."""
     n = 4096
     dA = conditioned_dev(n, kappa, seed=11)
     B = np.random.default_rng(5).uniform(-1, 1, (n, 16))
-    rc, st, be, fe = run_dev("gesv", dA, n, B, passes)
-    print(f"n={n} kappa={kappa:g} passes={passes} iters={st.iterations} fell_back={st.fell_back} "
-          f"berr={be:.2e} ferr={fe:.2e}")
+    rc, st, be, fe = run_dev("gesv", dA, n, B, passes, pivoting)
+    print(f"n={n} kappa={kappa:g} passes={passes} pivoting={pivoting} iters={st.iterations} "
+          f"fell_back={st.fell_back} berr={be:.2e} ferr={fe:.2e}")
     assert rc == 0 and be <= 1e-14 and fe <= kappa * 1e-13
     if passes == 6:
         assert st.fell_back == 0
