@@ -7,7 +7,7 @@
 //
 // MI355X design: recursive blocking.  A 64-column diagonal block is
 // factored by one wavefront in registers (right-looking: scale the column,
-// rank-1 update of the trailing lower triangle); the panel below it is solved by
+// rank-1 update of the trailing columns); the panel below it is solved by
 // the triangular-solve kernels (A21 := A21 L11^-T), and the trailing lower
 // triangle is updated by one MFMA GEMM restricted to tiles on or below the
 // diagonal (the reference's SYRK, mat.h:5471-5487).  The first failing
@@ -31,54 +31,87 @@ __device__ __forceinline__ T lane_value(T v, int src) {
   }
 }
 
-// Column j of the one-wavefront leaf, then the columns after it; compile-time
-// recursion so that every register index is static.
-template <typename T, int J, int NB>
-__device__ __forceinline__ void chol_steps(T (&a)[NB], int i, int n, int j0, int *info, bool &failed) {
-  if constexpr (J < NB) {
-    if (J >= n) return;
-    const T d = lane_value(a[J], J);
+// Orders one wavefront's LDS traffic around a lane-to-lane hand-off.
+__device__ __forceinline__ void chol_wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+// Rank-1 update of the slots [4*C0, 4*(C0+NC)): a[s-1] = a[s] - v * colb[s]
+// (the row's remaining columns move down one slot as they update).  The
+// group's LDS reads are all issued before its first update: one LDS latency
+// per group, not per four slots.
+template <typename T, int C0, int NC>
+__device__ __forceinline__ void chol_group(T (&a)[CHOL_NB], const T *colb, T v) {
+  T c[4 * NC];
+#pragma unroll
+  for (int k = 0; k < NC; ++k) {
+    if constexpr (sizeof(T) == 4) {
+      const float4 q = reinterpret_cast<const float4 *>(colb)[C0 + k];
+      c[4 * k] = q.x, c[4 * k + 1] = q.y, c[4 * k + 2] = q.z, c[4 * k + 3] = q.w;
+    } else {
+      const double2 q0 = reinterpret_cast<const double2 *>(colb)[2 * (C0 + k)];
+      const double2 q1 = reinterpret_cast<const double2 *>(colb)[2 * (C0 + k) + 1];
+      c[4 * k] = q0.x, c[4 * k + 1] = q0.y, c[4 * k + 2] = q1.x, c[4 * k + 3] = q1.y;
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < 4 * NC; ++u) {
+    const int sl = 4 * C0 + u;
+    if (sl >= 1) a[sl - 1] = fma(-v, c[u], a[sl]);
+  }
+}
+
+// One wavefront factors the NB x NB diagonal block, right-looking: lane i
+// holds row i in registers with the current column in slot 0.  Step J:
+// the pivot comes from lane J (v_readlane), the column is scaled by 1/l,
+// each finished entry goes straight to LDS, and the column is handed to
+// every lane through LDS, rotated so that lane J+s's value sits at slot s
+// (aligned reads at static offsets); the rank-1 update then moves each
+// row's remaining columns down one slot, in groups of slots that uniform
+// branches skip past the block's end.  Entries above the diagonal
+// take updates too and are never read.  (A fully unrolled 64-step leaf is
+// ~16k instructions, run once per launch from a cold instruction cache.)
+template <typename T>
+__global__ __launch_bounds__(64) void chol_leaf_kernel(int n, T *A, long lda, int j0, int *info) {
+  constexpr int NB = CHOL_NB;
+  __shared__ T S[NB * (NB + 1)];        // S[c*(NB+1) + r] = L(r, c), as finished
+  __shared__ __align__(16) T colb[NB];  // column J: lane i's value at slot (i - J) mod NB
+  if (*(volatile int *)info != 0) return;  // an earlier column already failed
+  const int i = threadIdx.x;
+  // row i of the lower triangle straight into registers: every load is
+  // issued before any is used (one memory latency, not NB)
+  T a[NB];
+#pragma unroll
+  for (int c = 0; c < NB; ++c) a[c] = (i < n && c < n && i >= c) ? A[i + (long)c * lda] : T(0);
+#pragma unroll 1
+  for (int J = 0; J < n; ++J) {
+    const T d = lane_value(a[0], J);
     if (d <= T(0)) {  // the reference's test; NaN passes it
       if (i == 0) *info = j0 + J + 1;
-      failed = true;
       return;
     }
     const T l = sqrt(d);
     const T inv = T(1) / l;
-    a[J] = i == J ? l : (i > J ? a[J] * inv : a[J]);
-    const T lij = a[J];
-#pragma unroll
-    for (int c = J + 1; c < NB; ++c) {
-      const T lcj = lane_value(lij, c);
-      a[c] = i >= c ? fma(-lij, lcj, a[c]) : a[c];
+    const T v = i == J ? l : (i > J ? a[0] * inv : a[0]);
+    if (i >= J) S[J * (NB + 1) + i] = v;
+    colb[(i - J) & (NB - 1)] = v;
+    chol_wave_sync();
+    const int live = n - J;  // slots 0 .. live-1 hold columns J .. n-1
+    // groups of slots past the block's end are skipped (uniform branches)
+    if constexpr (sizeof(T) == 4) {
+      chol_group<T, 0, 8>(a, colb, v);
+      if (32 < live) chol_group<T, 8, 8>(a, colb, v);
+    } else {
+      chol_group<T, 0, 4>(a, colb, v);
+      if (16 < live) chol_group<T, 4, 4>(a, colb, v);
+      if (32 < live) chol_group<T, 8, 4>(a, colb, v);
+      if (48 < live) chol_group<T, 12, 4>(a, colb, v);
     }
-    chol_steps<T, J + 1, NB>(a, i, n, j0, info, failed);
+    chol_wave_sync();  // colb is rewritten by the next step
   }
-}
-
-// One wavefront factors the NB x NB diagonal block: lane i holds row i in
-// registers; column j's entries are broadcast with v_readlane (static lane
-// and register indices, no barrier); right-looking rank-1 updates.  The
-// block goes through LDS only to be read and written column-coalesced.
-template <typename T>
-__global__ __launch_bounds__(64) void chol_leaf_kernel(int n, T *A, long lda, int j0, int *info) {
-  constexpr int NB = CHOL_NB;
-  __shared__ T S[NB * (NB + 1)];  // S[c*(NB+1) + r] = A(r, c)
-  if (*(volatile int *)info != 0) return;  // an earlier column already failed
-  const int i = threadIdx.x;
-  for (int c = 0; c < NB; ++c) S[c * (NB + 1) + i] = (i < n && c < n && i >= c) ? A[i + (long)c * lda] : T(0);
-  __builtin_amdgcn_s_waitcnt(0);
-  __builtin_amdgcn_wave_barrier();
-  T a[NB];
-#pragma unroll
-  for (int c = 0; c < NB; ++c) a[c] = S[c * (NB + 1) + i];
-  bool failed = false;
-  chol_steps<T, 0, NB>(a, i, n, j0, info, failed);
-  if (failed) return;
-#pragma unroll
-  for (int c = 0; c < NB; ++c) S[c * (NB + 1) + i] = a[c];
-  __builtin_amdgcn_s_waitcnt(0);
-  __builtin_amdgcn_wave_barrier();
+  chol_wave_sync();
+#pragma unroll 8
   for (int c = 0; c < NB; ++c)
     if (i < n && c < n && i >= c) A[i + (long)c * lda] = S[c * (NB + 1) + i];
 }

@@ -7,85 +7,95 @@
 //
 // Recursive blocking: split T at a multiple of 64, solve the leading block,
 // update the other half of B with one GEMM (MFMA), solve the trailing
-// block.  Leaves (n <= 64) run one thread per right-hand-side column with
-// the column in registers and the 64x64 diagonal block in LDS, stored
-// column-major so each elimination step reads a contiguous LDS run.
-// Non-unit leaves divide by the diagonal, as the reference's TRSVs do.
+// block.  Leaves (n <= 64) solve one right-hand-side column per thread.
 #include <type_traits>
 
 #include "ops.hpp"
 
 namespace mg {
 
-template <int B, int E, typename F>
-__device__ __forceinline__ void sfor(F &&f) {
-  if constexpr (B < E) {
-    f(std::integral_constant<int, B>{});
-    sfor<B + 1, E>(f);
-  }
-}
-
 constexpr int TRSM_TB = 64;
 
 constexpr int TRSM_NT = 64;  // one wavefront per workgroup: more workgroups per solve
 
+// One step's update of the remaining rows in the slots [4*C0, 4*(C0+NC)):
+// x[s-1] = x[s] - col[s] * xc (the rows move down one slot as they
+// update).  The group's LDS reads are all issued before its first update:
+// one LDS latency per group, not per four slots.
+template <typename T, int C0, int NC>
+__device__ __forceinline__ void trsm_group(T (&x)[TRSM_TB], const T *col, T xc) {
+  T v[4 * NC];
+#pragma unroll
+  for (int k = 0; k < NC; ++k) {
+    if constexpr (sizeof(T) == 4) {
+      const float4 q = reinterpret_cast<const float4 *>(col)[C0 + k];
+      v[4 * k] = q.x, v[4 * k + 1] = q.y, v[4 * k + 2] = q.z, v[4 * k + 3] = q.w;
+    } else {
+      const double2 q0 = reinterpret_cast<const double2 *>(col)[2 * (C0 + k)];
+      const double2 q1 = reinterpret_cast<const double2 *>(col)[2 * (C0 + k) + 1];
+      v[4 * k] = q0.x, v[4 * k + 1] = q0.y, v[4 * k + 2] = q1.x, v[4 * k + 3] = q1.y;
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < 4 * NC; ++u) {
+    const int sl = 4 * C0 + u;
+    if (sl >= 1) x[sl - 1] = fma(-v[u], xc, x[sl]);
+  }
+}
+
+// Leaves (n <= 64): one thread per right-hand-side column with the column
+// in registers and the triangle in LDS in a skewed layout, Sk[c][s] =
+// T(c + s, c) (lower) or T(c - s, c) (upper), zero outside the triangle,
+// so the step for column c reads one contiguous aligned run.  The column
+// loop is a runtime loop: the current element sits in slot 0, each step
+// stores it (finished) and moves the remaining rows down one slot while
+// updating them, in groups of slots that uniform branches skip once no
+// remaining row is in them.  (A fully unrolled 64-step solve is ~9k
+// instructions, and executed once per launch it runs from a cold
+// instruction cache.)  Upper solves run the same loop from the last row.
+// Non-unit leaves divide by the diagonal, as the reference's TRSVs do.
 template <typename T, bool LOWER, bool UNIT>
 __global__ __launch_bounds__(TRSM_NT) void trsm_leaf_kernel(int n, int nrhs, View<const T> Tm,
                                                             View<T> B) {
   constexpr int TB = TRSM_TB;
-  __shared__ T S[TB * (TB + 1)];  // S[c*(TB+1) + r] = T(r, c)
+  __shared__ __align__(16) T Sk[TB * TB];
   {
-    // issue every load before the LDS stores (one latency, not 64)
-    constexpr int PER = TB * TB / TRSM_NT;
-    T v[PER];
+    // every load issued before the LDS stores (one memory latency, not TB)
+    const int sl = threadIdx.x;
+    T v[TB];
 #pragma unroll
-    for (int u = 0; u < PER; ++u) {
-      const int idx = threadIdx.x + u * TRSM_NT;
-      const int r = Tm.rs == 1 ? idx % TB : idx / TB;
-      const int c = Tm.rs == 1 ? idx / TB : idx % TB;
-      v[u] = (r < n && c < n) ? Tm(r, c) : T(0);
+    for (int c = 0; c < TB; ++c) {
+      const int r = LOWER ? c + sl : c - sl;
+      v[c] = (c < n && r >= 0 && r < n) ? Tm(r, c) : T(0);
     }
 #pragma unroll
-    for (int u = 0; u < PER; ++u) {
-      const int idx = threadIdx.x + u * TRSM_NT;
-      const int r = Tm.rs == 1 ? idx % TB : idx / TB;
-      const int c = Tm.rs == 1 ? idx / TB : idx % TB;
-      S[c * (TB + 1) + r] = v[u];
-    }
+    for (int c = 0; c < TB; ++c) Sk[c * TB + sl] = v[c];
   }
   __syncthreads();
   const int j = blockIdx.x * TRSM_NT + threadIdx.x;
   if (j >= nrhs) return;
   T x[TB];
-  sfor<0, TB>([&](auto rc) {
-    constexpr int r = decltype(rc)::value;
-    x[r] = (r < n) ? B(r, j) : T(0);
-  });
-  if constexpr (LOWER) {
-    sfor<0, TB>([&](auto cc) {
-      constexpr int c = decltype(cc)::value;
-      if (c < n) {
-        if constexpr (!UNIT) x[c] = x[c] / S[c * (TB + 1) + c];
-        const T xc = x[c];
 #pragma unroll
-        for (int r = c + 1; r < TB; ++r) x[r] = fma(-S[c * (TB + 1) + r], xc, x[r]);
-      }
-    });
-  } else {
-    sfor<0, TB>([&](auto cc) {
-      constexpr int c = TB - 1 - decltype(cc)::value;
-      if (c < n) {
-        if constexpr (!UNIT) x[c] = x[c] / S[c * (TB + 1) + c];
-        const T xc = x[c];
-#pragma unroll
-        for (int r = 0; r < c; ++r) x[r] = fma(-S[c * (TB + 1) + r], xc, x[r]);
-      }
-    });
+  for (int r = 0; r < TB; ++r) x[r] = r < n ? B(LOWER ? r : n - 1 - r, j) : T(0);
+#pragma unroll 1
+  for (int st = 0; st < n; ++st) {
+    const int c = LOWER ? st : n - 1 - st;
+    const T *col = Sk + c * TB;
+    if constexpr (!UNIT) x[0] = x[0] / col[0];
+    const T xc = x[0];
+    B(c, j) = xc;
+    const int live = n - st;  // slots 0 .. live-1 hold this and the remaining rows
+    // groups of slots past the remaining rows are skipped (uniform branches)
+    if constexpr (sizeof(T) == 4) {
+      trsm_group<T, 0, 8>(x, col, xc);
+      if (32 < live) trsm_group<T, 8, 8>(x, col, xc);
+    } else {
+      trsm_group<T, 0, 4>(x, col, xc);
+      if (16 < live) trsm_group<T, 4, 4>(x, col, xc);
+      if (32 < live) trsm_group<T, 8, 4>(x, col, xc);
+      if (48 < live) trsm_group<T, 12, 4>(x, col, xc);
+    }
   }
-  sfor<0, TB>([&](auto rc) {
-    constexpr int r = decltype(rc)::value;
-    if (r < n) B(r, j) = x[r];
-  });
 }
 
 // C -= A * B with the requested multiply precision.
