@@ -238,9 +238,8 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmP<T> p) {
     }
 }
 
-template <typename T, bool A_MN, bool B_MN>
-static void launch_mode(hipStream_t s, GemmP<T> &p, bool vec) {
-  constexpr int BM = 128, BN = 128, BK = 16;
+template <typename T, bool A_MN, bool B_MN, int BM, int BN, int BK>
+static void launch_tiles(hipStream_t s, GemmP<T> &p, bool vec) {
   p.tiles_m = ceil_div(p.M, BM);
   p.tiles_n = ceil_div(p.N, BN);
   p.tiles = p.tiles_m * p.tiles_n;
@@ -252,6 +251,30 @@ static void launch_mode(hipStream_t s, GemmP<T> &p, bool vec) {
   else
     hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, A_MN, B_MN, false>), dim3(grid), dim3(256), 0, s, p);
   MG_HIP(hipGetLastError());
+}
+
+// Tile shape by problem shape.  Large products: 128 x 128 tiles walking K
+// in 16-deep slabs.  Products with fewer 128 x 128 tiles than the device
+// has CUs (the skinny updates on the critical path of the factorizations)
+// get one workgroup per CU at most, and every slab then costs a memory
+// round trip that nothing hides; 64 x 64 tiles (four times the workgroups,
+// two per CU) with 64-deep slabs (32 for f64, for LDS) take a quarter of the
+// round trips.
+template <typename T, bool A_MN, bool B_MN>
+static void launch_mode(hipStream_t s, GemmP<T> &p, bool vec, int in_place) {
+  const long tiles128 = (long)ceil_div(p.M, 128) * ceil_div(p.N, 128);
+  if (in_place) {
+    // C is an operand (B := D B, or its transpose B^T := B^T D^T): safe only
+    // when every workgroup reads exactly the part of the operand it
+    // overwrites, i.e. one tile along M (C is B) or along N (C is A)
+    MG_CHECK(in_place == 2 ? p.M <= 128 : p.N <= 128, "gemm: in-place product spans more than one tile");
+    launch_tiles<T, A_MN, B_MN, 128, 128, 16>(s, p, vec);
+  } else if (tiles128 < 256 && p.K >= 32) {
+    constexpr int BKS = sizeof(T) == 4 ? 64 : 32;
+    launch_tiles<T, A_MN, B_MN, 64, 64, BKS>(s, p, vec);
+  } else {
+    launch_tiles<T, A_MN, B_MN, 128, 128, 16>(s, p, vec);
+  }
 }
 
 template <typename T>
@@ -289,10 +312,14 @@ void gemm(hipStream_t s, long M, long N, long K, T alpha, View<const T> A, View<
     return ((uintptr_t)ptr % 16 == 0) && (ld % VW == 0);
   };
   const bool vec = aligned(p.A, p.lda) && aligned(p.B, p.ldb);
-  if (a_mn && !b_mn) launch_mode<T, true, false>(s, p, vec);
-  else if (a_mn && b_mn) launch_mode<T, true, true>(s, p, vec);
-  else if (!a_mn && !b_mn) launch_mode<T, false, false>(s, p, vec);
-  else launch_mode<T, false, true>(s, p, vec);
+  // C is the same buffer as an operand (B := D B, the in-place application
+  // of an inverted diagonal block); disjoint sub-views of one matrix, as in
+  // the factorizations' updates, are not in place
+  const int in_place = (const void *)C.p == (const void *)B.p ? 2 : ((const void *)C.p == (const void *)A.p ? 1 : 0);
+  if (a_mn && !b_mn) launch_mode<T, true, false>(s, p, vec, in_place);
+  else if (a_mn && b_mn) launch_mode<T, true, true>(s, p, vec, in_place);
+  else if (!a_mn && !b_mn) launch_mode<T, false, false>(s, p, vec, in_place);
+  else launch_mode<T, false, true>(s, p, vec, in_place);
 }
 
 template void gemm<float>(hipStream_t, long, long, long, float, View<const float>,
