@@ -268,7 +268,11 @@ static void launch_mode(hipStream_t s, GemmP<T> &p, bool vec, int in_place) {
     // when every workgroup reads exactly the part of the operand it
     // overwrites, i.e. one tile along M (C is B) or along N (C is A)
     MG_CHECK(in_place == 2 ? p.M <= 128 : p.N <= 128, "gemm: in-place product spans more than one tile");
-    launch_tiles<T, A_MN, B_MN, 128, 128, 16>(s, p, vec);
+    // the other dimension in 64-wide tiles and K in deep slabs: these
+    // products are small and latency-bound (one round trip per slab)
+    constexpr int BKI = sizeof(T) == 4 ? 64 : 32;
+    if (in_place == 2) launch_tiles<T, A_MN, B_MN, 128, 64, BKI>(s, p, vec);
+    else launch_tiles<T, A_MN, B_MN, 64, 128, BKI>(s, p, vec);
   } else if (tiles128 < 256 && p.K >= 32) {
     constexpr int BKS = sizeof(T) == 4 ? 64 : 32;
     launch_tiles<T, A_MN, B_MN, 64, 64, BKS>(s, p, vec);
