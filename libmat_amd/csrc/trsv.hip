@@ -106,7 +106,8 @@ __global__ __launch_bounds__(256) void tri_inv_kernel(int n, View<const T> Tm, T
   __syncthreads();
   // each thread owns 4 x 4 micro-tiles of the off-diagonal blocks; the
   // triangular operands are zero outside their triangle (X starts at zero),
-  // so the k ranges may overshoot to the micro-tile's bound
+  // so the k ranges may overshoot to the micro-tile's bound.  The k loops
+  // are unrolled by four so that four steps' LDS reads are in flight at once.
   constexpr int MT = 4;
   for (int sz = BS; sz < TB; sz *= 2) {
     const int tiles = sz / MT;                    // micro-tiles per block side
@@ -116,6 +117,7 @@ __global__ __launch_bounds__(256) void tri_inv_kernel(int n, View<const T> Tm, T
       const int q = 2 * sz * p;
       T acc[MT][MT] = {};
       if (LOWER) {  // Tmp = C A^-1, A^-1(k, j) = 0 for k < j
+#pragma unroll 4
         for (int k = j0; k < sz; ++k) {
           T x[MT], y[MT];
 #pragma unroll
@@ -128,6 +130,7 @@ __global__ __launch_bounds__(256) void tri_inv_kernel(int n, View<const T> Tm, T
             for (int v = 0; v < MT; ++v) acc[u][v] = fma(x[u], y[v], acc[u][v]);
         }
       } else {  // Tmp = C B^-1, B^-1(k, j) = 0 for k > j
+#pragma unroll 4
         for (int k = 0; k < j0 + MT; ++k) {
           T x[MT], y[MT];
 #pragma unroll
@@ -155,6 +158,7 @@ __global__ __launch_bounds__(256) void tri_inv_kernel(int n, View<const T> Tm, T
       // lower: -B^-1 Tmp, B^-1(i, k) = 0 for k > i;  upper: -A^-1 Tmp, A^-1(i, k) = 0 for k < i
       const int k0 = LOWER ? 0 : i0, k1 = LOWER ? i0 + MT : sz;
       const int xr = LOWER ? q + sz : q, xc = LOWER ? q + sz : q;
+#pragma unroll 4
       for (int k = k0; k < k1; ++k) {
         T x[MT], y[MT];
 #pragma unroll
