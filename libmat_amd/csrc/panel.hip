@@ -196,6 +196,39 @@ __device__ inline T value_of(const uint32_t *w) {
   }
 }
 
+// A row of N values between registers and a 16-byte aligned LDS row, in
+// 16-byte accesses.
+template <typename T, int N>
+__device__ __forceinline__ void lds_store_row(T *dst, const T (&v)[N]) {
+  if constexpr (sizeof(T) == 4) {
+    float4 *d = reinterpret_cast<float4 *>(dst);
+#pragma unroll
+    for (int i = 0; i < N / 4; ++i) d[i] = make_float4(v[4 * i], v[4 * i + 1], v[4 * i + 2], v[4 * i + 3]);
+  } else {
+    double2 *d = reinterpret_cast<double2 *>(dst);
+#pragma unroll
+    for (int i = 0; i < N / 2; ++i) d[i] = make_double2(v[2 * i], v[2 * i + 1]);
+  }
+}
+template <typename T, int N>
+__device__ __forceinline__ void lds_load_row(const T *src, T (&v)[N]) {
+  if constexpr (sizeof(T) == 4) {
+    const float4 *s4 = reinterpret_cast<const float4 *>(src);
+#pragma unroll
+    for (int i = 0; i < N / 4; ++i) {
+      const float4 x = s4[i];
+      v[4 * i] = x.x, v[4 * i + 1] = x.y, v[4 * i + 2] = x.z, v[4 * i + 3] = x.w;
+    }
+  } else {
+    const double2 *s2 = reinterpret_cast<const double2 *>(src);
+#pragma unroll
+    for (int i = 0; i < N / 2; ++i) {
+      const double2 x = s2[i];
+      v[2 * i] = x.x, v[2 * i + 1] = x.y;
+    }
+  }
+}
+
 template <typename T, int WMAX, int RPT>
 __global__ __launch_bounds__(PanelNT<T>::value) void panel_gepp_kernel(PanelP<T> p) {
   constexpr int PANEL_NT = PanelNT<T>::value;
@@ -277,12 +310,8 @@ __global__ __launch_bounds__(PanelNT<T>::value) void panel_gepp_kernel(PanelP<T>
       if (lane == 0) wkey[par][wave] = key;
 #pragma unroll
       for (int q = 0; q < RPT; ++q) {
-        if (row[q] == wbest)
-#pragma unroll
-          for (int c = 0; c < WMAX; ++c) wrow[par][wave][c] = a[q][c];
-        if (row[q] == k)
-#pragma unroll
-          for (int c = 0; c < WMAX; ++c) krow[par][c] = a[q][c];
+        if (row[q] == wbest) lds_store_row<T, WMAX>(wrow[par][wave], a[q]);
+        if (row[q] == k) lds_store_row<T, WMAX>(krow[par], a[q]);
       }
       __syncthreads();
       // 2. workgroup candidate, lane-parallel in wavefront 0: lane x < 8
@@ -414,42 +443,33 @@ __global__ __launch_bounds__(PanelNT<T>::value) void panel_gepp_kernel(PanelP<T>
       }
       if (b == 0 && t == 0) p.ipiv[k] = piv;
       if (t == 0) s_hist[j] = piv;
+      // the pivot row into registers (16-byte LDS reads; every row of
+      // wrow/krow/stage is 16-byte aligned): the update needs it, and row k
+      // takes it over on an interchange
+      // (f64 rows of 64 columns: two rows in registers at once would not
+      // fit, so row k reads the pivot row from LDS before it is loaded)
+      constexpr bool PRA_FIRST = !(sizeof(T) == 8 && WMAX == 64);
+      T pra[WMAX];
+      if constexpr (PRA_FIRST) lds_load_row<T, WMAX>(pr, pra);
       if (piv != k) {
 #pragma unroll
         for (int q = 0; q < RPT; ++q) {
           if (row[q] == k) {
+            if constexpr (PRA_FIRST) {
 #pragma unroll
-            for (int c = 0; c < WMAX; ++c) a[q][c] = pr[c];
+              for (int c = 0; c < WMAX; ++c) a[q][c] = pra[c];
+            } else {
+              lds_load_row<T, WMAX>(pr, a[q]);
+            }
           } else if (row[q] == piv) {
-#pragma unroll
-            for (int c = 0; c < WMAX; ++c) a[q][c] = dr[c];
+            lds_load_row<T, WMAX>(dr, a[q]);
           }
         }
       }
+      if constexpr (!PRA_FIRST) lds_load_row<T, WMAX>(pr, pra);
       // 5. scale (1/pivot, then multiply) and rank-1 update of the columns
       //    still to factor, fused with the rotation: column c lands in slot
       //    c-1 and column 0 (now L) in the last slot
-      // the pivot row into registers with 16-byte LDS reads (pr is 16-byte aligned)
-      T pra[WMAX];
-      if constexpr (sizeof(T) == 4) {
-        const float4 *p4 = reinterpret_cast<const float4 *>(pr);
-#pragma unroll
-        for (int i = 0; i < WMAX / 4; ++i) {
-          const float4 v = p4[i];
-          pra[4 * i] = v.x;
-          pra[4 * i + 1] = v.y;
-          pra[4 * i + 2] = v.z;
-          pra[4 * i + 3] = v.w;
-        }
-      } else {
-        const double2 *p2 = reinterpret_cast<const double2 *>(pr);
-#pragma unroll
-        for (int i = 0; i < WMAX / 2; ++i) {
-          const double2 v = p2[i];
-          pra[2 * i] = v.x;
-          pra[2 * i + 1] = v.y;
-        }
-      }
       const T pv = pra[0];
       if (!((T)fabs(pv) < p.eps)) {
         const T inv = T(1) / pv;
