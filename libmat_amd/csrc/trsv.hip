@@ -265,6 +265,33 @@ __host__ __device__ inline long fold_ld(bool lower, int r, int n, int TB) {
 
 // FOLD: tiles are the folded T'(r,k) and the workgroup starts from D_r b_r,
 // so the product with the diagonal block leaves the chain's critical path.
+// fp32 tile product on MFMA for the chained solve: acc (+|-)= T(16w.., :)
+// X(:, 0..15) for wavefront w's 16 rows of the 128-row block, with
+// v_mfma_f32_16x16x4f32 (A: lane l gives row l%16, k l/16; B: k l/16,
+// column l%16; result: register r of lane l is row 4*(l/16) + r, column
+// l%16).  T is in LDS as S[row * ld + k], X as Xs[k * 16 + column].  Two
+// accumulators let consecutive MFMAs overlap.
+typedef float chain_f32x4 __attribute__((ext_vector_type(4)));
+template <bool NEG>
+__device__ __forceinline__ void chain_mfma(const float *S, int ld, const float *Xs, chain_f32x4 &acc) {
+  const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const float *arow = S + (16 * w + (l & 15)) * ld;
+  const int g = l >> 4, c = l & 15;
+  chain_f32x4 acc2 = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (int st = 0; st < 32; st += 2) {
+    float a0 = arow[4 * st + g], a1 = arow[4 * st + 4 + g];
+    const float b0 = Xs[(4 * st + g) * CHAIN_NC + c], b1 = Xs[(4 * st + 4 + g) * CHAIN_NC + c];
+    if (NEG) {
+      a0 = -a0;
+      a1 = -a1;
+    }
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b0, acc, 0, 0, 0);
+    acc2 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, acc2, 0, 0, 0);
+  }
+  acc += acc2;
+}
+
 template <typename T, bool LOWER, bool FOLD>
 __global__ __launch_bounds__(ChainNT<T>::NT) void tri_chain_kernel(int n, int nrhs, View<const T> Tm,
                                                                     const T *D, const T *Tp, T *X,
@@ -278,6 +305,9 @@ __global__ __launch_bounds__(ChainNT<T>::NT) void tri_chain_kernel(int n, int nr
   constexpr int LDS = TB + 1;
   constexpr int NW = ChainWords<T>::N;
   constexpr int BW = TB * NC * NW;  // published words per block, [c][j][h]
+  // fp32 folded solves run their tile products on MFMA (8 wavefronts x 16
+  // rows = the 128-row block, all 16 right-hand sides)
+  constexpr bool MF = std::is_same<T, float>::value && FOLD && NT == 512 && TB == 128 && NC == 16;
   __shared__ T S[2][TB * LDS];
   __shared__ T Xs[TB * NC];  // [row j][rhs c]
   __shared__ int s_rb, s_fail;
@@ -304,6 +334,7 @@ __global__ __launch_bounds__(ChainNT<T>::NT) void tri_chain_kernel(int n, int nr
     auto tcol = [&](int k) { return FOLD ? (LOWER ? k : k - r - 1) * TB : k * TB; };
     T v[PER];
     T acc[CPT];
+    chain_f32x4 macc = {0.0f, 0.0f, 0.0f, 0.0f};
     if constexpr (FOLD) {
       // acc = D_r b_r while the earlier blocks are still being solved
       chain_load<T, TB, NT>(Dv, 0, TB, 0, TB, v);
@@ -315,13 +346,17 @@ __global__ __launch_bounds__(ChainNT<T>::NT) void tri_chain_kernel(int n, int nr
         Xs[j * NC + c] = (j < rrows && c < nrhs) ? X[(ro + j) + (long)c * ldx] : T(0);
       }
       __syncthreads();
+      if constexpr (MF) {
+        chain_mfma<false>(S[1], LDS, Xs, macc);
+      } else {
 #pragma unroll
-      for (int q = 0; q < CPT; ++q) acc[q] = T(0);
+        for (int q = 0; q < CPT; ++q) acc[q] = T(0);
 #pragma unroll 8
-      for (int j = 0; j < TB; ++j) {
-        const T a = S[1][i * LDS + j];
+        for (int j = 0; j < TB; ++j) {
+          const T a = S[1][i * LDS + j];
 #pragma unroll
-        for (int q = 0; q < CPT; ++q) acc[q] = fma(a, Xs[j * NC + g * CPT + q], acc[q]);
+          for (int q = 0; q < CPT; ++q) acc[q] = fma(a, Xs[j * NC + g * CPT + q], acc[q]);
+        }
       }
       __syncthreads();
       if (cnt > 0) {
@@ -386,11 +421,15 @@ __global__ __launch_bounds__(ChainNT<T>::NT) void tri_chain_kernel(int n, int nr
         if (t == 0) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         return;
       }
+      if constexpr (MF) {
+        chain_mfma<true>(Sb, LDS, Xs, macc);
+      } else {
 #pragma unroll 8
-      for (int j = 0; j < TB; ++j) {
-        const T a = Sb[i * LDS + j];
+        for (int j = 0; j < TB; ++j) {
+          const T a = Sb[i * LDS + j];
 #pragma unroll
-        for (int q = 0; q < CPT; ++q) acc[q] = fma(-a, Xs[j * NC + g * CPT + q], acc[q]);
+          for (int q = 0; q < CPT; ++q) acc[q] = fma(-a, Xs[j * NC + g * CPT + q], acc[q]);
+        }
       }
       __syncthreads();
     }
@@ -416,6 +455,25 @@ __global__ __launch_bounds__(ChainNT<T>::NT) void tri_chain_kernel(int n, int nr
     }
     // publish first (the chain waits on it), then the result itself
     uint64_t *dst = pub + (long)r * BW;
+    if constexpr (MF) {
+      // register q of lane l: row 16*wave + 4*(l/16) + q, column l%16
+      const int l = t & 63, wv = t >> 6;
+      const int c = l & 15;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int row = 16 * wv + 4 * (l >> 4) + q;
+        __hip_atomic_store(dst + (long)c * TB + row,
+                           ((uint64_t)tag << 32) | chain_word<T>(macc[q], 0), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int row = 16 * wv + 4 * (l >> 4) + q;
+        if (row < rrows && c < nrhs) X[(ro + row) + (long)c * ldx] = macc[q];
+      }
+      __syncthreads();  // Xs and S are reused by the next claim
+      continue;
+    }
 #pragma unroll
     for (int q = 0; q < CPT; ++q) {
       const int c = g * CPT + q;
