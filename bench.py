@@ -83,7 +83,34 @@ def kernel_table(lib):
     return out
 
 
-def roofline(table, family, gemm_dtype="fp32", steps=1):
+def hbm_traffic(family, workload):
+    """HBM bytes per launch of one kernel family on one workload, from the
+    rocprofv3 FETCH_SIZE / WRITE_SIZE passes committed under profiles/
+    (hbm_traffic__<kernels>__<workload>.md, calibrated against a copy kernel
+    of known size).  None when no pass covers it or the workload was run at
+    other sizes than the profiled one."""
+    if workload is None:
+        return None
+    import glob
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", f"hbm_traffic__*__{workload}.md"))):
+        with open(path) as f:
+            rows = [ln.strip().strip("|").split("|") for ln in f if ln.startswith("|")]
+        if not rows:
+            continue
+        head = [c.strip() for c in rows[0]]
+        if "family" not in head or "HBM bytes per launch" not in head:
+            continue
+        fi, ti = head.index("family"), head.index("HBM bytes per launch")
+        for r in rows[1:]:
+            if len(r) > max(fi, ti) and r[fi].strip() == family:
+                try:
+                    return int(float(r[ti].strip()))
+                except ValueError:
+                    return None
+    return None
+
+
+def roofline(table, family, gemm_dtype="fp32", steps=1, workload=None):
     """Roofline object for one kernel family from the library's HIP-event table."""
     rec = table[family]
     bound, unit = BOUND[family]
@@ -95,7 +122,8 @@ def roofline(table, family, gemm_dtype="fp32", steps=1):
         achieved = rec["work"] / sec / 1e12
         peak = PEAK[{"gemm_bf16x": "bf16", "gemm_fp8": "fp8"}.get(family, gemm_dtype)]
     return {"kernel": family, "bound": bound, "achieved": round(achieved, 3), "peak": peak,
-            "unit": unit, "frac": round(achieved / peak, 5), "traffic": None,
+            "unit": unit, "frac": round(achieved / peak, 5),
+            "traffic": hbm_traffic(family, workload),
             "launches": rec["launches"], "avg_launch_us": round(rec["ms"] * 1e3 / rec["launches"], 3),
             "ms_per_step": round(rec["ms"] / steps, 3)}
 
@@ -192,6 +220,18 @@ def make_general(torch, n, kappas, nrhs, seed=2026):
     return mats, B.contiguous()
 
 
+def make_uniform(torch, n, count, nrhs, seed=2028):
+    """count systems with entries ~ U[-1,1] (fp64) and one B: the same shapes
+    and launches as the conditioned systems without a QR factorization (for
+    counter profilers that cannot run the vendor QR)."""
+    gen = torch.Generator(device="cuda").manual_seed(seed)
+    mats = [(torch.rand(n, n, dtype=torch.float64, device="cuda", generator=gen) * 2 - 1)
+            for _ in range(count)]
+    B = (torch.rand(nrhs, n, dtype=torch.float64, device="cuda", generator=gen) * 2 - 1)
+    torch.cuda.synchronize()
+    return mats, B.contiguous()
+
+
 def berr_fp64(torch, A, X, B):
     """Normwise backward error (inf norms) with A as a column-major buffer."""
     n = A.shape[0]
@@ -214,7 +254,11 @@ def run_mixed(torch, libmat_amd, args, spd):
     n, nrhs = args.n, args.nrhs
     kappas = [float(k) for k in args.kappas.split(",")]
     log(f"generating {len(kappas)} {'SPD' if spd else 'general'} systems n={n}")
-    mats, B = (make_spd(torch, n, kappas, nrhs) if spd else make_general(torch, n, kappas, nrhs))
+    if args.data == "uniform":
+        mats, B = make_uniform(torch, n, len(kappas), nrhs)
+    else:
+        mats, B = (make_spd(torch, n, kappas, nrhs) if spd else
+                   make_general(torch, n, kappas, nrhs))
     X = torch.zeros(nrhs * n, dtype=torch.float64, device="cuda")
     stream = torch.cuda.current_stream().cuda_stream
     opts = libmat_amd.IrOpts()
@@ -300,8 +344,10 @@ def run_mixed(torch, libmat_amd, args, spd):
            "scaling": "weak", "vs_baseline": None,
            "dtype": ("f32+fp8 factorization, f64 refinement" if spd else
                      "f32+bf16 factorization, f64 refinement"),
-           "data": "synthetic (seeded; " + ("Q diag(lambda) Q^T, Q = QR of Gaussian" if spd else
-                                             "U diag(sigma) V^T, U,V = QR of Gaussian") + ")",
+           "data": ("synthetic (seeded; U[-1,1] entries, kappa not controlled)"
+                    if args.data == "uniform" else
+                    "synthetic (seeded; " + ("Q diag(lambda) Q^T, Q = QR of Gaussian" if spd else
+                                             "U diag(sigma) V^T, U,V = QR of Gaussian") + ")"),
            "config": {"workload": name, "n": n, "nrhs": nrhs, "kappas": kappas,
                       "factorization": ("fp32 Cholesky, FP8 e4m3 per-128x128-tile-scaled MFMA "
                                         "trailing updates" if spd else
@@ -323,10 +369,11 @@ def run_mixed(torch, libmat_amd, args, spd):
     del mats
     torch.cuda.empty_cache()
     dom = dominant(table)
-    out["roofline"] = roofline(table, dom, steps=args.steps)
+    out["roofline"] = roofline(table, dom, steps=args.steps, workload=args.profiled)
     gem = "gemm_fp8" if spd else "gemm_bf16x"
     if gem in table and gem != dom:
-        out["roofline_trailing_update"] = roofline(table, gem, steps=args.steps)
+        out["roofline_trailing_update"] = roofline(table, gem, steps=args.steps,
+                                                   workload=args.profiled)
     if not args.no_cpu:
         out["cpu_baseline"] = (cpu_chol(args.cpu_n, nrhs, "f64") if spd else
                                cpu_solve_multi(args.cpu_n, nrhs))
@@ -407,7 +454,7 @@ def run_gemm(torch, libmat_amd, args):
            "config": {"workload": "fp32 GEMM (mat_gemm)", "m": n, "n": n, "k": n,
                       "parallelism": "single GPU"},
            "max_rel_err_vs_fp64_sampled_rows": rel, "kernels": breakdown(table, args.steps),
-           "roofline": roofline(table, "gemm", "fp32", args.steps)}
+           "roofline": roofline(table, "gemm", "fp32", args.steps, args.profiled)}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_gemm(min(args.cpu_n, 1024))
     return out
@@ -456,7 +503,7 @@ def run_chol(torch, libmat_amd, args):
            "config": {"workload": "fp32 Cholesky solve (mat_chol + mat_solve_spd)", "n": n,
                       "nrhs": nrhs, "parallelism": "single GPU"},
            "backward_error": berr, "kernels": breakdown(table, args.steps),
-           "roofline": roofline(table, dominant(table), "fp32", args.steps)}
+           "roofline": roofline(table, dominant(table), "fp32", args.steps, args.profiled)}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_chol(min(args.cpu_n, 1024), nrhs, "f32")
     return out
@@ -608,7 +655,8 @@ def run_batched(torch, libmat_amd, args):
            "roofline": {"kernel": "batched persistent solve (group_lu / group_chol)",
                         "bound": "hbm", "achieved": round(achieved, 3), "peak": PEAK["hbm"],
                         "unit": "GB/s", "frac": round(achieved / PEAK["hbm"], 5),
-                        "traffic": None, "avg_launch_us": round(batch_ms * 1e3, 3)}}
+                        "traffic": hbm_traffic("batched", args.profiled),
+                        "avg_launch_us": round(batch_ms * 1e3, 3)}}
     if not args.no_cpu:
         o, kind = _oracle("f32")
         csample = min(args.cpu_count, count)
@@ -651,6 +699,9 @@ def main():
     ap.add_argument("--cpu-count", type=int, default=20000)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--data", default="conditioned", choices=["conditioned", "uniform"],
+                    help="mixed-lu: U[-1,1] systems instead of the conditioned family (same "
+                         "shapes; for counter profilers that cannot run the vendor QR)")
     ap.add_argument("--cpu-n", type=int, default=1024)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -664,6 +715,12 @@ def main():
     args.nrhs = args.nrhs or dr
     args.kappas = args.kappas or dk
     args.steps = args.steps or ds
+    # workload name of the committed HBM-traffic passes, when run at their sizes
+    tags = {"mixed-lu": "mixed_lu_n16384", "mixed-spd": "mixed_spd_n32768", "gemm": "gemm_8192",
+            "chol": "chol_8192", "batched": "batched_200k"}
+    same = (args.n == dn and args.nrhs == dr and
+            (args.workload != "batched" or args.count == 200000))
+    args.profiled = tags[args.workload] if same else None
 
     import torch
     import libmat_amd

@@ -26,6 +26,9 @@ void lu_leaf(hipStream_t s, int n, T *W, long ld, int c0, int w, int *ipiv, int 
 template <typename T>
 void lu_rowswap(hipStream_t s, T *W, long ld, const int *pairs, int *perm, int k0, int w, int cl,
                 int ch);
+template <typename T>
+void lu_rowswap_block(hipStream_t s, T *W, long ld, const int *pairs, int b0, int b1, int leaf,
+                      int cl, int ch);
 constexpr int kPairStride = 257;  // ints per leaf in the row-pair buffer (panel.hip)
 int lu_leaf_max(int m, int elem_size);  // widest leaf the panel kernel takes for m rows
 
@@ -76,8 +79,7 @@ template <typename T>
 static void block_swaps(hipStream_t s, T *W, long ld, const int *pairs, int b0, int b1, int leaf,
                         int cl, int ch) {
   if (ch <= cl) return;
-  for (int k = b0; k < b1; k += leaf)
-    lu_rowswap<T>(s, W, ld, pairs, nullptr, k, std::min(leaf, b1 - k), cl, ch);
+  lu_rowswap_block<T>(s, W, ld, pairs, b0, b1, leaf, cl, ch);
 }
 
 // U12 = L11^-1 A12 and A22 -= L21 U12 for the factored block [b0, b1) and

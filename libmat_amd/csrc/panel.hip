@@ -633,6 +633,65 @@ void lu_rowswap(hipStream_t s, T *W, long ld, const int *pairs, int *perm, int k
   MG_HIP(hipGetLastError());
 }
 
+// The interchanges of up to BLK_LEAVES consecutive leaves (leaf l starts at
+// column b0 + l * leaf) applied in order to columns [cl, ch) in one launch:
+// each thread owns one column, so applying leaf after leaf in its registers
+// is the sequential order; only the launches are saved.
+constexpr int BLK_LEAVES = 16;
+
+template <typename T>
+__global__ __launch_bounds__(256) void rowswap_block_kernel(T *W, long ld, int cl, int ch,
+                                                            const int *pairs, int b0, int leaf,
+                                                            int nleaf) {
+  constexpr int NP = 64;  // leaves of at most 32 columns
+  __shared__ int dst[BLK_LEAVES][NP], src[BLK_LEAVES][NP];
+  __shared__ int npair[BLK_LEAVES];
+  const int t = threadIdx.x;
+  for (int i = t; i < nleaf * NP; i += 256) {
+    const int l = i / NP, x = i % NP;
+    const int *pr = pairs + (long)(b0 + l * leaf) * PAIR_STRIDE;
+    const int np0 = pr[0];
+    const int s0 = pr[1 + PAIR_MAX];
+    // unused slots copy row src[0] onto itself, stored before every real one
+    dst[l][x] = x < np0 ? pr[1 + x] : s0;
+    src[l][x] = x < np0 ? pr[1 + PAIR_MAX + x] : s0;
+    if (x == 0) npair[l] = np0;
+  }
+  __syncthreads();
+  const int c = cl + blockIdx.x * 256 + t;
+  if (c >= ch) return;
+  T *Wc = W + c;
+  const unsigned uld = (unsigned)ld;
+  for (int l = 0; l < nleaf; ++l) {
+    if (npair[l] == 0) continue;
+    T v[NP];
+#pragma unroll
+    for (int x = 0; x < NP; ++x) v[x] = Wc[(unsigned long)(unsigned)src[l][x] * uld];
+#pragma unroll
+    for (int x = NP - 1; x >= 0; --x) Wc[(unsigned long)(unsigned)dst[l][x] * uld] = v[x];
+  }
+}
+
+// The interchanges of the factored block [b0, b1) (leaves of `leaf`
+// columns, pairs + k * PAIR_STRIDE for the leaf at column k) applied to
+// columns [cl, ch), which lie outside the block.
+template <typename T>
+void lu_rowswap_block(hipStream_t s, T *W, long ld, const int *pairs, int b0, int b1, int leaf,
+                      int cl, int ch) {
+  if (ch <= cl || b1 <= b0) return;
+  const int nleaf = ceil_div(b1 - b0, leaf);
+  if (leaf > 32 || nleaf > BLK_LEAVES) {
+    for (int k = b0; k < b1; k += leaf)
+      lu_rowswap<T>(s, W, ld, pairs, nullptr, k, std::min(leaf, b1 - k), cl, ch);
+    return;
+  }
+  MG_CHECK(ch <= b0 || cl >= b1, "lu_rowswap_block: the columns must lie outside the block");
+  KScope ks(s, "rowswap");
+  hipLaunchKernelGGL((rowswap_block_kernel<T>), dim3(ceil_div(ch - cl, 256)), dim3(256), 0, s, W,
+                     ld, cl, ch, pairs, b0, leaf, nleaf);
+  MG_HIP(hipGetLastError());
+}
+
 // Factor the leaf block W[c0:n, c0:c0+w] (w <= 32) and apply its row swaps
 // to the columns [cl, ch) of the n x n row-major matrix outside the leaf
 // (and to perm).
@@ -711,5 +770,9 @@ template void lu_leaf<double>(hipStream_t, int, double *, long, int, int, int *,
                               int, int, bool);
 template void lu_rowswap<float>(hipStream_t, float *, long, const int *, int *, int, int, int, int);
 template void lu_rowswap<double>(hipStream_t, double *, long, const int *, int *, int, int, int, int);
+template void lu_rowswap_block<float>(hipStream_t, float *, long, const int *, int, int, int, int,
+                                     int);
+template void lu_rowswap_block<double>(hipStream_t, double *, long, const int *, int, int, int, int,
+                                      int);
 
 }  // namespace mg
