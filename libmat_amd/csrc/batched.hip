@@ -49,8 +49,7 @@ struct BatchP {
   int *info;
   const int *order;
   const BatchTask *tasks;
-  int ntasks;
-  unsigned *counter;
+  unsigned *counter;  // one task counter per (bin, kind) class
   float eps;
 };
 
@@ -69,66 +68,141 @@ __device__ __forceinline__ void bfor_down(F &&f) {
   }
 }
 
+// Wavefront-level sync for the LDS hand-offs below (one wavefront owns its
+// buffers): order the LDS stores before the loads that follow.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int CTRL>
+__device__ __forceinline__ uint64_t dpp_max_key(uint64_t k) {
+  const uint32_t lo = __builtin_amdgcn_update_dpp((uint32_t)k, (uint32_t)k, CTRL, 0xf, 0xf, false);
+  const uint32_t hi =
+      __builtin_amdgcn_update_dpp((uint32_t)(k >> 32), (uint32_t)(k >> 32), CTRL, 0xf, 0xf, false);
+  const uint64_t o = ((uint64_t)hi << 32) | lo;
+  return o > k ? o : k;
+}
+
+// Max of a 64-bit key over each aligned group of NB lanes (every lane of the
+// group gets it): DPP within 16-lane rows (xor 1, xor 2, then the half-row
+// and row mirrors, which pair every lane with the other half), a swizzle
+// for xor 16 and readlanes across the two 32-lane halves.
 template <int NB>
-__device__ void group_lu(const BatchP &p, const BatchTask &t, int lane) {
-  const int gid = lane / NB, r = lane % NB, gbase = gid * NB;
+__device__ __forceinline__ uint64_t group_max_key(uint64_t k) {
+  k = dpp_max_key<0xB1>(k);  // quad_perm(1,0,3,2)
+  k = dpp_max_key<0x4E>(k);  // quad_perm(2,3,0,1)
+  k = dpp_max_key<0x141>(k);  // row_half_mirror
+  if constexpr (NB >= 16) k = dpp_max_key<0x140>(k);  // row_mirror
+  if constexpr (NB >= 32) {
+    constexpr int XOR16 = 0x1f | (0x10 << 10);  // bit mode: and 0x1f, xor 0x10
+    const uint32_t lo = __builtin_amdgcn_ds_swizzle((int)(uint32_t)k, XOR16);
+    const uint32_t hi = __builtin_amdgcn_ds_swizzle((int)(uint32_t)(k >> 32), XOR16);
+    const uint64_t o = ((uint64_t)hi << 32) | lo;
+    k = o > k ? o : k;
+  }
+  if constexpr (NB == 64) {
+    const uint64_t a = ((uint64_t)__builtin_amdgcn_readlane((uint32_t)(k >> 32), 0) << 32) |
+                       __builtin_amdgcn_readlane((uint32_t)k, 0);
+    const uint64_t b = ((uint64_t)__builtin_amdgcn_readlane((uint32_t)(k >> 32), 32) << 32) |
+                       __builtin_amdgcn_readlane((uint32_t)k, 32);
+    k = a > b ? a : b;
+  }
+  return k;
+}
+
+// Per-wavefront LDS: two hand-off buffers (by step parity) of 64 row or
+// column values plus one value per group, then (Cholesky) the staged L.
+constexpr int HB = 80;  // floats per hand-off buffer (64 values + 8 groups + pad)
+
+template <int NB>
+__device__ __forceinline__ void store_row(float *dst, const float (&a)[NB], int from) {
+  // whole aligned float4 chunks from the one holding `from` (static after
+  // unrolling); entries before `from` are written but never read
+  bfor<0, NB / 4>([&](auto qc) __attribute__((always_inline)) {
+    constexpr int q = decltype(qc)::value;
+    if (4 * q + 3 >= from)
+      *reinterpret_cast<float4 *>(dst + 4 * q) =
+          make_float4(a[4 * q], a[4 * q + 1], a[4 * q + 2], a[4 * q + 3]);
+  });
+}
+
+// Partial-pivot LU + solve of the matrices of one task (64/NB per
+// wavefront, group g in lanes [g*NB, g*NB+NB), lane r of a group holding row
+// r).  The pivot search is one 64-bit key max per column (|a| bits, then the
+// lowest position, then the lane); the pivot row reaches the group through
+// LDS (one lane stores it, every lane reads it back in float4 chunks).
+template <int NB>
+__device__ __forceinline__ void group_lu(const BatchP &p, const BatchTask &t, int lane, float *buf) {
+  const int gid = lane / NB, r = lane % NB;
   const bool has = gid < t.count;
   const int mi = has ? p.order[t.first + gid] : 0;
   const int n = has ? p.n[mi] : 0;
   const bool active = r < n;
   const float *A = p.A + (has ? p.aoff[mi] : 0);
   float a[NB];
-  bfor<0, NB>([&](auto cc) {
+  bfor<0, NB>([&](auto cc) __attribute__((always_inline)) {
     constexpr int c = decltype(cc)::value;
     a[c] = (active && c < n) ? A[r + (long)c * n] : 0.f;
   });
   float bv = active ? p.b[p.voff[mi] + r] : 0.f;
   int pos = r;
   int pl[NB];
-  bfor<0, NB>([&](auto kc) {
+  bfor<0, NB>([&](auto kc) __attribute__((always_inline)) {
     constexpr int k = decltype(kc)::value;
     const bool kvalid = k < n;
-    // pivot: first position of max |a(:,k)| among positions >= k; the last
-    // column takes the row already at position k
-    float v;
-    if (k + 1 < n) v = (active && pos >= k) ? fabsf(a[k]) : -1.f;
-    else v = (active && pos == k) ? 1.f : -1.f;
-    int vp = pos, vl = lane;
-#pragma unroll
-    for (int off = NB / 2; off >= 1; off >>= 1) {
-      const float ov = __shfl_xor(v, off);
-      const int op = __shfl_xor(vp, off), ol = __shfl_xor(vl, off);
-      if (ov > v || (ov == v && op < vp)) {
-        v = ov;
-        vp = op;
-        vl = ol;
-      }
+    // pivot: first position of max |a(:,k)| among positions >= k (a NaN
+    // counts as 0); the last column takes the row already at position k
+    uint64_t key = 0;
+    const bool elig = active && (k + 1 < n ? pos >= k : pos == k);
+    if (elig) {
+      float v = k + 1 < n ? fabsf(a[k]) : 1.f;
+      if (!(v == v)) v = 0.f;
+      key = ((uint64_t)__float_as_uint(v) << 32) |
+            (uint64_t)((1u << 12) | ((uint32_t)(63 - pos) << 6) | (uint32_t)lane);
     }
-    const int wl = kvalid ? vl : lane;
-    const int wpos = vp;
+    key = group_max_key<NB>(key);
+    const int wl = kvalid ? (int)(key & 63) : lane;
+    const int wpos = 63 - (int)((key >> 6) & 63);
     pl[k] = wl;
     const bool was_k = (pos == k);
     if (kvalid) {
       if (lane == wl) pos = k;
       else if (was_k) pos = wpos;
     }
-    const float ukk = __shfl(a[k], wl);
-    const float yk = __shfl(bv, wl);
+    float *hb = buf + (k & 1) * HB;
+    float *row = hb + gid * NB;
+    if (kvalid && lane == wl) {
+      store_row<NB>(row, a, k);
+      hb[64 + gid] = bv;
+    }
+    wave_lds_sync();
+    const float yk = hb[64 + gid];
+    float u[NB];
+    bfor<k / 4, NB / 4>([&](auto qc) __attribute__((always_inline)) {
+      constexpr int q = decltype(qc)::value;
+      const float4 v = *reinterpret_cast<const float4 *>(row + 4 * q);
+      u[4 * q] = v.x;
+      u[4 * q + 1] = v.y;
+      u[4 * q + 2] = v.z;
+      u[4 * q + 3] = v.w;
+    });
+    const float ukk = u[k];
     const bool upd = kvalid && active && pos > k && !(fabsf(ukk) < p.eps);
     const float l = upd ? a[k] * (1.0f / ukk) : 0.f;
     if (upd) {
       a[k] = l;
       bv = fmaf(-l, yk, bv);
     }
-#pragma unroll
-    for (int c = k + 1; c < NB; ++c) {
-      const float uc = __shfl(a[c], wl);
-      if (upd) a[c] = fmaf(-l, uc, a[c]);
-    }
+    bfor<k + 1, NB>([&](auto cc) __attribute__((always_inline)) {
+      constexpr int c = decltype(cc)::value;
+      if (upd) a[c] = fmaf(-l, u[c], a[c]);
+    });
   });
   // back substitution: lane at position k owns U(k, :)
   float s = bv;
-  bfor_down<0, NB>([&](auto kc) {
+  bfor_down<0, NB>([&](auto kc) __attribute__((always_inline)) {
     constexpr int k = decltype(kc)::value;
     const bool kvalid = k < n;
     const int wl = pl[k];
@@ -140,25 +214,36 @@ __device__ void group_lu(const BatchP &p, const BatchTask &t, int lane) {
   if (has && r == 0) p.info[mi] = 0;
 }
 
+// Cholesky + solve of the matrices of one task.  Column k: the lanes at or
+// below the diagonal store their (unscaled) entries to LDS, every lane reads
+// the diagonal and the entries it needs back, scales them by 1/l(k,k)
+// itself (the same operation lane c performs on its own entry) and updates
+// its row; forward substitution is fused.  L is then staged in LDS for the
+// back substitution.
 template <int NB>
-__device__ void group_chol(const BatchP &p, const BatchTask &t, int lane, float *lds) {
-  const int gid = lane / NB, r = lane % NB, gbase = gid * NB;
+__device__ __forceinline__ void group_chol(const BatchP &p, const BatchTask &t, int lane, float *buf) {
+  const int gid = lane / NB, r = lane % NB;
   const bool has = gid < t.count;
   const int mi = has ? p.order[t.first + gid] : 0;
   const int n = has ? p.n[mi] : 0;
   const bool active = r < n;
   const float *A = p.A + (has ? p.aoff[mi] : 0);
   float a[NB];
-  bfor<0, NB>([&](auto cc) {
+  bfor<0, NB>([&](auto cc) __attribute__((always_inline)) {
     constexpr int c = decltype(cc)::value;
     a[c] = (active && c < n && c <= r) ? A[r + (long)c * n] : 0.f;  // lower triangle only
   });
   float bv = active ? p.b[p.voff[mi] + r] : 0.f;
   bool fail = false;
-  bfor<0, NB>([&](auto kc) {
+  bfor<0, NB>([&](auto kc) __attribute__((always_inline)) {
     constexpr int k = decltype(kc)::value;
     const bool kvalid = k < n;
-    const float d = __shfl(a[k], gbase + k);
+    float *hb = buf + (k & 1) * HB;
+    float *col = hb + gid * NB;
+    if (kvalid && active && r >= k) col[r] = a[k];
+    if (kvalid && r == k) hb[64 + gid] = bv;
+    wave_lds_sync();
+    const float d = col[k];
     if (kvalid && d <= 0.f) fail = true;
     const float lkk = sqrtf(d);
     const float inv = 1.0f / lkk;
@@ -167,68 +252,80 @@ __device__ void group_chol(const BatchP &p, const BatchTask &t, int lane, float 
       else if (r > k) a[k] = a[k] * inv;
     }
     const float lrk = a[k];
-#pragma unroll
-    for (int c = k + 1; c < NB; ++c) {
-      const float lck = __shfl(a[k], gbase + c);
-      if (kvalid && active && r >= c) a[c] = fmaf(-lrk, lck, a[c]);
-    }
+    // the rest of the column in float4 chunks, each used as it arrives
+    bfor<(k + 1) / 4, NB / 4>([&](auto qc) __attribute__((always_inline)) {
+      constexpr int q = decltype(qc)::value;
+      const float4 v = *reinterpret_cast<const float4 *>(col + 4 * q);
+      const float u[4] = {v.x, v.y, v.z, v.w};
+      bfor<0, 4>([&](auto jc) __attribute__((always_inline)) {
+        constexpr int c = 4 * q + decltype(jc)::value;
+        // unpredicated: in rows r < c this writes entries above the
+        // diagonal, and for k >= n or r >= n entries of unused rows and
+        // columns; none of them is read again (the staged L is read at and
+        // below the diagonal only)
+        if constexpr (c > k) a[c] = fmaf(-lrk, u[c - 4 * q] * inv, a[c]);
+      });
+    });
     // forward substitution L y = b (division by the diagonal, mat.h:4596)
-    const float bk = __shfl(bv, gbase + k);
-    const float yk = bk / lkk;
+    const float yk = hb[64 + gid] / lkk;
     if (kvalid && active) {
       if (r == k) bv = yk;
       else if (r > k) bv = fmaf(-lrk, yk, bv);
     }
+    // one column step at a time: interleaving steps only raises the
+    // register count past the file
+    __builtin_amdgcn_sched_barrier(0);
   });
   // stage L rows in LDS for L^T x = y
-  float *Ls = lds + gid * NB * NB;
-  bfor<0, NB>([&](auto cc) {
-    constexpr int c = decltype(cc)::value;
-    Ls[r * NB + c] = a[c];
+  float *Ls = buf + 2 * HB + gid * NB * NB;
+  bfor<0, NB / 4>([&](auto qc) __attribute__((always_inline)) {
+    constexpr int q = decltype(qc)::value;
+    *reinterpret_cast<float4 *>(Ls + r * NB + 4 * q) =
+        make_float4(a[4 * q], a[4 * q + 1], a[4 * q + 2], a[4 * q + 3]);
   });
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  wave_lds_sync();
+  // the row registers are free from here on: the diagonal comes from LDS
+  const float dg = Ls[r * NB + r];
   float s = bv;
-  bfor_down<0, NB>([&](auto kc) {
+  bfor_down<0, NB>([&](auto kc) __attribute__((always_inline)) {
     constexpr int k = decltype(kc)::value;
     const bool kvalid = k < n;
-    const float mine = (r == k) ? s / a[k] : 0.f;
-    const float xk = __shfl(mine, gbase + k);
+    const float mine = (r == k) ? s / dg : 0.f;
+    const float xk = __shfl(mine, gid * NB + k);
     if (kvalid && active && r < k) s = fmaf(-Ls[k * NB + r], xk, s);
     if (kvalid && r == k && !fail) p.x[p.voff[mi] + k] = xk;
   });
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
+  wave_lds_sync();
   if (has && r == 0) p.info[mi] = fail ? -1 : 0;
 }
 
-constexpr int BATCH_WAVES = 4;
+// One kernel per (kind, size bin), so each gets the registers and occupancy
+// its matrices need.  Persistent: the wavefronts pull tasks [t0, t1) of
+// their class from the class's counter.
+constexpr int NCLASS = 8;  // (bin 0..3) x (kind 0..1)
+constexpr int class_of(int bin, int kind) { return bin * 2 + kind; }
 
-__global__ __launch_bounds__(64 * BATCH_WAVES) void batched_kernel(BatchP p) {
-  __shared__ float lds[BATCH_WAVES][64 * 64];
+template <int NB, int KIND>
+struct ClassShape {
+  static constexpr int WAVES = (KIND == 1 && NB == 64) ? 2 : 4;
+  // floats per wavefront: hand-off buffers, plus the staged L for Cholesky
+  static constexpr int LDSW = 2 * HB + (KIND == 1 ? 64 * NB : 0);
+};
+
+template <int NB, int KIND>
+__global__ __launch_bounds__((64 * ClassShape<NB, KIND>::WAVES)) void batched_kernel(BatchP p, int t0,
+                                                                                  int t1, int cls) {
+  using S = ClassShape<NB, KIND>;
+  __shared__ __align__(16) float lds[S::WAVES][S::LDSW];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   while (true) {
     int task = 0;
-    if (lane == 0) task = (int)atomicAdd(p.counter, 1u);
+    if (lane == 0) task = t0 + (int)atomicAdd(p.counter + cls, 1u);
     task = __shfl(task, 0);
-    if (task >= p.ntasks) break;
+    if (task >= t1) break;
     const BatchTask t = p.tasks[task];
-    if (t.kind == 0) {
-      switch (t.bin) {
-        case 0: group_lu<8>(p, t, lane); break;
-        case 1: group_lu<16>(p, t, lane); break;
-        case 2: group_lu<32>(p, t, lane); break;
-        default: group_lu<64>(p, t, lane); break;
-      }
-    } else {
-      switch (t.bin) {
-        case 0: group_chol<8>(p, t, lane, lds[wave]); break;
-        case 1: group_chol<16>(p, t, lane, lds[wave]); break;
-        case 2: group_chol<32>(p, t, lane, lds[wave]); break;
-        default: group_chol<64>(p, t, lane, lds[wave]); break;
-      }
-    }
+    if constexpr (KIND == 0) group_lu<NB>(p, t, lane, lds[wave]);
+    else group_chol<NB>(p, t, lane, lds[wave]);
   }
 }
 
@@ -245,7 +342,9 @@ struct Plan {
   BatchTask *d_tasks = nullptr;
   unsigned *d_counter = nullptr;
   int *d_n = nullptr;
-  int grid = 0;
+  int wg_per_cu = 4;
+  int cls_t0[mg::NCLASS] = {}, cls_t1[mg::NCLASS] = {};
+  std::vector<int> cls_order;  // classes with tasks, in launch order
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   double tuned_ms = 0;
@@ -258,7 +357,7 @@ int bin_of(int n) { return n <= 8 ? 0 : (n <= 16 ? 1 : (n <= 32 ? 2 : 3)); }
 // Build tasks: bins (NB, kind) -> chunks of 64/NB matrices.  order_mode 0:
 // largest bins first; 1: smallest first.
 void make_tasks(size_t count, const int32_t *n, const int8_t *kind, int order_mode,
-                std::vector<int> &order, std::vector<BatchTask> &tasks) {
+                std::vector<int> &order, std::vector<BatchTask> &tasks, Plan &pl) {
   std::vector<int> idx[4][2];
   for (size_t i = 0; i < count; ++i) {
     MG_CHECK(n[i] >= 1 && n[i] <= 64, "batch: matrix order must be in [1, 64]");
@@ -266,11 +365,15 @@ void make_tasks(size_t count, const int32_t *n, const int8_t *kind, int order_mo
   }
   order.clear();
   tasks.clear();
+  pl.cls_order.clear();
   for (int bi = 0; bi < 4; ++bi) {
     const int b = order_mode == 0 ? 3 - bi : bi;
     const int per = 64 / (8 << b);
     for (int k = 0; k < 2; ++k) {
       auto &v = idx[b][k];
+      const int c = mg::class_of(b, k);
+      pl.cls_t0[c] = (int)tasks.size();
+      if (!v.empty()) pl.cls_order.push_back(c);
       for (size_t s = 0; s < v.size(); s += per) {
         BatchTask t;
         t.bin = b;
@@ -280,6 +383,7 @@ void make_tasks(size_t count, const int32_t *n, const int8_t *kind, int order_mo
         for (int q = 0; q < t.count; ++q) order.push_back(v[s + q]);
         tasks.push_back(t);
       }
+      pl.cls_t1[c] = (int)tasks.size();
     }
   }
 }
@@ -295,13 +399,35 @@ void upload_plan(hipStream_t s, Plan &pl, const std::vector<int> &order,
                         hipMemcpyHostToDevice, s));
   pl.p.order = pl.d_order;
   pl.p.tasks = pl.d_tasks;
-  pl.p.ntasks = (int)tasks.size();
+}
+
+template <int NB, int KIND>
+void launch_class(hipStream_t s, const Plan &pl, int c) {
+  constexpr int W = mg::ClassShape<NB, KIND>::WAVES;
+  const int t0 = pl.cls_t0[c], t1 = pl.cls_t1[c];
+  // resident wavefronts as for wg_per_cu workgroups of four per CU, and no
+  // more workgroups than the class has tasks for
+  const int grid = std::max(1, std::min((int)mg::ceil_div(t1 - t0, W),
+                                        mg::num_cus() * pl.wg_per_cu * 4 / W));
+  hipLaunchKernelGGL((mg::batched_kernel<NB, KIND>), dim3(grid), dim3(64 * W), 0, s, pl.p, t0, t1,
+                     c);
+  MG_HIP(hipGetLastError());
 }
 
 void launch(hipStream_t s, Plan &pl) {
-  mg::fill_u32(s, pl.d_counter, 1, 0u);
-  hipLaunchKernelGGL(mg::batched_kernel, dim3(pl.grid), dim3(64 * mg::BATCH_WAVES), 0, s, pl.p);
-  MG_HIP(hipGetLastError());
+  mg::fill_u32(s, pl.d_counter, mg::NCLASS, 0u);
+  for (int c : pl.cls_order) {
+    switch (c) {
+      case mg::class_of(0, 0): launch_class<8, 0>(s, pl, c); break;
+      case mg::class_of(0, 1): launch_class<8, 1>(s, pl, c); break;
+      case mg::class_of(1, 0): launch_class<16, 0>(s, pl, c); break;
+      case mg::class_of(1, 1): launch_class<16, 1>(s, pl, c); break;
+      case mg::class_of(2, 0): launch_class<32, 0>(s, pl, c); break;
+      case mg::class_of(2, 1): launch_class<32, 1>(s, pl, c); break;
+      case mg::class_of(3, 0): launch_class<64, 0>(s, pl, c); break;
+      default: launch_class<64, 1>(s, pl, c); break;
+    }
+  }
 }
 
 void capture(hipStream_t s, Plan &pl) {
@@ -360,7 +486,7 @@ int mat_gpu_batch_create_f32(void *stream, size_t count, const int32_t *n, const
     std::vector<int> hn(n, n + count);
     MG_HIP(hipMalloc(&pl->d_n, std::max<size_t>(1, count) * sizeof(int)));
     MG_HIP(hipMemcpyAsync(pl->d_n, hn.data(), count * sizeof(int), hipMemcpyHostToDevice, s));
-    MG_HIP(hipMalloc(&pl->d_counter, sizeof(unsigned)));
+    MG_HIP(hipMalloc(&pl->d_counter, mg::NCLASS * sizeof(unsigned)));
     pl->p.A = a;
     pl->p.aoff = (const long long *)a_off;
     pl->p.b = b;
@@ -370,23 +496,22 @@ int mat_gpu_batch_create_f32(void *stream, size_t count, const int32_t *n, const
     pl->p.info = info;
     pl->p.counter = pl->d_counter;
     pl->p.eps = 1e-6f;
-    const int cus = mg::num_cus();
     std::vector<int> order;
     std::vector<BatchTask> tasks;
     if (!autotune) {
-      make_tasks(count, n, kind, 0, order, tasks);
+      make_tasks(count, n, kind, 0, order, tasks, *pl);
       upload_plan(s, *pl, order, tasks);
-      pl->grid = cus * 4;
+      pl->wg_per_cu = 4;
       pl->tuned_wg_per_cu = 4;
       capture(s, *pl);
     } else {
       double best = 1e30;
       int best_wg = 4, best_order = 0;
       for (int om = 0; om < 2; ++om) {
-        make_tasks(count, n, kind, om, order, tasks);
+        make_tasks(count, n, kind, om, order, tasks, *pl);
         upload_plan(s, *pl, order, tasks);
         for (int wg : {1, 2, 4, 8}) {
-          pl->grid = cus * wg;
+          pl->wg_per_cu = wg;
           capture(s, *pl);
           const double ms = time_replays(s, *pl, 3);
           if (ms < best) {
@@ -396,9 +521,9 @@ int mat_gpu_batch_create_f32(void *stream, size_t count, const int32_t *n, const
           }
         }
       }
-      make_tasks(count, n, kind, best_order, order, tasks);
+      make_tasks(count, n, kind, best_order, order, tasks, *pl);
       upload_plan(s, *pl, order, tasks);
-      pl->grid = cus * best_wg;
+      pl->wg_per_cu = best_wg;
       pl->tuned_wg_per_cu = best_wg;
       pl->tuned_order = best_order;
       pl->tuned_ms = best;
@@ -457,18 +582,18 @@ int mat_gpu_batch_solve_f32(void *stream, size_t count, const int32_t *n, const 
     mg::Workspace &ws = mg::workspace(s);
     std::vector<int> order;
     std::vector<BatchTask> tasks;
-    make_tasks(count, n, kind, 0, order, tasks);
+    Plan pl;
+    make_tasks(count, n, kind, 0, order, tasks, pl);
     int *d_n = (int *)ws.get("batch.n", std::max<size_t>(1, count) * 4);
     int *d_order = (int *)ws.get("batch.order", std::max<size_t>(1, order.size()) * 4);
     BatchTask *d_tasks =
         (BatchTask *)ws.get("batch.tasks", std::max<size_t>(1, tasks.size()) * sizeof(BatchTask));
-    unsigned *d_counter = (unsigned *)ws.get("batch.counter", 4);
+    unsigned *d_counter = (unsigned *)ws.get("batch.counter", mg::NCLASS * sizeof(unsigned));
     std::vector<int> hn(n, n + count);
     MG_HIP(hipMemcpyAsync(d_n, hn.data(), count * 4, hipMemcpyHostToDevice, s));
     MG_HIP(hipMemcpyAsync(d_order, order.data(), order.size() * 4, hipMemcpyHostToDevice, s));
     MG_HIP(hipMemcpyAsync(d_tasks, tasks.data(), tasks.size() * sizeof(BatchTask),
                           hipMemcpyHostToDevice, s));
-    Plan pl;
     pl.p.A = a;
     pl.p.aoff = (const long long *)a_off;
     pl.p.b = b;
@@ -478,11 +603,10 @@ int mat_gpu_batch_solve_f32(void *stream, size_t count, const int32_t *n, const 
     pl.p.info = info;
     pl.p.order = d_order;
     pl.p.tasks = d_tasks;
-    pl.p.ntasks = (int)tasks.size();
     pl.p.counter = d_counter;
     pl.p.eps = 1e-6f;
     pl.d_counter = d_counter;
-    pl.grid = std::max(1, std::min((int)tasks.size(), mg::num_cus() * 4));
+    pl.wg_per_cu = 4;
     launch(s, pl);
     // host vectors must outlive the async copies
     MG_HIP(hipStreamSynchronize(s));
