@@ -29,7 +29,10 @@ Other workloads (--workload):
 Every line carries "roofline" for the workload's dominant kernel family:
 achieved = algorithmic bytes (or flops) of its launches / their summed
 durations, measured with HIP events recorded by the library on the launch
-stream during the timed steps; and "cpu_baseline": the reference's x86
+stream during the timed steps, with "traffic" = HBM bytes per launch from
+the rocprofv3 FETCH_SIZE / WRITE_SIZE passes committed under profiles/
+(hbm_traffic__*.md; null for a workload run at other sizes or not
+profiled); and "cpu_baseline": the reference's x86
 scalar path (oracle/_ref, compiled from the reference sources; the
 clean-room oracle restatement when that is absent) timed on one host core
 on a bounded sample.
