@@ -23,8 +23,9 @@ Other workloads (--workload):
              refinement) against mat_gpu_posv_f64
   batched    200000 systems with n log-uniform on [3, 64], half general
              (U[-1,1] + n I), half SPD (G G^T + I), one right-hand side
-             each, fp32: one persistent size-binned launch replayed from a
-             hipGraph, against one launch per matrix
+             each, fp32: one persistent kernel per size bin and kind,
+             captured in one hipGraph and replayed, against one launch per
+             matrix
 
 Every line carries "roofline" for the workload's dominant kernel family:
 achieved = algorithmic bytes (or flops) of its launches / their summed
@@ -161,13 +162,13 @@ def _cpu_loop(fn, flops, budget_s, max_runs=5):
     return flops / t / 1e9, t, len(times)
 
 
-def cpu_solve_multi(n, nrhs=16, budget_s=20.0):
+def cpu_solve_multi(n, nrhs=16, budget_s=12.0):
     o, kind = _oracle("f64")
     rng = np.random.default_rng(0)
     A = rng.uniform(-1, 1, (n, n))
     B = rng.uniform(-1, 1, (n, nrhs))
     v, t, runs = _cpu_loop(lambda: o.solve_multi(A, B), 2.0 / 3.0 * n ** 3 + 2.0 * n ** 2 * nrhs,
-                           budget_s)
+                           budget_s, max_runs=200)
     return {"value": round(v, 4), "unit": "GFLOP/s", "cores": 1, "kind": kind,
             "sample": f"mat_plu + {nrhs}x(mat_solve_tril_unit, mat_solve_triu), fp64, n={n}, "
                       f"median of {runs} runs ({t:.2f} s each), single thread"}
@@ -183,7 +184,7 @@ def cpu_gemm(n, budget_s=20.0):
             "sample": f"mat_mul fp32, n={n}, median of {runs} runs ({t:.2f} s each), single thread"}
 
 
-def cpu_chol(n, nrhs, prec, budget_s=20.0):
+def cpu_chol(n, nrhs, prec, budget_s=12.0):
     o, kind = _oracle(prec)
     rng = np.random.default_rng(0)
     G = rng.standard_normal((n, n))
@@ -195,7 +196,7 @@ def cpu_chol(n, nrhs, prec, budget_s=20.0):
         for b in bs:
             o.solve_trilt(L, o.solve_tril(L, b))
 
-    v, t, runs = _cpu_loop(run, n ** 3 / 3.0 + 2.0 * n ** 2 * nrhs, budget_s)
+    v, t, runs = _cpu_loop(run, n ** 3 / 3.0 + 2.0 * n ** 2 * nrhs, budget_s, max_runs=200)
     return {"value": round(v, 4), "unit": "GFLOP/s", "cores": 1, "kind": kind,
             "sample": f"mat_chol + {nrhs}x(mat_solve_tril, mat_solve_trilt), {prec}, n={n}, "
                       f"median of {runs} runs ({t:.2f} s each), single thread"}
@@ -655,7 +656,7 @@ def run_batched(torch, libmat_amd, args):
                "ms_for_all": round(per_call_s * count * 1e3, 2),
                "sample": f"{sample2} mat_gpu_gesv_f32 / mat_gpu_posv_f32 calls",
                "speedup_of_batched": round(per_call_s * count * 1e3 / batch_ms, 1)},
-           "roofline": {"kernel": "batched persistent solve (group_lu / group_chol)",
+           "roofline": {"kernel": "batched_kernel (one persistent kernel per size/kind class)",
                         "bound": "hbm", "achieved": round(achieved, 3), "peak": PEAK["hbm"],
                         "unit": "GB/s", "frac": round(achieved / PEAK["hbm"], 5),
                         "traffic": hbm_traffic("batched", args.profiled),
@@ -699,7 +700,7 @@ def main():
                     help="mixed LU: 0 partial, 1 butterfly, 2 tournament (library default if unset)")
     ap.add_argument("--count", type=int, default=200000)
     ap.add_argument("--loop-sample", type=int, default=2000)
-    ap.add_argument("--cpu-count", type=int, default=20000)
+    ap.add_argument("--cpu-count", type=int, default=200000)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--data", default="conditioned", choices=["conditioned", "uniform"],
