@@ -86,6 +86,9 @@ hipStream_t default_stream();
 hipStream_t side_stream();
 // Reusable events (timing disabled) of the current device, by index.
 hipEvent_t pooled_event(int i);
+// Held by a look-ahead factorization while it enqueues: the side stream and
+// the event pool are shared by every factorization on a device.
+std::mutex &lookahead_mutex();
 // Look-ahead block width of the LU (MATGPU_LU_NB, default 256; 0 = none).
 int lookahead_block();
 

@@ -114,8 +114,7 @@ template <typename T>
 static void getrf_lookahead(hipStream_t s, int n, T *W, long ld, int *ipiv, int *perm, int *pairs,
                             T eps, const LuOpts &o, int NB) {
   // the side stream and the event pool are per device: one enqueue at a time
-  static std::mutex mu;
-  std::lock_guard<std::mutex> lk(mu);
+  std::lock_guard<std::mutex> lk(lookahead_mutex());
   hipStream_t u = side_stream();
   const int nblk = (n + NB - 1) / NB;
   std::vector<hipEvent_t> evP(nblk), evU(nblk);

@@ -192,6 +192,11 @@ hipEvent_t pooled_event(int i) {
   return v[i];
 }
 
+std::mutex &lookahead_mutex() {
+  static std::mutex m;
+  return m;
+}
+
 int lookahead_block() {
   static int nb = [] {
     const char *e = getenv("MATGPU_LU_NB");

@@ -193,13 +193,16 @@ def test_gesv_posv(gpu, prec, n):
 
 
 @pytest.mark.parametrize("prec", PRECS)
-def test_potrf_not_spd(gpu, prec):
+@pytest.mark.parametrize("n", [300, 1300])
+def test_potrf_not_spd(gpu, prec, n):
+    """n = 1300: the blocked look-ahead driver (blocks of 256, the failing
+    column in the fifth block, after the side stream's updates began)."""
     g = gpu[prec]
-    n = 300
     rng = np.random.default_rng(1)
     G = rng.standard_normal((n, n))
     S = G @ G.T / n + np.eye(n)
-    S[200, 200] = -5.0  # fails at (or before) column 200
+    bad = 2 * n // 3
+    S[bad, bad] = -5.0  # fails at (or before) that column
     dA = to_dev(S, DT[prec])
     assert getattr(g.lib, f"mat_gpu_potrf_{prec}")(stream(), n, ptr(dA), n) == -1
     dA = to_dev(G @ G.T / n + np.eye(n), DT[prec])
