@@ -347,7 +347,6 @@ struct Plan {
   unsigned *d_counter = nullptr;
   int *d_n = nullptr;
   int wg_per_cu = 4;
-  int branches = 1;  // graph branches the class kernels are spread over
   int cls_t0[mg::NCLASS] = {}, cls_t1[mg::NCLASS] = {};
   std::vector<int> cls_order;  // classes with tasks, in launch order
   hipGraph_t graph = nullptr;
@@ -432,36 +431,10 @@ void launch_one(hipStream_t s, const Plan &pl, int c) {
   }
 }
 
-// The class kernels, on `s` or spread over `s` and the nside streams
-// (parallel branches when captured into a graph; they join back on `s`).
-void launch(hipStream_t s, Plan &pl, hipStream_t *side = nullptr, int nside = 0,
-            std::vector<hipEvent_t> *events = nullptr) {
+// The class kernels in launch order, after the counters' reset.
+void launch(hipStream_t s, Plan &pl) {
   mg::fill_u32(s, pl.d_counter, mg::NCLASS, 0u);
-  hipEvent_t fork = nullptr;
-  if (nside > 0) {
-    MG_HIP(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
-    events->push_back(fork);
-    MG_HIP(hipEventRecord(fork, s));
-    for (int i = 0; i < nside; ++i) MG_HIP(hipStreamWaitEvent(side[i], fork, 0));
-  }
-  int slot = 0;
-  for (int c : pl.cls_order) {
-    hipStream_t q = s;
-    if (nside > 0) {
-      const int j = slot++ % (nside + 1);
-      q = j == 0 ? s : side[j - 1];
-    }
-    launch_one(q, pl, c);
-  }
-  if (nside > 0) {
-    for (int i = 0; i < nside; ++i) {
-      hipEvent_t join;
-      MG_HIP(hipEventCreateWithFlags(&join, hipEventDisableTiming));
-      events->push_back(join);
-      MG_HIP(hipEventRecord(join, side[i]));
-      MG_HIP(hipStreamWaitEvent(s, join, 0));
-    }
-  }
+  for (int c : pl.cls_order) launch_one(s, pl, c);
 }
 
 void capture(hipStream_t s, Plan &pl) {
@@ -469,17 +442,11 @@ void capture(hipStream_t s, Plan &pl) {
   if (pl.graph) MG_HIP(hipGraphDestroy(pl.graph));
   hipStream_t cs;
   MG_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
-  const int nside = pl.branches - 1;
-  std::vector<hipStream_t> side(std::max(0, nside));
-  for (auto &q : side) MG_HIP(hipStreamCreateWithFlags(&q, hipStreamNonBlocking));
-  std::vector<hipEvent_t> events;
   MG_HIP(hipStreamSynchronize(s));
   MG_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-  launch(cs, pl, side.data(), nside, &events);
+  launch(cs, pl);
   MG_HIP(hipStreamEndCapture(cs, &pl.graph));
   MG_HIP(hipGraphInstantiate(&pl.exec, pl.graph, nullptr, nullptr, 0));
-  for (auto e : events) MG_HIP(hipEventDestroy(e));
-  for (auto q : side) MG_HIP(hipStreamDestroy(q));
   MG_HIP(hipStreamDestroy(cs));
 }
 
@@ -546,29 +513,24 @@ int mat_gpu_batch_create_f32(void *stream, size_t count, const int32_t *n, const
       capture(s, *pl);
     } else {
       double best = 1e30;
-      int best_wg = 4, best_order = 0, best_br = 1;
+      int best_wg = 4, best_order = 0;
       for (int om = 0; om < 2; ++om) {
         make_tasks(count, n, kind, om, order, tasks, *pl);
         upload_plan(s, *pl, order, tasks);
-        for (int br : {1, 4}) {
-          for (int wg : {1, 2, 4, 8}) {
-            pl->wg_per_cu = wg;
-            pl->branches = br;
-            capture(s, *pl);
-            const double ms = time_replays(s, *pl, 3);
-            if (ms < best) {
-              best = ms;
-              best_wg = wg;
-              best_order = om;
-              best_br = br;
-            }
+        for (int wg : {1, 2, 4, 8}) {
+          pl->wg_per_cu = wg;
+          capture(s, *pl);
+          const double ms = time_replays(s, *pl, 3);
+          if (ms < best) {
+            best = ms;
+            best_wg = wg;
+            best_order = om;
           }
         }
       }
       make_tasks(count, n, kind, best_order, order, tasks, *pl);
       upload_plan(s, *pl, order, tasks);
       pl->wg_per_cu = best_wg;
-      pl->branches = best_br;
       pl->tuned_wg_per_cu = best_wg;
       pl->tuned_order = best_order;
       pl->tuned_ms = best;

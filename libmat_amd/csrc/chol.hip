@@ -93,8 +93,21 @@ __global__ __launch_bounds__(64) void chol_leaf_kernel(int n, T *A, long lda, in
       if (i == 0) *info = j0 + J + 1;
       return;
     }
-    const T l = sqrt(d);
-    const T inv = T(1) / l;
+    // fp32: one v_rsq_f32 (1 ulp) instead of an IEEE square root and
+    // divide on the step's critical path; tiny pivots keep the IEEE path
+    T l, inv;
+    if constexpr (sizeof(T) == 4) {
+      if (d >= 1e-30f) {
+        inv = __builtin_amdgcn_rsqf(d);
+        l = d * inv;
+      } else {
+        l = sqrtf(d);
+        inv = 1.0f / l;
+      }
+    } else {
+      l = sqrt(d);
+      inv = T(1) / l;
+    }
     const T v = i == J ? l : (i > J ? a[0] * inv : a[0]);
     if (i >= J) S[J * (NB + 1) + i] = v;
     colb[(i - J) & (NB - 1)] = v;
