@@ -63,6 +63,13 @@ void mat_mul(Mat *out, const Mat *a, const Mat *b);
 Mat *mat_rmul(const Mat *a, const Mat *b);
 /* C = alpha * A * B + beta * C.    replaces mat.h:612  (impl 2787-2796) */
 void mat_gemm(Mat *C, mat_elem_t alpha, const Mat *A, const Mat *B, mat_elem_t beta);
+/* C = alpha * A * A^T + beta * C on the uplo triangle ('L'/'l' lower, else
+   upper; the other triangle is left as it is).
+                                    replaces mat.h:617  (impl 5670-5677) */
+void mat_syrk(Mat *C, const Mat *A, mat_elem_t alpha, mat_elem_t beta, char uplo);
+/* C = alpha * A^T * A + beta * C on the uplo triangle.
+                                    replaces mat.h:622  (impl 5727-5734) */
+void mat_syrk_t(Mat *C, const Mat *A, mat_elem_t alpha, mat_elem_t beta, char uplo);
 /* PAQ = LU, full pivoting; returns row swaps + column swaps.
                                     replaces mat.h:771  (impl 4476-4578) */
 int mat_lu(const Mat *A, Mat *L, Mat *U, Perm *p, Perm *q);

@@ -23,7 +23,8 @@ ROOT = os.path.dirname(HERE)
 
 # Drop-in symbols (include/mat_gpu.h part 1) and extension symbols (part 2).
 DROPIN_SYMBOLS = [
-    "mat_mul", "mat_rmul", "mat_gemm", "mat_lu", "mat_plu", "mat_chol", "mat_solve",
+    "mat_mul", "mat_rmul", "mat_gemm", "mat_syrk", "mat_syrk_t", "mat_lu", "mat_plu",
+    "mat_chol", "mat_solve",
     "mat_solve_spd", "mat_solve_tril", "mat_solve_tril_unit", "mat_solve_triu",
     "mat_solve_trilt", "mat_det", "mat_inv",
     "mat_empty", "mat_mat", "mat_from", "mat_init", "mat_free_mat", "mat_zeros", "mat_eye",
@@ -95,6 +96,8 @@ class GpuLib:
         s("mat_mul", None, [M, M, M])
         s("mat_rmul", M, [M, M])
         s("mat_gemm", None, [M, self.ct, M, M, self.ct])
+        s("mat_syrk", None, [M, M, self.ct, self.ct, C.c_char])
+        s("mat_syrk_t", None, [M, M, self.ct, self.ct, C.c_char])
         s("mat_lu", i32, [M, M, M, P, P])
         s("mat_plu", i32, [M, M, M, P])
         s("mat_chol", i32, [M, M])
@@ -164,6 +167,14 @@ class GpuLib:
         b, bb = self.to_mat(B)
         c, cb = self.to_mat(Cm)
         self.lib.mat_gemm(C.byref(c), alpha, C.byref(a), C.byref(b), beta)
+        return cb
+
+    def mat_syrk(self, alpha, A, beta, Cm, uplo="L", trans=False):
+        """mat_syrk (trans: mat_syrk_t) on copies of A and Cm; returns C."""
+        a, ab = self.to_mat(A)
+        c, cb = self.to_mat(Cm)
+        fn = self.lib.mat_syrk_t if trans else self.lib.mat_syrk
+        fn(C.byref(c), C.byref(a), alpha, beta, uplo.encode())
         return cb
 
     def mat_mul(self, A, B):

@@ -143,6 +143,43 @@ Mat *mat_rmul(const Mat *a, const Mat *b) {
   return r;
 }
 
+// C = alpha*op(A)*op(A)^T + beta*C on one triangle of the square C
+// (mat_syrk: op(A) = A, mat.h:5670-5677; mat_syrk_t: op(A) = A^T,
+// mat.h:5727-5734; semantics of their scalar kernels, mat.h:5641-5668 and
+// 5698-5725): uplo 'L' or 'l' updates row >= col, any other value row <=
+// col; the other triangle is left as it is.
+static void syrk_dropin(const char *fn, Mat *C, const Mat *A, T alpha, T beta, char uplo,
+                        bool trans) {
+  run(fn, [&] {
+    hipStream_t s = mg::default_stream();
+    const long n = (long)C->rows, k = (long)(trans ? A->rows : A->cols);
+    T *dA = upload(s, "dropin.A", A);
+    T *dC = upload(s, "dropin.C", C);
+    // op(A) is n x k: A itself (column-major, ld n) or A^T read from A (k x n, ld k)
+    const View<const T> a = trans ? View<const T>{dA, k, 1} : View<const T>{dA, 1, n};
+    const int tri = (uplo == 'L' || uplo == 'l') ? 1 : 2;
+    mg::gemm<T>(s, n, n, k, alpha, a, a.t(), beta, View<T>{dC, 1, n}, tri);
+    download(s, C, dC);
+    finish(s);
+  });
+}
+
+void mat_syrk(Mat *C, const Mat *A, T alpha, T beta, char uplo) {
+  REQUIRE_MAT(C);
+  REQUIRE_MAT(A);
+  REQUIRE(C->rows == C->cols);
+  REQUIRE(C->rows == A->rows);
+  syrk_dropin(__func__, C, A, alpha, beta, uplo, false);
+}
+
+void mat_syrk_t(Mat *C, const Mat *A, T alpha, T beta, char uplo) {
+  REQUIRE_MAT(C);
+  REQUIRE_MAT(A);
+  REQUIRE(C->rows == C->cols);
+  REQUIRE(C->rows == A->cols);
+  syrk_dropin(__func__, C, A, alpha, beta, uplo, true);
+}
+
 // -------------------------------------------------------- factorizations --
 int mat_plu(const Mat *A, Mat *L, Mat *U, Perm *p) {
   REQUIRE_MAT(A);

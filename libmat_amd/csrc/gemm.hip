@@ -164,7 +164,9 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmP<T> p) {
   int tm, tn;
   if (!map_tile(blockIdx.x, gridDim.x, p.tiles, p.tiles_m, p.tiles_n, tm, tn)) return;
   const int bm0 = tm * BM, bn0 = tn * BN;
-  if (p.lower && bm0 + BM - 1 < bn0) return;  // tile strictly above the diagonal
+  // triangular C: skip tiles wholly outside the requested triangle
+  if (p.lower == 1 && bm0 + BM - 1 < bn0) return;  // strictly above the diagonal
+  if (p.lower == 2 && bm0 > bn0 + BN - 1) return;  // strictly below the diagonal
 
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
@@ -229,7 +231,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmP<T> p) {
 #pragma unroll
       for (int r = 0; r < F::NACC; ++r) {
         const int i = bm0 + wm * WTM + a * FM + F::row(r, lane);
-        if (i >= p.M || (p.lower && i < j)) continue;
+        if (i >= p.M || (p.lower == 1 && i < j) || (p.lower == 2 && i > j)) continue;
         T *c = p.C + (long)i + (long)j * p.ldc;
         T v = p.alpha * acc[a][b][r];
         if (p.beta != T(0)) v += p.beta * *c;
@@ -283,11 +285,11 @@ static void launch_mode(hipStream_t s, GemmP<T> &p, bool vec, int in_place) {
 
 template <typename T>
 void gemm(hipStream_t s, long M, long N, long K, T alpha, View<const T> A, View<const T> B,
-          T beta, View<T> C, bool lower_only) {
+          T beta, View<T> C, int tri) {
   if (M <= 0 || N <= 0) return;
   if (C.rs != 1) {
     // Row-major C: compute C^T = B^T A^T into a column-major view.
-    MG_CHECK(C.cs == 1 && !lower_only, "gemm: unsupported C layout");
+    MG_CHECK(C.cs == 1 && tri == 0, "gemm: unsupported C layout");
     gemm<T>(s, N, M, K, alpha, B.t(), A.t(), beta, View<T>{C.p, 1, C.rs}, false);
     return;
   }
@@ -300,7 +302,8 @@ void gemm(hipStream_t s, long M, long N, long K, T alpha, View<const T> A, View<
   p.K = (int)K;
   p.alpha = alpha;
   p.beta = beta;
-  p.lower = lower_only ? 1 : 0;
+  MG_CHECK(tri >= 0 && tri <= 2, "gemm: tri must be 0, 1 or 2");
+  p.lower = tri;
   // A: element (i,k).  MN-contiguous when rs == 1.
   const bool a_mn = (A.rs == 1);
   p.A = A.p;
@@ -327,8 +330,8 @@ void gemm(hipStream_t s, long M, long N, long K, T alpha, View<const T> A, View<
 }
 
 template void gemm<float>(hipStream_t, long, long, long, float, View<const float>,
-                          View<const float>, float, View<float>, bool);
+                          View<const float>, float, View<float>, int);
 template void gemm<double>(hipStream_t, long, long, long, double, View<const double>,
-                           View<const double>, double, View<double>, bool);
+                           View<const double>, double, View<double>, int);
 
 }  // namespace mg

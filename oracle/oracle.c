@@ -87,6 +87,29 @@ void ORC(gemm)(size_t M, size_t K, size_t N, elem alpha, const elem *A,
   }
 }
 
+/* Symmetric rank-k update on one triangle of the n x n matrix C:
+ * C = alpha*A*A^T + beta*C with A n x k (mat_syrk -> mat__syrk_scalar,
+ * mat.h:5641-5668, 5670-5677), or C = alpha*A^T*A + beta*C with A k x n
+ * when trans (mat_syrk_t -> mat__syrk_t_scalar, mat.h:5698-5725,
+ * 5727-5734).  Row i outer, then j over the triangle (lower: j <= i,
+ * upper: j >= i); the k-sum runs in order, then C(i,j) = beta*C(i,j) +
+ * alpha*sum.  The other triangle is not touched. */
+void ORC(syrk)(size_t n, size_t k, elem alpha, const elem *A, elem beta,
+               elem *C, int upper, int trans) {
+  for (size_t i = 0; i < n; i++) {
+    const size_t j0 = upper ? i : 0, j1 = upper ? n : i + 1;
+    for (size_t j = j0; j < j1; j++) {
+      elem sum = 0;
+      for (size_t kk = 0; kk < k; kk++) {
+        const elem a = trans ? AT(A, k, kk, i) : AT(A, n, i, kk);
+        const elem b = trans ? AT(A, k, kk, j) : AT(A, n, j, kk);
+        sum += a * b;
+      }
+      AT(C, n, i, j) = beta * AT(C, n, i, j) + alpha * sum;
+    }
+  }
+}
+
 /* Strided GEMM without transposes, scalar branch of mat__gemm_strided_
  * (mat.h:2719-2743): per-column beta-scale, then j-k-i with alpha folded
  * into B(k,j). */

@@ -60,6 +60,7 @@ class Oracle:
         f = self._fn
         sz, P, I = C.c_size_t, C.c_void_p, C.c_int
         f("gemm", None, [sz, sz, sz, self.ct, P, P, self.ct, P])
+        f("syrk", None, [sz, sz, self.ct, P, self.ct, P, I, I])
         f("plu", I, [sz, P, P, P, P])
         f("lu", I, [sz, P, P, P, P, P])
         f("chol", I, [sz, P, P])
@@ -88,6 +89,18 @@ class Oracle:
         M, K = A.shape
         N = B.shape[1]
         self._gemm(M, K, N, alpha, self._p(A), self._p(B), beta, self._p(out))
+        return out
+
+    def syrk(self, alpha, A, beta, Cm, uplo="L", trans=False):
+        """C = alpha A A^T + beta C (trans: alpha A^T A + beta C) on the
+        uplo triangle; the other triangle of Cm is returned unchanged."""
+        A = self._cm(A)
+        out = self._cm(Cm).copy(order="F")
+        n = A.shape[1] if trans else A.shape[0]
+        k = A.shape[0] if trans else A.shape[1]
+        assert out.shape == (n, n)
+        self._syrk(n, k, alpha, self._p(A), beta, self._p(out), int(uplo in ("U", "u")),
+                   int(bool(trans)))
         return out
 
     def mul(self, A, B):

@@ -37,6 +37,13 @@ void REF(gemm)(size_t M, size_t K, size_t N, elem alpha, const elem *A,
   mat_gemm(&c, alpha, &a, &b, beta);
 }
 
+void REF(syrk)(size_t n, size_t k, elem alpha, const elem *A, elem beta,
+               elem *C, int upper, int trans) {
+  Mat a = trans ? wrap(k, n, A) : wrap(n, k, A), c = wrap(n, n, C);
+  if (trans) mat_syrk_t(&c, &a, alpha, beta, upper ? 'U' : 'L');
+  else mat_syrk(&c, &a, alpha, beta, upper ? 'U' : 'L');
+}
+
 void REF(mul)(size_t M, size_t K, size_t N, const elem *A, const elem *B,
               elem *C) {
   Mat a = wrap(M, K, A), b = wrap(K, N, B), c = wrap(M, N, C);

@@ -33,6 +33,30 @@ def test_gemm_fixtures(gpu, prec):
         assert rel_fro(out, gold[t + "_C"]) <= 2 * K * EPS[prec], t
 
 
+@pytest.mark.parametrize("prec", PRECS)
+@pytest.mark.parametrize("uplo", ["L", "U"])
+@pytest.mark.parametrize("trans", [False, True])
+def test_syrk_against_oracle(gpu, prec, uplo, trans):
+    """mat_syrk / mat_syrk_t (mat.h:5670-5677, 5727-5734) against the oracle
+    on the updated triangle (|error| <= 2 K eps of the row norms), the other
+    triangle bit for bit untouched; shapes cover both the 64x64 and the
+    128x128 GEMM tiles and tiles that straddle the diagonal."""
+    from oracle.oracle import Oracle
+    g, o = gpu[prec], Oracle(prec)
+    rng = np.random.default_rng(3)
+    for n, k in ((1, 1), (3, 5), (64, 64), (130, 33), (257, 300), (2100, 64)):
+        A = rng.uniform(-1, 1, (k, n) if trans else (n, k)).astype(DT[prec])
+        Cm = rng.uniform(-1, 1, (n, n)).astype(DT[prec])
+        for alpha, beta in ((1.0, 0.0), (-0.75, 1.5)):
+            got = g.mat_syrk(alpha, A, beta, Cm, uplo, trans)
+            want = o.syrk(alpha, A, beta, Cm, uplo, trans)
+            tri = (np.tril(np.ones((n, n), bool)) if uplo == "L"
+                   else np.triu(np.ones((n, n), bool)))
+            scale = abs(alpha) * k + abs(beta)
+            assert np.abs(got[tri] - want[tri]).max() <= 2 * k * EPS[prec] * scale, (n, k)
+            assert np.array_equal(got[~tri], Cm[~tri]), (n, k)
+
+
 def test_cfg1_against_reference(gpu):
     g, gold = gpu["f32"], golden("f32")
     C = g.mat_mul(gold["cfg1_A"], gold["cfg1_B"])

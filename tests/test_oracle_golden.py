@@ -193,3 +193,31 @@ def test_lu_full_pivot_invariants(orc):
     assert rel_fro(L @ U, PAQ) <= 200 * EPS[orc.prec]
     # full pivoting: |U[k,k]| is the max of the trailing block at step k
     assert np.all(np.abs(np.diag(U))[:-1] >= np.abs(np.triu(U, 1)).max(axis=1)[:-1] * (1 - 1e-6))
+
+
+@pytest.mark.parametrize("uplo", ["L", "U"])
+@pytest.mark.parametrize("trans", [False, True])
+def test_syrk_brute_force_and_known_answer(orc, uplo, trans):
+    """mat_syrk / mat_syrk_t (mat.h:5641-5734): the updated triangle against
+    an fp64 brute force, the other triangle untouched bit for bit, and a 2x2
+    known answer: A = [1 2; 3 4] gives A A^T = [5 11; 11 25] and
+    A^T A = [10 14; 14 20]."""
+    rng = np.random.default_rng(5)
+    n, k = 37, 19
+    A = rng.uniform(-1, 1, (k, n) if trans else (n, k)).astype(orc.dtype)
+    Cm = rng.uniform(-1, 1, (n, n)).astype(orc.dtype)
+    out = orc.syrk(-0.5, A, 2.0, Cm, uplo, trans)
+    A64 = A.astype(np.float64)
+    full = -0.5 * (A64.T @ A64 if trans else A64 @ A64.T) + 2.0 * Cm.astype(np.float64)
+    tri = np.tril(np.ones((n, n), bool)) if uplo == "L" else np.triu(np.ones((n, n), bool))
+    eps = np.finfo(orc.dtype).eps
+    assert np.abs(out[tri] - full[tri]).max() <= 4 * k * eps * (np.abs(A64).max() ** 2 * k + 2)
+    assert np.array_equal(out[~tri], Cm[~tri])
+    a = np.array([[1.0, 2.0], [3.0, 4.0]], dtype=orc.dtype)
+    want = np.array([[10.0, 14.0], [14.0, 20.0]]) if trans else np.array([[5.0, 11.0],
+                                                                            [11.0, 25.0]])
+    got = orc.syrk(1.0, a, 0.0, np.full((2, 2), 7.0, dtype=orc.dtype), uplo, trans)
+    keep = np.full((2, 2), 7.0)
+    expect = np.where(np.tril(np.ones((2, 2), bool)) if uplo == "L"
+                      else np.triu(np.ones((2, 2), bool)), want, keep)
+    assert np.array_equal(got, expect)

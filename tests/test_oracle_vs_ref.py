@@ -39,3 +39,18 @@ def test_bit_exact(prec):
         Bm = rng.uniform(-1, 1, (n, 4)).astype(dt)
         assert np.array_equal(o.solve_multi(A, Bm), r.solve_multi(A, Bm))
         assert o.det(A) == r.det(A) or np.isnan(o.det(A))
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+@pytest.mark.parametrize("uplo", ["L", "U"])
+@pytest.mark.parametrize("trans", [False, True])
+def test_syrk_bit_exact(prec, uplo, trans):
+    o, r = Oracle(prec), Oracle(prec, "ref")
+    rng = np.random.default_rng(11)
+    dt = o.dtype
+    for n, k in ((1, 1), (2, 3), (5, 1), (24, 7), (49, 33), (65, 130)):
+        A = rng.uniform(-1, 1, (k, n) if trans else (n, k)).astype(dt)
+        Cm = rng.uniform(-1, 1, (n, n)).astype(dt)
+        for alpha, beta in ((1.0, 0.0), (-0.75, 1.5)):
+            assert np.array_equal(o.syrk(alpha, A, beta, Cm, uplo, trans),
+                                  r.syrk(alpha, A, beta, Cm, uplo, trans)), (n, k)
