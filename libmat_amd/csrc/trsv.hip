@@ -292,6 +292,35 @@ __device__ __forceinline__ void chain_mfma(const float *S, int ld, const float *
   acc += acc2;
 }
 
+// The same product in two parts, so that the tile's operand (A) is read
+// from LDS while the workgroup still waits for X: chain_mfma_a reads this
+// lane's 32 A values; chain_mfma_b reads all 32 B values at once, then runs
+// the MFMAs back to back.
+__device__ __forceinline__ void chain_mfma_a(const float *S, int ld, float (&a)[32]) {
+  const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const float *arow = S + (16 * w + (l & 15)) * ld;
+  const int g = l >> 4;
+#pragma unroll
+  for (int st = 0; st < 32; ++st) a[st] = arow[4 * st + g];
+}
+template <bool NEG>
+__device__ __forceinline__ void chain_mfma_b(const float (&a)[32], const float *Xs,
+                                             chain_f32x4 &acc) {
+  const int l = threadIdx.x & 63;
+  const int g = l >> 4, c = l & 15;
+  float b[32];
+#pragma unroll
+  for (int st = 0; st < 32; ++st) b[st] = Xs[(4 * st + g) * CHAIN_NC + c];
+  chain_f32x4 acc2 = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (int st = 0; st < 32; st += 2) {
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(NEG ? -a[st] : a[st], b[st], acc, 0, 0, 0);
+    acc2 = __builtin_amdgcn_mfma_f32_16x16x4f32(NEG ? -a[st + 1] : a[st + 1], b[st + 1], acc2, 0,
+                                                0, 0);
+  }
+  acc += acc2;
+}
+
 template <typename T, bool LOWER, bool FOLD>
 __global__ __launch_bounds__(ChainNT<T>::NT) void tri_chain_kernel(int n, int nrhs, View<const T> Tm,
                                                                     const T *D, const T *Tp, T *X,
@@ -386,6 +415,12 @@ __global__ __launch_bounds__(ChainNT<T>::NT) void tri_chain_kernel(int n, int nr
       } else if (!FOLD) {
         chain_load<T, TB, NT>(Dv, 0, TB, 0, TB, v);
       }
+      // MF: the tile's A operand from LDS before X_k arrives
+      float areg[MF ? 32 : 1];
+      if constexpr (MF) {
+        __syncthreads();
+        chain_mfma_a(Sb, LDS, reinterpret_cast<float(&)[32]>(areg));
+      }
       // X_k from its published words
       const uint64_t *src = pub + (long)k * BW;
       bool timeout = false;
@@ -422,7 +457,7 @@ __global__ __launch_bounds__(ChainNT<T>::NT) void tri_chain_kernel(int n, int nr
         return;
       }
       if constexpr (MF) {
-        chain_mfma<true>(Sb, LDS, Xs, macc);
+        chain_mfma_b<true>(reinterpret_cast<const float(&)[32]>(areg), Xs, macc);
       } else {
 #pragma unroll 8
         for (int j = 0; j < TB; ++j) {
@@ -431,7 +466,9 @@ __global__ __launch_bounds__(ChainNT<T>::NT) void tri_chain_kernel(int n, int nr
           for (int q = 0; q < CPT; ++q) acc[q] = fma(-a, Xs[j * NC + g * CPT + q], acc[q]);
         }
       }
-      __syncthreads();
+      // after the last tile the folded path publishes without touching LDS
+      // (and ends with its own barrier): no barrier in between
+      if (!FOLD || idx + 1 < cnt) __syncthreads();
     }
     T out[CPT];
     if constexpr (FOLD) {
