@@ -256,8 +256,9 @@ int mat_gpu_posv_mixed(void *stream, size_t n, size_t nrhs, const double *a, siz
  * x untouched when not positive definite); column-major at a + a_off[i];
  * right-hand side b + v_off[i], solution x + v_off[i].  n and kind are host
  * arrays; everything else is device memory.
- * A plan sorts the batch into size bins once and captures the single
- * persistent launch in a hipGraph; mat_gpu_batch_run replays it (the data
+ * A plan sorts the batch into size bins once and captures the launches (a
+ * counter reset and one persistent kernel per size bin and kind) in a
+ * hipGraph; mat_gpu_batch_run replays it (the data
  * behind the pointers may change between runs, the pointers and sizes may
  * not).  autotune != 0 times the candidate launch shapes on this batch. */
 typedef struct mat_gpu_batch_plan mat_gpu_batch_plan;

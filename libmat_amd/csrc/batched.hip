@@ -8,21 +8,25 @@
 //  * Size bins: a matrix of order n is handled by a group of NB lanes,
 //    NB = max(8, next power of two >= n) in {8, 16, 32, 64}, so a
 //    wavefront factors 64/NB matrices at once.  Lane r of a group holds row
-//    r of its matrix in registers; column steps exchange values with
-//    in-group shuffles, so a factorization never touches LDS or a barrier
-//    (the Cholesky back substitution stages L through LDS).
+//    r of its matrix in registers.  A column step finds the pivot with one
+//    64-bit key max over the group (DPP / swizzle) and hands the pivot row
+//    (LU) or the column (Cholesky) to the group through the wavefront's own
+//    LDS (one lane or the column's lanes store, every lane reads float4
+//    chunks back); no workgroup barrier is involved.  The Cholesky back
+//    substitution stages L through LDS.
 //  * Row interchanges are logical: each lane tracks the position of its
 //    row; the pivot search picks max |a| with the lowest position on ties
 //    (mat__iamax, mat.h:1247-1262); the last column is not pivoted
 //    (mat.h:4214) and a pivot below eps skips its elimination step.
 //  * Forward substitution is fused into the elimination; back
 //    substitution reads U from the owning lanes' registers.
-//  * One persistent launch: wavefronts pull tasks (one task = the matrices
-//    of one wavefront, same bin and kind) from an atomic counter, largest
-//    bins first.  The launch (counter reset + kernel) is captured once in a
-//    hipGraph and replayed.
+//  * One persistent kernel per (bin, kind) class, so each class gets the
+//    registers and occupancy its matrices need: its wavefronts pull tasks
+//    (one task = the matrices of one wavefront) from the class's atomic
+//    counter.  The launches (one counter reset, then the class kernels in
+//    order) are captured once in a hipGraph and replayed.
 //  * Autotuning picks the number of resident workgroups per CU and the
-//    task order by timing graph replays on the actual batch.
+//    class order by timing graph replays on the actual batch.
 #include <algorithm>
 #include <chrono>
 #include <vector>
