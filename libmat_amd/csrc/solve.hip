@@ -171,7 +171,20 @@ __global__ __launch_bounds__(256) void residual_partial_kernel(int n, int nrhs, 
   for (int c = 0; c < RES_NR; ++c) acc[c] = 0.0;
   if (i < n) {
     const double *a = A + i + (long)k0 * lda;
-    for (int kk = 0; kk < kn; ++kk) {
+    // eight loads in flight per thread before their FMAs (the k order of the
+    // sums is unchanged, so the result is too)
+    constexpr int U = 8;
+    int kk = 0;
+    for (; kk + U <= kn; kk += U) {
+      double v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) v[u] = a[(long)(kk + u) * lda];
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int c = 0; c < RES_NR; ++c) acc[c] = fma(v[u], Xs[(kk + u) * RES_NR + c], acc[c]);
+    }
+    for (; kk < kn; ++kk) {
       const double v = a[(long)kk * lda];
 #pragma unroll
       for (int c = 0; c < RES_NR; ++c) acc[c] = fma(v, Xs[kk * RES_NR + c], acc[c]);
