@@ -98,6 +98,87 @@ __global__ __launch_bounds__(TRSM_NT) void trsm_leaf_kernel(int n, int nrhs, Vie
   }
 }
 
+// Leaves of 17..64 rows with each right-hand-side column split over G =
+// ceil(n/16) lanes of one wavefront, 16 rows per lane (64/G columns per
+// wavefront, four wavefronts per workgroup).  The triangle is staged once
+// in LDS in solve order (an upper solve is the lower solve of the reversed
+// rows), column-major with a padded stride.  Step c: the lane holding row c
+// divides by the diagonal (non-unit), stores the finished element and hands
+// it to the other lanes of its column (one cross-lane read); lanes holding
+// later rows apply the column's 16 coefficients of their rows.  Per step a
+// lane does 16 FMAs where the one-lane-per-column leaf does up to 64.
+template <typename T, bool LOWER, bool UNIT, int G>
+__global__ __launch_bounds__(256) void trsm_split_kernel(int n, int nrhs, View<const T> Tm,
+                                                         View<T> B) {
+  constexpr int CW = 64 / G;                   // columns per wavefront
+  constexpr int NP = 16 * G;                   // rows held (solve order)
+  constexpr int LS = NP + 16 / (int)sizeof(T);  // padded column stride, 16-byte aligned
+  __shared__ __align__(16) T Ls[NP * LS];      // Ls[c * LS + p] = L'(p, c)
+  const int t = threadIdx.x, l = t & 63, w = t >> 6;
+  auto row = [&](int p) { return LOWER ? p : n - 1 - p; };
+  // stage L'(p, c) = T(row(p), row(c)) for p >= c, zero elsewhere; the
+  // index order follows the view's contiguous dimension
+  const bool tr_rm = Tm.cs == 1;
+  for (int e = t; e < NP * NP; e += 256) {
+    const int a = e / NP, b = e % NP;
+    const int p = (tr_rm == LOWER) ? a : b, c = (tr_rm == LOWER) ? b : a;
+    T v = T(0);
+    if (p < n && c < n && p >= c) v = Tm(row(p), row(c));
+    Ls[c * LS + p] = v;
+  }
+  __syncthreads();
+  const int jl = l % CW, q = l / CW;
+  const int j = (blockIdx.x * 4 + w) * CW + jl;
+  if ((blockIdx.x * 4 + w) * CW >= nrhs) return;  // whole wavefront idle
+  const bool colok = j < nrhs;
+  T x[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    const int p = 16 * q + i;
+    x[i] = (colok && p < n) ? B(row(p), j) : T(0);
+  }
+#pragma unroll
+  for (int qc = 0; qc < G; ++qc) {
+    const bool same = q == qc, below = q > qc;
+    const int owner = (jl + CW * qc) * 4;  // bpermute byte address of the owner lane
+#pragma unroll
+    for (int sl = 0; sl < 16; ++sl) {
+      const int c = 16 * qc + sl;
+      if (c >= n) break;
+      // this lane's 16 rows of column c
+      T tc[16];
+      const T *col = Ls + c * LS + 16 * q;
+#pragma unroll
+      for (int k = 0; k < 16; k += 16 / (int)sizeof(T)) {
+        if constexpr (sizeof(T) == 4) {
+          const float4 v = *reinterpret_cast<const float4 *>(col + k);
+          tc[k] = v.x, tc[k + 1] = v.y, tc[k + 2] = v.z, tc[k + 3] = v.w;
+        } else {
+          const double2 v = *reinterpret_cast<const double2 *>(col + k);
+          tc[k] = v.x, tc[k + 1] = v.y;
+        }
+      }
+      if constexpr (!UNIT) {
+        if (same) x[sl] = x[sl] / tc[sl];
+      }
+      T xc;
+      if constexpr (sizeof(T) == 4) {
+        xc = __int_as_float(__builtin_amdgcn_ds_bpermute(owner, __float_as_int(x[sl])));
+      } else {
+        const long long u = __double_as_longlong(x[sl]);
+        const int lo = __builtin_amdgcn_ds_bpermute(owner, (int)(unsigned)u);
+        const int hi = __builtin_amdgcn_ds_bpermute(owner, (int)(unsigned)(u >> 32));
+        xc = __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) |
+                                              (unsigned)lo));
+      }
+      if (same && colok) B(row(c), j) = x[sl];
+#pragma unroll
+      for (int i = 0; i < 16; ++i)
+        if (below || (same && i > sl)) x[i] = fma(-tc[i], xc, x[i]);
+    }
+  }
+}
+
 // C -= A * B with the requested multiply precision.
 template <typename T>
 static void gemm_sub(hipStream_t s, long M, long N, long K, View<const T> A, View<const T> B,
@@ -154,6 +235,22 @@ template <typename T>
 void trsm_left(hipStream_t s, bool lower, bool unit, int n, int nrhs, View<const T> Tm, View<T> B,
                Mul mul) {
   if (n <= 0 || nrhs <= 0) return;
+  if (n > 16 && n <= 64 && nrhs <= 4096) {
+    KScope ks(s, "trsm");
+    const dim3 blk(256);
+#define MG_SPLIT(G_)                                                                                \
+  {                                                                                                 \
+    const dim3 grid(ceil_div(nrhs, 4 * (64 / G_)));                                               \
+    if (lower && unit) hipLaunchKernelGGL((trsm_split_kernel<T, true, true, G_>), grid, blk, 0, s, n, nrhs, Tm, B);   \
+    else if (lower) hipLaunchKernelGGL((trsm_split_kernel<T, true, false, G_>), grid, blk, 0, s, n, nrhs, Tm, B);     \
+    else if (unit) hipLaunchKernelGGL((trsm_split_kernel<T, false, true, G_>), grid, blk, 0, s, n, nrhs, Tm, B);      \
+    else hipLaunchKernelGGL((trsm_split_kernel<T, false, false, G_>), grid, blk, 0, s, n, nrhs, Tm, B);              \
+  }
+    if (n <= 32) MG_SPLIT(2) else MG_SPLIT(4)
+#undef MG_SPLIT
+    MG_HIP(hipGetLastError());
+    return;
+  }
   if (n <= TRSM_TB && nrhs <= 4096) {
     // small: one launch of the register-resident leaf
     dim3 grid(ceil_div(nrhs, TRSM_NT)), blk(TRSM_NT);
