@@ -98,8 +98,8 @@ __global__ __launch_bounds__(TRSM_NT) void trsm_leaf_kernel(int n, int nrhs, Vie
   }
 }
 
-// Leaves of 17..64 rows with each right-hand-side column split over G =
-// ceil(n/16) lanes of one wavefront, 16 rows per lane (64/G columns per
+// Leaves of 17..128 rows with each right-hand-side column split over G =
+// 2, 4 or 8 lanes of one wavefront, 16 rows per lane (64/G columns per
 // wavefront, four wavefronts per workgroup).  The triangle is staged once
 // in LDS in solve order (an upper solve is the lower solve of the reversed
 // rows), column-major with a padded stride.  Step c: the lane holding row c
@@ -235,7 +235,24 @@ template <typename T>
 void trsm_left(hipStream_t s, bool lower, bool unit, int n, int nrhs, View<const T> Tm, View<T> B,
                Mul mul) {
   if (n <= 0 || nrhs <= 0) return;
-  if (n > 16 && n <= 64 && nrhs <= 4096) {
+  // narrow right-hand sides (the factorizations' critical-path solves): two
+  // split leaves and one GEMM, fewer and shorter launches than inverting
+  // the diagonal blocks first; wide ones keep the inverse-block GEMMs,
+  // which run on MFMA
+  if (n > 128 && n <= 256 && nrhs <= 256) {
+    const int h = 128, r = n - h;
+    if (lower) {
+      trsm_left<T>(s, true, unit, h, nrhs, Tm, B, mul);
+      gemm_sub<T>(s, r, nrhs, h, Tm.sub(h, 0), cview(B), B.sub(h, 0), mul);
+      trsm_left<T>(s, true, unit, r, nrhs, Tm.sub(h, h), B.sub(h, 0), mul);
+    } else {
+      trsm_left<T>(s, false, unit, r, nrhs, Tm.sub(h, h), B.sub(h, 0), mul);
+      gemm_sub<T>(s, h, nrhs, r, Tm.sub(0, h), cview(B.sub(h, 0)), B, mul);
+      trsm_left<T>(s, false, unit, h, nrhs, Tm, B, mul);
+    }
+    return;
+  }
+  if (n > 16 && (n <= 64 ? nrhs <= 4096 : n <= 128 && nrhs <= 256)) {
     KScope ks(s, "trsm");
     const dim3 blk(256);
 #define MG_SPLIT(G_)                                                                                \
@@ -246,7 +263,7 @@ void trsm_left(hipStream_t s, bool lower, bool unit, int n, int nrhs, View<const
     else if (unit) hipLaunchKernelGGL((trsm_split_kernel<T, false, true, G_>), grid, blk, 0, s, n, nrhs, Tm, B);      \
     else hipLaunchKernelGGL((trsm_split_kernel<T, false, false, G_>), grid, blk, 0, s, n, nrhs, Tm, B);              \
   }
-    if (n <= 32) MG_SPLIT(2) else MG_SPLIT(4)
+    if (n <= 32) MG_SPLIT(2) else if (n <= 64) MG_SPLIT(4) else MG_SPLIT(8)
 #undef MG_SPLIT
     MG_HIP(hipGetLastError());
     return;
