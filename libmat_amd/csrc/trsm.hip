@@ -113,17 +113,24 @@ __global__ __launch_bounds__(256) void trsm_split_kernel(int n, int nrhs, View<c
   constexpr int CW = 64 / G;                   // columns per wavefront
   constexpr int NP = 16 * G;                   // rows held (solve order)
   constexpr int LS = NP + 16 / (int)sizeof(T);  // padded column stride, 16-byte aligned
-  __shared__ __align__(16) T Ls[NP * LS];      // Ls[c * LS + p] = L'(p, c)
+  constexpr int VW = 16 / (int)sizeof(T);       // elements per 16-byte LDS read
+  // Ls[c * LS + p] = L'(p, c) strictly below the diagonal, zero elsewhere;
+  // the diagonal apart in Ds (non-unit solves)
+  __shared__ __align__(16) T Ls[NP * LS];
+  __shared__ T Ds[NP];
   const int t = threadIdx.x, l = t & 63, w = t >> 6;
   auto row = [&](int p) { return LOWER ? p : n - 1 - p; };
-  // stage L'(p, c) = T(row(p), row(c)) for p >= c, zero elsewhere; the
-  // index order follows the view's contiguous dimension
+  // the index order follows the view's contiguous dimension
   const bool tr_rm = Tm.cs == 1;
   for (int e = t; e < NP * NP; e += 256) {
     const int a = e / NP, b = e % NP;
     const int p = (tr_rm == LOWER) ? a : b, c = (tr_rm == LOWER) ? b : a;
     T v = T(0);
     if (p < n && c < n && p >= c) v = Tm(row(p), row(c));
+    if (p == c) {
+      Ds[c] = c < n ? v : T(1);  // padding steps divide 0 by 1
+      v = T(0);
+    }
     Ls[c * LS + p] = v;
   }
   __syncthreads();
@@ -137,29 +144,41 @@ __global__ __launch_bounds__(256) void trsm_split_kernel(int n, int nrhs, View<c
     const int p = 16 * q + i;
     x[i] = (colok && p < n) ? B(row(p), j) : T(0);
   }
+  // this lane's 16 rows of column c
+  auto column = [&](int c, T(&tc)[16]) {
+    const T *col = Ls + c * LS + 16 * q;
 #pragma unroll
+    for (int k = 0; k < 16; k += VW) {
+      if constexpr (sizeof(T) == 4) {
+        const float4 v = *reinterpret_cast<const float4 *>(col + k);
+        tc[k] = v.x, tc[k + 1] = v.y, tc[k + 2] = v.z, tc[k + 3] = v.w;
+      } else {
+        const double2 v = *reinterpret_cast<const double2 *>(col + k);
+        tc[k] = v.x, tc[k + 1] = v.y;
+      }
+    }
+  };
+  T tc[16], fin[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) fin[i] = T(0);
+  column(0, tc);
+  // runtime loop over the row groups, unrolled over the 16 steps inside one
+  // (static register indices): the loop body stays small enough to run
+  // from a warm instruction cache, where 16*G unrolled steps would not.
+  // No branch per step: steps past n see zero coefficients and broadcast
+  // zero, and each finished element is kept in fin[] (one select) and
+  // stored at the end.
+#pragma unroll 1
   for (int qc = 0; qc < G; ++qc) {
-    const bool same = q == qc, below = q > qc;
+    const bool same = q == qc;
     const int owner = (jl + CW * qc) * 4;  // bpermute byte address of the owner lane
 #pragma unroll
     for (int sl = 0; sl < 16; ++sl) {
       const int c = 16 * qc + sl;
-      if (c >= n) break;
-      // this lane's 16 rows of column c
-      T tc[16];
-      const T *col = Ls + c * LS + 16 * q;
-#pragma unroll
-      for (int k = 0; k < 16; k += 16 / (int)sizeof(T)) {
-        if constexpr (sizeof(T) == 4) {
-          const float4 v = *reinterpret_cast<const float4 *>(col + k);
-          tc[k] = v.x, tc[k + 1] = v.y, tc[k + 2] = v.z, tc[k + 3] = v.w;
-        } else {
-          const double2 v = *reinterpret_cast<const double2 *>(col + k);
-          tc[k] = v.x, tc[k + 1] = v.y;
-        }
-      }
       if constexpr (!UNIT) {
-        if (same) x[sl] = x[sl] / tc[sl];
+        const T d = Ds[c];
+        const T xd = x[sl] / d;
+        x[sl] = same ? xd : x[sl];
       }
       T xc;
       if constexpr (sizeof(T) == 4) {
@@ -171,11 +190,23 @@ __global__ __launch_bounds__(256) void trsm_split_kernel(int n, int nrhs, View<c
         xc = __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) |
                                               (unsigned)lo));
       }
-      if (same && colok) B(row(c), j) = x[sl];
+      // the next column's coefficients while xc is in flight
+      T tn[16];
+      column(c + 1 < NP ? c + 1 : NP - 1, tn);
+      fin[sl] = same ? x[sl] : fin[sl];
+      // unconditional: coefficients on or above the diagonal are zero, so
+      // rows already solved only take zero updates in registers whose
+      // values were kept in fin[]
 #pragma unroll
-      for (int i = 0; i < 16; ++i)
-        if (below || (same && i > sl)) x[i] = fma(-tc[i], xc, x[i]);
+      for (int i = 0; i < 16; ++i) x[i] = fma(-tc[i], xc, x[i]);
+#pragma unroll
+      for (int i = 0; i < 16; ++i) tc[i] = tn[i];
     }
+  }
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    const int p = 16 * q + i;
+    if (colok && p < n) B(row(p), j) = fin[i];
   }
 }
 
