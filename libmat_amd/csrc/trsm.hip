@@ -98,20 +98,20 @@ __global__ __launch_bounds__(TRSM_NT) void trsm_leaf_kernel(int n, int nrhs, Vie
   }
 }
 
-// Leaves of 17..128 rows with each right-hand-side column split over G =
-// 2, 4 or 8 lanes of one wavefront, 16 rows per lane (64/G columns per
-// wavefront, four wavefronts per workgroup).  The triangle is staged once
+// Leaves of 17..128 rows with each right-hand-side column split over G
+// lanes of one wavefront, R rows per lane (64/G columns per wavefront, four
+// wavefronts per workgroup).  The triangle is staged once
 // in LDS in solve order (an upper solve is the lower solve of the reversed
 // rows), column-major with a padded stride.  Step c: the lane holding row c
 // divides by the diagonal (non-unit), stores the finished element and hands
 // it to the other lanes of its column (one cross-lane read); lanes holding
-// later rows apply the column's 16 coefficients of their rows.  Per step a
-// lane does 16 FMAs where the one-lane-per-column leaf does up to 64.
-template <typename T, bool LOWER, bool UNIT, int G>
+// later rows apply the column's R coefficients of their rows.  Per step a
+// lane does R FMAs where the one-lane-per-column leaf does up to 64.
+template <typename T, bool LOWER, bool UNIT, int G, int R>
 __global__ __launch_bounds__(256) void trsm_split_kernel(int n, int nrhs, View<const T> Tm,
                                                          View<T> B) {
   constexpr int CW = 64 / G;                   // columns per wavefront
-  constexpr int NP = 16 * G;                   // rows held (solve order)
+  constexpr int NP = R * G;                    // rows held (solve order)
   constexpr int LS = NP + 16 / (int)sizeof(T);  // padded column stride, 16-byte aligned
   constexpr int VW = 16 / (int)sizeof(T);       // elements per 16-byte LDS read
   // Ls[c * LS + p] = L'(p, c) strictly below the diagonal, zero elsewhere;
@@ -138,17 +138,17 @@ __global__ __launch_bounds__(256) void trsm_split_kernel(int n, int nrhs, View<c
   const int j = (blockIdx.x * 4 + w) * CW + jl;
   if ((blockIdx.x * 4 + w) * CW >= nrhs) return;  // whole wavefront idle
   const bool colok = j < nrhs;
-  T x[16];
+  T x[R];
 #pragma unroll
-  for (int i = 0; i < 16; ++i) {
-    const int p = 16 * q + i;
+  for (int i = 0; i < R; ++i) {
+    const int p = R * q + i;
     x[i] = (colok && p < n) ? B(row(p), j) : T(0);
   }
-  // this lane's 16 rows of column c
-  auto column = [&](int c, T(&tc)[16]) {
-    const T *col = Ls + c * LS + 16 * q;
+  // this lane's R rows of column c
+  auto column = [&](int c, T(&tc)[R]) {
+    const T *col = Ls + c * LS + R * q;
 #pragma unroll
-    for (int k = 0; k < 16; k += VW) {
+    for (int k = 0; k < R; k += VW) {
       if constexpr (sizeof(T) == 4) {
         const float4 v = *reinterpret_cast<const float4 *>(col + k);
         tc[k] = v.x, tc[k + 1] = v.y, tc[k + 2] = v.z, tc[k + 3] = v.w;
@@ -158,13 +158,13 @@ __global__ __launch_bounds__(256) void trsm_split_kernel(int n, int nrhs, View<c
       }
     }
   };
-  T tc[16], fin[16];
+  T tc[R], fin[R];
 #pragma unroll
-  for (int i = 0; i < 16; ++i) fin[i] = T(0);
+  for (int i = 0; i < R; ++i) fin[i] = T(0);
   column(0, tc);
-  // runtime loop over the row groups, unrolled over the 16 steps inside one
+  // runtime loop over the row groups, unrolled over the R steps inside one
   // (static register indices): the loop body stays small enough to run
-  // from a warm instruction cache, where 16*G unrolled steps would not.
+  // from a warm instruction cache, where R*G unrolled steps would not.
   // No branch per step: steps past n see zero coefficients and broadcast
   // zero, and each finished element is kept in fin[] (one select) and
   // stored at the end.
@@ -173,8 +173,8 @@ __global__ __launch_bounds__(256) void trsm_split_kernel(int n, int nrhs, View<c
     const bool same = q == qc;
     const int owner = (jl + CW * qc) * 4;  // bpermute byte address of the owner lane
 #pragma unroll
-    for (int sl = 0; sl < 16; ++sl) {
-      const int c = 16 * qc + sl;
+    for (int sl = 0; sl < R; ++sl) {
+      const int c = R * qc + sl;
       if constexpr (!UNIT) {
         const T d = Ds[c];
         const T xd = x[sl] / d;
@@ -191,21 +191,21 @@ __global__ __launch_bounds__(256) void trsm_split_kernel(int n, int nrhs, View<c
                                               (unsigned)lo));
       }
       // the next column's coefficients while xc is in flight
-      T tn[16];
+      T tn[R];
       column(c + 1 < NP ? c + 1 : NP - 1, tn);
       fin[sl] = same ? x[sl] : fin[sl];
       // unconditional: coefficients on or above the diagonal are zero, so
       // rows already solved only take zero updates in registers whose
       // values were kept in fin[]
 #pragma unroll
-      for (int i = 0; i < 16; ++i) x[i] = fma(-tc[i], xc, x[i]);
+      for (int i = 0; i < R; ++i) x[i] = fma(-tc[i], xc, x[i]);
 #pragma unroll
-      for (int i = 0; i < 16; ++i) tc[i] = tn[i];
+      for (int i = 0; i < R; ++i) tc[i] = tn[i];
     }
   }
 #pragma unroll
-  for (int i = 0; i < 16; ++i) {
-    const int p = 16 * q + i;
+  for (int i = 0; i < R; ++i) {
+    const int p = R * q + i;
     if (colok && p < n) B(row(p), j) = fin[i];
   }
 }
@@ -286,15 +286,15 @@ void trsm_left(hipStream_t s, bool lower, bool unit, int n, int nrhs, View<const
   if (n > 16 && (n <= 64 ? nrhs <= 4096 : n <= 128 && nrhs <= 256)) {
     KScope ks(s, "trsm");
     const dim3 blk(256);
-#define MG_SPLIT(G_)                                                                                \
+#define MG_SPLIT(G_, R_)                                                                            \
   {                                                                                                 \
     const dim3 grid(ceil_div(nrhs, 4 * (64 / G_)));                                               \
-    if (lower && unit) hipLaunchKernelGGL((trsm_split_kernel<T, true, true, G_>), grid, blk, 0, s, n, nrhs, Tm, B);   \
-    else if (lower) hipLaunchKernelGGL((trsm_split_kernel<T, true, false, G_>), grid, blk, 0, s, n, nrhs, Tm, B);     \
-    else if (unit) hipLaunchKernelGGL((trsm_split_kernel<T, false, true, G_>), grid, blk, 0, s, n, nrhs, Tm, B);      \
-    else hipLaunchKernelGGL((trsm_split_kernel<T, false, false, G_>), grid, blk, 0, s, n, nrhs, Tm, B);              \
+    if (lower && unit) hipLaunchKernelGGL((trsm_split_kernel<T, true, true, G_, R_>), grid, blk, 0, s, n, nrhs, Tm, B);   \
+    else if (lower) hipLaunchKernelGGL((trsm_split_kernel<T, true, false, G_, R_>), grid, blk, 0, s, n, nrhs, Tm, B);     \
+    else if (unit) hipLaunchKernelGGL((trsm_split_kernel<T, false, true, G_, R_>), grid, blk, 0, s, n, nrhs, Tm, B);      \
+    else hipLaunchKernelGGL((trsm_split_kernel<T, false, false, G_, R_>), grid, blk, 0, s, n, nrhs, Tm, B);              \
   }
-    if (n <= 32) MG_SPLIT(2) else if (n <= 64) MG_SPLIT(4) else MG_SPLIT(8)
+    if (n <= 32) MG_SPLIT(2, 16) else if (n <= 64) MG_SPLIT(4, 16) else MG_SPLIT(16, 8)
 #undef MG_SPLIT
     MG_HIP(hipGetLastError());
     return;
