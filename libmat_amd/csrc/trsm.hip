@@ -294,7 +294,7 @@ void trsm_left(hipStream_t s, bool lower, bool unit, int n, int nrhs, View<const
     else if (unit) hipLaunchKernelGGL((trsm_split_kernel<T, false, true, G_, R_>), grid, blk, 0, s, n, nrhs, Tm, B);      \
     else hipLaunchKernelGGL((trsm_split_kernel<T, false, false, G_, R_>), grid, blk, 0, s, n, nrhs, Tm, B);              \
   }
-    if (n <= 32) MG_SPLIT(2, 16) else if (n <= 64) MG_SPLIT(4, 16) else MG_SPLIT(16, 8)
+    if (n <= 32) MG_SPLIT(4, 8) else if (n <= 64) MG_SPLIT(8, 8) else MG_SPLIT(16, 8)
 #undef MG_SPLIT
     MG_HIP(hipGetLastError());
     return;
