@@ -6,7 +6,7 @@
 // d <= 0, l(j,j) = sqrt(d) and the column below is multiplied by 1/l(j,j).
 //
 // MI355X design: recursive blocking.  A 64-column diagonal block is
-// factored by one wavefront in registers (right-looking: scale the column,
+// factored by one workgroup in registers (right-looking: scale the column,
 // rank-1 update of the trailing columns); the panel below it is solved by
 // the triangular-solve kernels (A21 := A21 L11^-T), and the trailing lower
 // triangle is updated by one MFMA GEMM restricted to tiles on or below the
@@ -21,121 +21,89 @@ namespace mg {
 
 constexpr int CHOL_NB = 64;
 
+// One workgroup factors the (at most 64 x 64) diagonal block, right-looking,
+// with four lanes per row: 256 threads, thread t holds row t/4, columns
+// 16*(t%4) .. +15 in registers, so a step costs 16 FMAs per lane (one lane
+// per row would do up to 64).  Step J: the lanes holding column J write it,
+// unscaled and with zeros above the diagonal, to LDS (by step parity); one
+// barrier; every lane reads the pivot and the entries it needs and scales
+// them itself, then updates its 16 columns unconditionally (entries above
+// the diagonal take updates too and are never read); the lane holding
+// (r, J) then keeps the finished L(r, J).  The loop runs over the four
+// column slices, unrolled over the 16 steps of one (static register
+// indices, a small loop body).
 template <typename T>
-__device__ __forceinline__ T lane_value(T v, int src) {
-  if constexpr (sizeof(T) == 4) {
-    return __uint_as_float(__builtin_amdgcn_readlane(__float_as_uint(v), src));
-  } else {
-    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
-    const unsigned lo = __builtin_amdgcn_readlane((unsigned)u, src);
-    const unsigned hi = __builtin_amdgcn_readlane((unsigned)(u >> 32), src);
-    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
-  }
-}
-
-// Orders one wavefront's LDS traffic around a lane-to-lane hand-off.
-__device__ __forceinline__ void chol_wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-}
-
-// Rank-1 update of the slots [4*C0, 4*(C0+NC)): a[s-1] = a[s] - v * colb[s]
-// (the row's remaining columns move down one slot as they update).  The
-// group's LDS reads are all issued before its first update: one LDS latency
-// per group, not per four slots.
-template <typename T, int C0, int NC>
-__device__ __forceinline__ void chol_group(T (&a)[CHOL_NB], const T *colb, T v) {
-  T c[4 * NC];
-#pragma unroll
-  for (int k = 0; k < NC; ++k) {
-    if constexpr (sizeof(T) == 4) {
-      const float4 q = reinterpret_cast<const float4 *>(colb)[C0 + k];
-      c[4 * k] = q.x, c[4 * k + 1] = q.y, c[4 * k + 2] = q.z, c[4 * k + 3] = q.w;
-    } else {
-      const double2 q0 = reinterpret_cast<const double2 *>(colb)[2 * (C0 + k)];
-      const double2 q1 = reinterpret_cast<const double2 *>(colb)[2 * (C0 + k) + 1];
-      c[4 * k] = q0.x, c[4 * k + 1] = q0.y, c[4 * k + 2] = q1.x, c[4 * k + 3] = q1.y;
-    }
-  }
-#pragma unroll
-  for (int u = 0; u < 4 * NC; ++u) {
-    const int sl = 4 * C0 + u;
-    if (sl >= 1) a[sl - 1] = fma(-v, c[u], a[sl]);
-  }
-}
-
-// One wavefront factors the NB x NB diagonal block, right-looking: lane i
-// holds row i in registers with the current column in slot 0.  Step J:
-// the pivot comes from lane J (v_readlane), the column is scaled by 1/l,
-// each finished entry goes straight to LDS, and the column is handed to
-// every lane through LDS, rotated so that lane J+s's value sits at slot s
-// (aligned reads at static offsets); the rank-1 update then moves each
-// row's remaining columns down one slot, in groups of slots that uniform
-// branches skip past the block's end.  Entries above the diagonal
-// take updates too and are never read.  (A fully unrolled 64-step leaf is
-// ~16k instructions, run once per launch from a cold instruction cache.)
-template <typename T>
-__global__ __launch_bounds__(64) void chol_leaf_kernel(int n, T *A, long lda, int j0, int *info) {
-  constexpr int NB = CHOL_NB;
-  __shared__ T S[NB * (NB + 1)];        // S[c*(NB+1) + r] = L(r, c), as finished
-  __shared__ __align__(16) T colb[NB];  // column J: lane i's value at slot (i - J) mod NB
+__global__ __launch_bounds__(256) void chol_split_kernel(int n, T *A, long lda, int j0, int *info) {
+  constexpr int NB = CHOL_NB, CS = 16;
+  __shared__ __align__(16) T col[2][NB];
   if (*(volatile int *)info != 0) return;  // an earlier column already failed
-  const int i = threadIdx.x;
-  // row i of the lower triangle straight into registers: every load is
-  // issued before any is used (one memory latency, not NB)
-  T a[NB];
+  const int t = threadIdx.x, r = t >> 2, g = t & 3, c0 = CS * g;
+  if (t < 2 * NB) col[t / NB][t % NB] = T(0);  // rows past n stay zero
+  __syncthreads();
+  T a[CS];
 #pragma unroll
-  for (int c = 0; c < NB; ++c) a[c] = (i < n && c < n && i >= c) ? A[i + (long)c * lda] : T(0);
-#pragma unroll 1
-  for (int J = 0; J < n; ++J) {
-    const T d = lane_value(a[0], J);
-    if (d <= T(0)) {  // the reference's test; NaN passes it
-      if (i == 0) *info = j0 + J + 1;
-      return;
-    }
-    // fp32: one v_rsq_f32 (1 ulp) instead of an IEEE square root and
-    // divide on the step's critical path; tiny pivots keep the IEEE path
-    T l, inv;
-    if constexpr (sizeof(T) == 4) {
-      if (d >= 1e-30f) {
-        inv = __builtin_amdgcn_rsqf(d);
-        l = d * inv;
-      } else {
-        l = sqrtf(d);
-        inv = 1.0f / l;
-      }
-    } else {
-      l = sqrt(d);
-      inv = T(1) / l;
-    }
-    const T v = i == J ? l : (i > J ? a[0] * inv : a[0]);
-    if (i >= J) S[J * (NB + 1) + i] = v;
-    colb[(i - J) & (NB - 1)] = v;
-    chol_wave_sync();
-    const int live = n - J;  // slots 0 .. live-1 hold columns J .. n-1
-    // groups of slots past the block's end are skipped (uniform branches)
-    if constexpr (sizeof(T) == 4) {
-      chol_group<T, 0, 8>(a, colb, v);
-      if (32 < live) chol_group<T, 8, 8>(a, colb, v);
-    } else {
-      chol_group<T, 0, 4>(a, colb, v);
-      if (16 < live) chol_group<T, 4, 4>(a, colb, v);
-      if (32 < live) chol_group<T, 8, 4>(a, colb, v);
-      if (48 < live) chol_group<T, 12, 4>(a, colb, v);
-    }
-    chol_wave_sync();  // colb is rewritten by the next step
+  for (int i = 0; i < CS; ++i) {
+    const int c = c0 + i;
+    a[i] = (r < n && c < n && c <= r) ? A[r + (long)c * lda] : T(0);
   }
-  chol_wave_sync();
-#pragma unroll 8
-  for (int c = 0; c < NB; ++c)
-    if (i < n && c < n && i >= c) A[i + (long)c * lda] = S[c * (NB + 1) + i];
+#pragma unroll 1
+  for (int gq = 0; gq < NB / CS; ++gq) {
+#pragma unroll
+    for (int sl = 0; sl < CS; ++sl) {
+      const int J = CS * gq + sl;
+      if (J >= n) break;
+      T *cb = col[J & 1];
+      if (g == gq && r < n) cb[r] = r >= J ? a[sl] : T(0);
+      __syncthreads();
+      const T d = cb[J];
+      if (d <= T(0)) {  // the reference's test (every lane reads the same d); NaN passes it
+        if (t == 0) *info = j0 + J + 1;
+        return;
+      }
+      T l, inv;
+      if constexpr (sizeof(T) == 4) {
+        if (d >= 1e-30f) {
+          inv = __builtin_amdgcn_rsqf(d);
+          l = d * inv;
+        } else {
+          l = sqrtf(d);
+          inv = 1.0f / l;
+        }
+      } else {
+        l = sqrt(d);
+        inv = T(1) / l;
+      }
+      const T lr = r > J ? cb[r] * inv : T(0);  // L(r, J) below the diagonal
+      const T f = lr * inv;
+      T cv[CS];
+#pragma unroll
+      for (int k = 0; k < CS; k += 16 / (int)sizeof(T)) {
+        if constexpr (sizeof(T) == 4) {
+          const float4 v = *reinterpret_cast<const float4 *>(cb + c0 + k);
+          cv[k] = v.x, cv[k + 1] = v.y, cv[k + 2] = v.z, cv[k + 3] = v.w;
+        } else {
+          const double2 v = *reinterpret_cast<const double2 *>(cb + c0 + k);
+          cv[k] = v.x, cv[k + 1] = v.y;
+        }
+      }
+      // a(r, c) -= L(r, J) L(c, J), L(c, J) = cv[c] / l (zero for c < J)
+#pragma unroll
+      for (int i = 0; i < CS; ++i) a[i] = fma(-f, cv[i], a[i]);
+      if (g == gq) a[sl] = r > J ? lr : (r == J ? l : a[sl]);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < CS; ++i) {
+    const int c = c0 + i;
+    if (r < n && c < n && c <= r) A[r + (long)c * lda] = a[i];
+  }
 }
 
 template <typename T>
 static void potrf_rec(hipStream_t s, int n, T *A, long lda, int j0, int *info, Mul mul) {
   if (n <= CHOL_NB) {
     KScope ks(s, "chol_leaf", (double)n * n * sizeof(T));
-    hipLaunchKernelGGL((chol_leaf_kernel<T>), dim3(1), dim3(64), 0, s, n, A, lda, j0, info);
+    hipLaunchKernelGGL((chol_split_kernel<T>), dim3(1), dim3(256), 0, s, n, A, lda, j0, info);
     MG_HIP(hipGetLastError());
     return;
   }
