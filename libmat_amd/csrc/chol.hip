@@ -20,24 +20,28 @@
 namespace mg {
 
 constexpr int CHOL_NB = 64;
+// columns per lane in the diagonal-block kernel (measured at n = 8192:
+// 8 beats 16 by ~2%, 32 is 5% slower)
+constexpr int CHOL_CS = 8;
 
 // One workgroup factors the (at most 64 x 64) diagonal block, right-looking,
-// with four lanes per row: 256 threads, thread t holds row t/4, columns
-// 16*(t%4) .. +15 in registers, so a step costs 16 FMAs per lane (one lane
-// per row would do up to 64).  Step J: the lanes holding column J write it,
+// with G = 64/CS lanes per row: thread t holds row t/G, columns CS*(t%G) ..
+// +CS-1 in registers, so a step costs CS FMAs per lane (one lane per row
+// would do up to 64).  Step J: the lanes holding column J write it,
 // unscaled and with zeros above the diagonal, to LDS (by step parity); one
 // barrier; every lane reads the pivot and the entries it needs and scales
 // them itself, then updates its 16 columns unconditionally (entries above
 // the diagonal take updates too and are never read); the lane holding
-// (r, J) then keeps the finished L(r, J).  The loop runs over the four
-// column slices, unrolled over the 16 steps of one (static register
-// indices, a small loop body).
-template <typename T>
-__global__ __launch_bounds__(256) void chol_split_kernel(int n, T *A, long lda, int j0, int *info) {
-  constexpr int NB = CHOL_NB, CS = 16;
+// (r, J) then keeps the finished L(r, J).  The loop runs over the G column
+// slices, unrolled over the CS steps of one (static register indices, a
+// small loop body).
+template <typename T, int CS>
+__global__ __launch_bounds__(CHOL_NB * (CHOL_NB / CS)) void chol_split_kernel(int n, T *A, long lda,
+                                                                             int j0, int *info) {
+  constexpr int NB = CHOL_NB, G = NB / CS;  // lanes per row
   __shared__ __align__(16) T col[2][NB];
   if (*(volatile int *)info != 0) return;  // an earlier column already failed
-  const int t = threadIdx.x, r = t >> 2, g = t & 3, c0 = CS * g;
+  const int t = threadIdx.x, r = t / G, g = t % G, c0 = CS * g;
   if (t < 2 * NB) col[t / NB][t % NB] = T(0);  // rows past n stay zero
   __syncthreads();
   T a[CS];
@@ -103,7 +107,8 @@ template <typename T>
 static void potrf_rec(hipStream_t s, int n, T *A, long lda, int j0, int *info, Mul mul) {
   if (n <= CHOL_NB) {
     KScope ks(s, "chol_leaf", (double)n * n * sizeof(T));
-    hipLaunchKernelGGL((chol_split_kernel<T>), dim3(1), dim3(256), 0, s, n, A, lda, j0, info);
+    hipLaunchKernelGGL((chol_split_kernel<T, CHOL_CS>), dim3(1), dim3(CHOL_NB * (CHOL_NB / CHOL_CS)),
+                       0, s, n, A, lda, j0, info);
     MG_HIP(hipGetLastError());
     return;
   }

@@ -203,12 +203,15 @@ def test_reference_random_chol(gpu, prec, n, tol):
 
 
 @pytest.mark.parametrize("prec", PRECS)
-def test_det_inv(gpu, prec):
+@pytest.mark.parametrize("n", [70, 130, 200, 256])
+def test_det_inv(gpu, prec, n):
+    """mat_inv solves with n right-hand sides: n = 70 runs the 65..128-row
+    split triangular leaves, 130..256 the two-leaf path, lower unit and
+    upper non-unit."""
     g = gpu[prec]
     A = np.array([[2, 0, 0, 0], [0, 3, 0, 0], [0, 0, 4, 0], [0, 0, 0, 3]], DT[prec])
     assert abs(g.mat_det(A) - 72) < 1e-4  # tests/test_mat_det.c:73-89
     rng = np.random.default_rng(5)
-    n = 70
     A = rng.uniform(-1, 1, (n, n)) + n * np.eye(n)
     Ai = g.mat_inv(A.astype(DT[prec]))
     assert np.abs(A @ Ai - np.eye(n)).max() < 100 * n * EPS[prec]
