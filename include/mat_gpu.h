@@ -70,6 +70,12 @@ void mat_syrk(Mat *C, const Mat *A, mat_elem_t alpha, mat_elem_t beta, char uplo
 /* C = alpha * A^T * A + beta * C on the uplo triangle.
                                     replaces mat.h:622  (impl 5727-5734) */
 void mat_syrk_t(Mat *C, const Mat *A, mat_elem_t alpha, mat_elem_t beta, char uplo);
+/* A = Q R by Householder reflections, A m x n with m >= n, Q m x m, R m x n
+   (R keeps the reflectors' rounding residue below its diagonal, as libmat
+   leaves it).                      replaces mat.h:744  (impl 3907-4063) */
+void mat_qr(const Mat *A, Mat *Q, Mat *R);
+/* R of A = Q R only.               replaces mat.h:749  (impl 4066-4192) */
+void mat_qr_r(const Mat *A, Mat *R);
 /* PAQ = LU, full pivoting; returns row swaps + column swaps.
                                     replaces mat.h:771  (impl 4476-4578) */
 int mat_lu(const Mat *A, Mat *L, Mat *U, Perm *p, Perm *q);

@@ -23,7 +23,8 @@ ROOT = os.path.dirname(HERE)
 
 # Drop-in symbols (include/mat_gpu.h part 1) and extension symbols (part 2).
 DROPIN_SYMBOLS = [
-    "mat_mul", "mat_rmul", "mat_gemm", "mat_syrk", "mat_syrk_t", "mat_lu", "mat_plu",
+    "mat_mul", "mat_rmul", "mat_gemm", "mat_syrk", "mat_syrk_t", "mat_qr", "mat_qr_r", "mat_lu",
+    "mat_plu",
     "mat_chol", "mat_solve",
     "mat_solve_spd", "mat_solve_tril", "mat_solve_tril_unit", "mat_solve_triu",
     "mat_solve_trilt", "mat_det", "mat_inv",
@@ -98,6 +99,8 @@ class GpuLib:
         s("mat_gemm", None, [M, self.ct, M, M, self.ct])
         s("mat_syrk", None, [M, M, self.ct, self.ct, C.c_char])
         s("mat_syrk_t", None, [M, M, self.ct, self.ct, C.c_char])
+        s("mat_qr", None, [M, M, M])
+        s("mat_qr_r", None, [M, M])
         s("mat_lu", i32, [M, M, M, P, P])
         s("mat_plu", i32, [M, M, M, P])
         s("mat_chol", i32, [M, M])
@@ -176,6 +179,21 @@ class GpuLib:
         fn = self.lib.mat_syrk_t if trans else self.lib.mat_syrk
         fn(C.byref(c), C.byref(a), alpha, beta, uplo.encode())
         return cb
+
+    def mat_qr(self, A):
+        """mat_qr on a copy of A; returns (Q, R)."""
+        a, ab = self.to_mat(A)
+        q, qb = self.empty_mat(a.rows, a.rows)
+        r, rb = self.empty_mat(a.rows, a.cols)
+        self.lib.mat_qr(C.byref(a), C.byref(q), C.byref(r))
+        return qb, rb
+
+    def mat_qr_r(self, A):
+        """mat_qr_r on a copy of A; returns R."""
+        a, ab = self.to_mat(A)
+        r, rb = self.empty_mat(a.rows, a.cols)
+        self.lib.mat_qr_r(C.byref(a), C.byref(r))
+        return rb
 
     def mat_mul(self, A, B):
         a, ab = self.to_mat(A)

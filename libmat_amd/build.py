@@ -21,7 +21,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
 HIP_SOURCES = ["gemm.hip", "lowp.hip", "panel.hip", "trsm.hip", "chol.hip", "lu.hip",
-               "solve.hip", "trsv.hip", "rbt.hip", "batched.hip", "tslu.hip"]
+               "solve.hip", "trsv.hip", "rbt.hip", "batched.hip", "tslu.hip", "qr.hip"]
 CPP_SOURCES = ["runtime.cpp", "capi_ext.cpp"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", f"-I{CSRC}", f"-I{os.path.join(ROOT, 'include')}"]

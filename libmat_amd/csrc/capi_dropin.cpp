@@ -180,6 +180,44 @@ void mat_syrk_t(Mat *C, const Mat *A, T alpha, T beta, char uplo) {
   syrk_dropin(__func__, C, A, alpha, beta, uplo, true);
 }
 
+// -------------------------------------------------------------------- QR --
+// Householder QR (mat_qr, mat.h:3907-4063; mat_qr_r, mat.h:4066-4192): the
+// reference asserts m >= n and the output shapes (mat.h:3914-3916,
+// 4072-4073).
+void mat_qr(const Mat *A, Mat *Q, Mat *R) {
+  REQUIRE_MAT(A);
+  REQUIRE_MAT(Q);
+  REQUIRE_MAT(R);
+  REQUIRE(A->rows >= A->cols);
+  REQUIRE(Q->rows == A->rows && Q->cols == A->rows);
+  REQUIRE(R->rows == A->rows && R->cols == A->cols);
+  run(__func__, [&] {
+    hipStream_t s = mg::default_stream();
+    const int m = (int)A->rows, n = (int)A->cols;
+    T *dR = upload(s, "dropin.C", A);
+    T *dQ = (T *)mg::workspace(s).get("dropin.B", (size_t)m * m * sizeof(T));
+    mg::geqrf<T>(s, m, n, dR, m, dQ, kEps);
+    download(s, R, dR);
+    download(s, Q, dQ);
+    finish(s);
+  });
+}
+
+void mat_qr_r(const Mat *A, Mat *R) {
+  REQUIRE_MAT(A);
+  REQUIRE_MAT(R);
+  REQUIRE(A->rows >= A->cols);
+  REQUIRE(R->rows == A->rows && R->cols == A->cols);
+  run(__func__, [&] {
+    hipStream_t s = mg::default_stream();
+    const int m = (int)A->rows, n = (int)A->cols;
+    T *dR = upload(s, "dropin.C", A);
+    mg::geqrf<T>(s, m, n, dR, m, (T *)nullptr, kEps);
+    download(s, R, dR);
+    finish(s);
+  });
+}
+
 // -------------------------------------------------------- factorizations --
 int mat_plu(const Mat *A, Mat *L, Mat *U, Perm *p) {
   REQUIRE_MAT(A);

@@ -41,6 +41,13 @@ int ipiv_to_perm(int n, const int *ipiv, size_t *perm);
 template <typename T>
 void getrf_full(hipStream_t s, int n, T *A, long lda, int *rec, T eps);
 
+// -------------------------------------------------------------------- QR --
+// Householder QR in place on the column-major m x n R (m >= n, ld ldr);
+// Q (m x m, ld m), when not null, receives the orthogonal factor.
+// Columns with ||x|| < eps get no reflector.
+template <typename T>
+void geqrf(hipStream_t s, int m, int n, T *R, long ldr, T *Q, T eps);
+
 // -------------------------------------------------------------- Cholesky --
 // In-place lower Cholesky of a column-major matrix (only the lower triangle
 // is read or written).  info (device int) gets j+1 for the first column j

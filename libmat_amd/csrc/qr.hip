@@ -1,0 +1,192 @@
+// Householder QR of a column-major m x n matrix (m >= n), in place on R,
+// with the orthogonal factor Q accumulated on request.
+//
+// Reference semantics (mat_qr, mat.h:3907-4063; mat_qr_r, mat.h:4066-4192):
+// reflector j of column x = R(j:, j) is v = [1, x(1:)/(x0 - beta)],
+// beta = -sign(x0) ||x||, tau = -(x0 - beta)/beta (mat_householder,
+// mat.h:3634-3716); a column with ||x|| < eps gets v = 0 and tau = 0 and is
+// left as it is.  Every column c >= j of the block is updated as
+// R(j:, c) -= tau (v . R(j:, c)) v, so R keeps the reflectors' rounding
+// residue below its diagonal, as the reference leaves it.  Blocks of
+// reflectors reach the trailing columns and Q in compact-WY form,
+// H_0 ... H_{k-1} = I - Y T Y^T (mat__qr_build_T_, mat.h:3772-3827;
+// applies, mat.h:3846-3903).  The reference switches to the unblocked
+// algorithm below 64 columns; it is the same product of reflectors, so one
+// blocked path serves every size here.
+//
+// MI355X design.  Blocks of 32 reflectors (the reference uses 16; fewer,
+// deeper GEMMs).  One workgroup of 1024 threads factors a block's panel
+// column by column out of L2 (norm, reflector, then the block's remaining
+// columns, each column's dot product reduced by one wavefront); a
+// one-wavefront kernel builds T from Y^T Y; the trailing columns of R and
+// the columns of Q are updated by the library's MFMA GEMM (three products
+// each).
+#include "ops.hpp"
+
+namespace mg {
+
+namespace {
+
+constexpr int QR_NB = 32;   // reflectors per block
+constexpr int QR_NT = 1024; // threads of the panel workgroup
+constexpr int QR_NW = QR_NT / 64;
+
+template <typename T>
+__device__ inline T wave_sum(T v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+// Reflectors jb .. jb+kb-1 of R (column-major, ld ldr) into Y (len x kb,
+// len = m - jb, ld ldy; zeros above each reflector's unit entry) and tau;
+// the block's own columns of R are updated in place.
+template <typename T>
+__global__ __launch_bounds__(QR_NT) void qr_panel_kernel(int m, int jb, int kb, T *R, long ldr,
+                                                         T *Y, long ldy, T *tau_out, T eps) {
+  __shared__ T red[QR_NW];
+  __shared__ T wdot[QR_NB];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  for (int j = 0; j < kb; ++j) {
+    const int col = jb + j;
+    const long vlen = (long)m - col;
+    const T *x = R + (long)col * ldr + col;
+    T *y = Y + (long)j * ldy;
+    // ||x||^2: every thread sums the wavefront partials in the same order,
+    // so every thread holds the same norm and takes the same branch
+    T s = 0;
+    for (long i = t; i < vlen; i += QR_NT) s += x[i] * x[i];
+    s = wave_sum(s);
+    if (lane == 0) red[wave] = s;
+    __syncthreads();
+    T nsq = 0;
+#pragma unroll
+    for (int q = 0; q < QR_NW; ++q) nsq += red[q];
+    const T norm = sqrt(nsq);
+    for (long i = t; i < j; i += QR_NT) y[i] = 0;
+    T tau = 0;
+    if (norm < eps) {
+      for (long i = t; i < vlen; i += QR_NT) y[j + i] = 0;
+    } else {
+      const T x0 = x[0];
+      const T beta = x0 >= 0 ? -norm : norm;
+      const T v0 = x0 - beta;
+      const T v0_inv = T(1) / v0;
+      tau = -v0 / beta;
+      for (long i = t; i < vlen; i += QR_NT) y[j + i] = i == 0 ? T(1) : x[i] * v0_inv;
+    }
+    if (t == 0) tau_out[j] = tau;
+    // v complete; every thread has read x0 before column col changes
+    __syncthreads();
+    if (tau != 0) {
+      // w_c = (v . R(col:, c)) tau for the block's columns c >= j, one
+      // wavefront per column
+      for (int c = wave; c < kb; c += QR_NW) {
+        if (c < j) continue;
+        const T *rc = R + (long)(jb + c) * ldr + col;
+        T d = 0;
+        for (long i = lane; i < vlen; i += 64) d += y[j + i] * rc[i];
+        d = wave_sum(d);
+        if (lane == 0) wdot[c] = d * tau;
+      }
+      __syncthreads();
+      // R(col:, c) += (-w_c) v
+      for (long i = t; i < vlen; i += QR_NT) {
+        const T vi = y[j + i];
+        for (int c = j; c < kb; ++c) {
+          T *rc = R + (long)(jb + c) * ldr + col;
+          rc[i] += -wdot[c] * vi;
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// T (kb x kb, upper, ld kb) of the compact-WY form from G = Y^T Y (ld kb)
+// and tau: T(j,j) = tau_j, T(i,j) = -tau_j sum_{p=i}^{j-1} T(i,p) G(p,j)
+// (mat.h:3810-3823).  Lane i builds row i, which only reads its own row.
+template <typename T>
+__global__ __launch_bounds__(64) void qr_tfactor_kernel(int kb, const T *G, const T *tau, T *Tm) {
+  __shared__ T Ts[QR_NB][QR_NB + 1];
+  const int i = threadIdx.x;
+  if (i >= kb) return;
+  for (int c = 0; c < kb; ++c) Ts[i][c] = 0;
+  for (int j = 0; j < kb; ++j) {
+    if (i < j) {
+      T sum = 0;
+      for (int p = i; p < j; ++p) sum += Ts[i][p] * G[p + (long)j * kb];
+      Ts[i][j] = -tau[j] * sum;
+    } else if (i == j) {
+      Ts[i][j] = tau[j];
+    }
+  }
+  for (int c = 0; c < kb; ++c) Tm[i + (long)c * kb] = Ts[i][c];
+}
+
+template <typename T>
+__global__ void qr_eye_kernel(long m, T *Q) {
+  const long total = m * m;
+  for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (long)gridDim.x * blockDim.x)
+    Q[idx] = (idx % m == idx / m) ? T(1) : T(0);
+}
+
+}  // namespace
+
+template <typename T>
+void geqrf(hipStream_t s, int m, int n, T *R, long ldr, T *Q, T eps) {
+  MG_CHECK(m >= n && n >= 1, "geqrf: needs m >= n >= 1");
+  MG_CHECK(ldr >= m, "geqrf: leading dimension below m");
+  Workspace &ws = workspace(s);
+  T *Y = (T *)ws.get("qr.Y", (size_t)m * QR_NB * sizeof(T));
+  T *tau = (T *)ws.get("qr.tau", QR_NB * sizeof(T));
+  T *G = (T *)ws.get("qr.G", QR_NB * QR_NB * sizeof(T));
+  T *Tm = (T *)ws.get("qr.T", QR_NB * QR_NB * sizeof(T));
+  T *Ck = (T *)ws.get("qr.Ck", (size_t)QR_NB * n * sizeof(T));
+  T *Ck2 = (T *)ws.get("qr.Ck2", (size_t)QR_NB * n * sizeof(T));
+  T *Cm = Q ? (T *)ws.get("qr.Cm", (size_t)m * QR_NB * sizeof(T)) : nullptr;
+  T *Cm2 = Q ? (T *)ws.get("qr.Cm2", (size_t)m * QR_NB * sizeof(T)) : nullptr;
+  if (Q) {
+    const long total = (long)m * m;
+    const unsigned blocks = (unsigned)std::min<long>((total + 255) / 256, 8192);
+    hipLaunchKernelGGL((qr_eye_kernel<T>), dim3(blocks), dim3(256), 0, s, (long)m, Q);
+    MG_HIP(hipGetLastError());
+  }
+  for (int jb = 0; jb < n; jb += QR_NB) {
+    const int kb = std::min(QR_NB, n - jb);
+    const long len = (long)m - jb;
+    {
+      KScope ks(s, "qr_panel", 2.0 * len * kb * sizeof(T));
+      hipLaunchKernelGGL((qr_panel_kernel<T>), dim3(1), dim3(QR_NT), 0, s, m, jb, kb, R, ldr, Y,
+                         len, tau, eps);
+      MG_HIP(hipGetLastError());
+    }
+    const View<const T> y{Y, 1, len};
+    // G = Y^T Y, then T
+    gemm<T>(s, kb, kb, len, T(1), y.t(), y, T(0), View<T>{G, 1, kb});
+    hipLaunchKernelGGL((qr_tfactor_kernel<T>), dim3(1), dim3(64), 0, s, kb, G, tau, Tm);
+    MG_HIP(hipGetLastError());
+    const View<const T> tm{Tm, 1, kb};
+    if (jb + kb < n) {
+      // R(jb:, jb+kb:) -= Y (T^T (Y^T R(jb:, jb+kb:)))
+      const int tc = n - (jb + kb);
+      T *Rs = R + (long)(jb + kb) * ldr + jb;
+      gemm<T>(s, kb, tc, len, T(1), y.t(), View<const T>{Rs, 1, ldr}, T(0), View<T>{Ck, 1, kb});
+      gemm<T>(s, kb, tc, kb, T(1), tm.t(), View<const T>{Ck, 1, kb}, T(0), View<T>{Ck2, 1, kb});
+      gemm<T>(s, len, tc, kb, T(-1), y, View<const T>{Ck2, 1, kb}, T(1), View<T>{Rs, 1, ldr});
+    }
+    if (Q) {
+      // Q(:, jb:) -= ((Q(:, jb:) Y) T) Y^T
+      T *Qt = Q + (long)jb * m;
+      gemm<T>(s, m, kb, len, T(1), View<const T>{Qt, 1, m}, y, T(0), View<T>{Cm, 1, m});
+      gemm<T>(s, m, kb, kb, T(1), View<const T>{Cm, 1, m}, tm, T(0), View<T>{Cm2, 1, m});
+      gemm<T>(s, m, len, kb, T(-1), View<const T>{Cm2, 1, m}, y.t(), T(1), View<T>{Qt, 1, m});
+    }
+  }
+}
+
+template void geqrf<float>(hipStream_t, int, int, float *, long, float *, float);
+template void geqrf<double>(hipStream_t, int, int, double *, long, double *, double);
+
+}  // namespace mg

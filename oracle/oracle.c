@@ -402,3 +402,170 @@ elem ORC(det)(size_t n, const elem *A) {
   free(p);
   return d;
 }
+
+/* ---------------------------------------------------------------- QR --- */
+
+/* Strided GEMM with transpose flags, scalar branch of mat__gemm_strided_
+ * (mat.h:2719-2743): per-column beta-scale (mat__scal_kernel's special
+ * cases), then j-k-i with alpha folded into op(B)(k,j). */
+static void gemm_strided(elem *C, size_t ldc, elem alpha, const elem *A,
+                         size_t lda, int ta, const elem *B, size_t ldb, int tb,
+                         size_t M, size_t K, size_t N, elem beta) {
+  for (size_t j = 0; j < N; j++) scal(&C[j * ldc], beta, M);
+  for (size_t j = 0; j < N; j++) {
+    for (size_t k = 0; k < K; k++) {
+      elem ab = alpha * (tb ? B[k * ldb + j] : B[j * ldb + k]);
+      for (size_t i = 0; i < M; i++)
+        C[j * ldc + i] += (ta ? A[i * lda + k] : A[k * lda + i]) * ab;
+    }
+  }
+}
+
+static elem dot(const elem *a, const elem *b, size_t n) {
+  elem r = 0;
+  for (size_t i = 0; i < n; i++) r += a[i] * b[i];
+  return r;
+}
+
+/* Householder vector of x (mat_householder, mat.h:3634-3716): v[0] = 1,
+ * v[i] = x[i] / (x0 - beta), beta = -sign(x0) ||x||, tau = -v0 / beta;
+ * ||x|| < eps gives v = 0, tau = 0.  v may alias x. */
+static void householder(elem *v, elem *tau, const elem *x, size_t n) {
+  if (n == 0) {
+    *tau = 0;
+    return;
+  }
+  elem norm_sq = 0;
+  for (size_t i = 0; i < n; i++) norm_sq += x[i] * x[i];
+  elem norm_x = ORC_SQRT(norm_sq);
+  if (norm_x < ORC_EPS) {
+    *tau = 0;
+    for (size_t i = 0; i < n; i++) v[i] = 0;
+    return;
+  }
+  elem x0 = x[0];
+  elem beta = (x0 >= 0) ? -norm_x : norm_x;
+  elem v0 = x0 - beta;
+  elem v0_inv = 1 / v0;
+  v[0] = 1;
+  for (size_t i = 1; i < n; i++) v[i] = x[i] * v0_inv;
+  *tau = -v0 / beta;
+}
+
+#define QR_NB 16        /* MAT_QR_BLOCK_SIZE, mat.h:3762 */
+#define QR_THRESHOLD 64 /* MAT_QR_BLOCK_THRESHOLD, mat.h:3766 */
+
+/* Compact-WY factor T of k reflectors (mat__qr_build_T_, mat.h:3772-3827):
+ * YtY(p,j) = Y(:,p).Y(:,j) for p < j, T(j,j) = tau_j,
+ * T(i,j) = -tau_j * sum_{p=i}^{j-1} T(i,p) YtY(p,j). */
+static void build_T(elem *T, size_t ldt, const elem *Y, size_t ldy,
+                    const elem *tau, size_t m, size_t k) {
+  for (size_t i = 0; i < k * k; i++) T[i] = 0;
+  elem *YtY = (elem *)malloc(k * k * sizeof(elem));
+  for (size_t j = 1; j < k; j++)
+    for (size_t p = 0; p < j; p++) YtY[j * k + p] = dot(&Y[p * ldy], &Y[j * ldy], m);
+  for (size_t j = 0; j < k; j++) {
+    T[j * ldt + j] = tau[j];
+    for (size_t i = 0; i < j; i++) {
+      elem sum = 0;
+      for (size_t p = i; p < j; p++) sum += T[p * ldt + i] * YtY[j * k + p];
+      T[j * ldt + i] = -tau[j] * sum;
+    }
+  }
+  free(YtY);
+}
+
+/* Blocked and unblocked Householder QR (mat_qr, mat.h:3907-4063, with the
+ * block applies of mat.h:3846-3903; mat_qr_r, mat.h:4066-4192, when Q is
+ * NULL).  A is m x n (m >= n), Q m x m, R m x n, all column-major.  R
+ * keeps what the reflectors leave below its diagonal (rounding residue),
+ * as the reference does. */
+static void qr_impl(size_t m, size_t n, const elem *A, elem *Q, elem *R) {
+  memcpy(R, A, m * n * sizeof(elem));
+  if (Q) {
+    memset(Q, 0, m * m * sizeof(elem));
+    for (size_t i = 0; i < m; i++) Q[i * m + i] = 1;
+  }
+  if (n >= QR_THRESHOLD) {
+    const size_t nb = QR_NB;
+    elem *tau = (elem *)malloc(nb * sizeof(elem));
+    elem *Y = (elem *)malloc(m * nb * sizeof(elem));
+    elem *T = (elem *)malloc(nb * nb * sizeof(elem));
+    elem *x = (elem *)malloc(m * sizeof(elem));
+    elem *Wmk = (elem *)malloc(m * nb * sizeof(elem));
+    elem *Wmk2 = (elem *)malloc(m * nb * sizeof(elem));
+    elem *Wkn = (elem *)malloc(nb * n * sizeof(elem));
+    elem *Wkn2 = (elem *)malloc(nb * n * sizeof(elem));
+    for (size_t jb = 0; jb < n; jb += nb) {
+      const size_t kb = (jb + nb <= n) ? nb : n - jb;
+      const size_t len = m - jb;
+      for (size_t j = 0; j < kb; j++) {
+        const size_t col = jb + j, vlen = m - col;
+        memcpy(x, &R[col * m + col], vlen * sizeof(elem));
+        householder(x, &tau[j], x, vlen);
+        for (size_t i = 0; i < j; i++) Y[j * len + i] = 0;
+        for (size_t i = 0; i < vlen; i++) Y[j * len + j + i] = x[i];
+        if (tau[j] != 0) {
+          for (size_t c = col; c < jb + kb; c++) {
+            elem *rc = &R[c * m + col];
+            elem w = dot(x, rc, vlen) * tau[j];
+            axpy(rc, -w, x, vlen);
+          }
+        }
+      }
+      build_T(T, kb, Y, len, tau, len, kb);
+      if (jb + kb < n) {
+        /* R(jb:, jb+kb:) -= Y T^T (Y^T R(jb:, jb+kb:)), mat.h:3846-3869 */
+        const size_t tc = n - (jb + kb);
+        elem *Rs = &R[(jb + kb) * m + jb];
+        gemm_strided(Wkn, kb, 1, Y, len, 1, Rs, m, 0, kb, len, tc, 0);
+        gemm_strided(Wkn2, kb, 1, T, kb, 1, Wkn, kb, 0, kb, kb, tc, 0);
+        gemm_strided(Rs, m, -1, Y, len, 0, Wkn2, kb, 0, len, kb, tc, 1);
+      }
+      if (Q) {
+        /* Q(:, jb:) -= (Q(:, jb:) Y) T Y^T, mat.h:3873-3903 */
+        elem *Qt = &Q[jb * m];
+        gemm_strided(Wmk, m, 1, Qt, m, 0, Y, len, 0, m, len, kb, 0);
+        gemm_strided(Wmk2, m, 1, Wmk, m, 0, T, kb, 0, m, kb, kb, 0);
+        gemm_strided(Qt, m, -1, Wmk2, m, 0, Y, len, 1, m, kb, len, 1);
+      }
+    }
+    free(tau);
+    free(Y);
+    free(T);
+    free(x);
+    free(Wmk);
+    free(Wmk2);
+    free(Wkn);
+    free(Wkn2);
+    return;
+  }
+  elem *x = (elem *)malloc(m * sizeof(elem));
+  elem *v = (elem *)malloc(m * sizeof(elem));
+  elem *u = (elem *)malloc(m * sizeof(elem));
+  for (size_t j = 0; j < n; j++) {
+    const size_t len = m - j;
+    elem tau;
+    memcpy(x, &R[j * m + j], len * sizeof(elem));
+    householder(v, &tau, x, len);
+    if (tau == 0) continue;
+    for (size_t k = j; k < n; k++) {
+      elem *rk = &R[k * m + j];
+      elem w = tau * dot(v, rk, len);
+      axpy(rk, -w, v, len);
+    }
+    if (Q) {
+      /* Q(:, j:) = Q(:, j:) - tau (Q(:, j:) v) v^T, column by column */
+      memset(u, 0, m * sizeof(elem));
+      for (size_t jj = 0; jj < len; jj++) axpy(u, v[jj], &Q[(j + jj) * m], m);
+      for (size_t jj = 0; jj < len; jj++) axpy(&Q[(j + jj) * m], -tau * v[jj], u, m);
+    }
+  }
+  free(x);
+  free(v);
+  free(u);
+}
+
+void ORC(qr)(size_t m, size_t n, const elem *A, elem *Q, elem *R) { qr_impl(m, n, A, Q, R); }
+
+void ORC(qr_r)(size_t m, size_t n, const elem *A, elem *R) { qr_impl(m, n, A, NULL, R); }

@@ -69,6 +69,8 @@ class Oracle:
         f("solve_spd", I, [sz, P, P, P])
         f("solve_multi", None, [sz, sz, P, P, P])
         f("det", self.ct, [sz, P])
+        f("qr", None, [sz, sz, P, P, P])
+        f("qr_r", None, [sz, sz, P, P])
 
     def _fn(self, name, res, args):
         fn = getattr(self.lib, self.pre + name)
@@ -102,6 +104,26 @@ class Oracle:
         self._syrk(n, k, alpha, self._p(A), beta, self._p(out), int(uplo in ("U", "u")),
                    int(bool(trans)))
         return out
+
+    def qr(self, A):
+        """(Q, R) of the m x n A, m >= n (mat_qr); R keeps the reflectors'
+        rounding residue below its diagonal, as libmat leaves it."""
+        A = self._cm(A)
+        m, n = A.shape
+        assert m >= n
+        Q = np.zeros((m, m), self.dtype, order="F")
+        R = np.zeros((m, n), self.dtype, order="F")
+        self._qr(m, n, self._p(A), self._p(Q), self._p(R))
+        return Q, R
+
+    def qr_r(self, A):
+        """R of the m x n A, m >= n (mat_qr_r)."""
+        A = self._cm(A)
+        m, n = A.shape
+        assert m >= n
+        R = np.zeros((m, n), self.dtype, order="F")
+        self._qr_r(m, n, self._p(A), self._p(R))
+        return R
 
     def mul(self, A, B):
         A = self._cm(A)

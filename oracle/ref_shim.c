@@ -121,3 +121,13 @@ elem REF(det)(size_t n, const elem *A) {
   Mat a = wrap(n, n, A);
   return mat_det(&a);
 }
+
+void REF(qr)(size_t m, size_t n, const elem *A, elem *Q, elem *R) {
+  Mat a = wrap(m, n, A), q = wrap(m, m, Q), r = wrap(m, n, R);
+  mat_qr(&a, &q, &r);
+}
+
+void REF(qr_r)(size_t m, size_t n, const elem *A, elem *R) {
+  Mat a = wrap(m, n, A), r = wrap(m, n, R);
+  mat_qr_r(&a, &r);
+}

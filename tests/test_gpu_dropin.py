@@ -240,3 +240,36 @@ def test_one_by_one(gpu, prec):
             assert np.allclose(x, o.solve(A, [1.5]), rtol=4 * EPS[prec])
     C = g.mat_mul([[3.0]], [[-2.0]])
     assert C[0, 0] == -6.0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", PRECS)
+def test_qr_against_oracle(gpu, prec):
+    """mat_qr / mat_qr_r (mat.h:3907-4192) against the oracle: Q and R to
+    8 kappa(A) sqrt(m) eps in the Frobenius norm (both are backward-stable
+    Householder QRs with the same sign rule; the reductions differ in
+    order), Q orthogonal, A = Q triu(R), the residue below R's diagonal
+    within the reference test's m * eps bound; mat_qr_r's R bit for bit
+    that of mat_qr.  Shapes: the reference's unblocked sizes (n < 64), one
+    and several 32-column blocks with a ragged last block, tall and
+    square; one case with a zero column (no reflector)."""
+    from oracle.oracle import Oracle
+    g, o = gpu[prec], Oracle(prec)
+    eps = EPS[prec]
+    rng = np.random.default_rng(31)
+    for m, n in ((1, 1), (3, 3), (4, 3), (10, 7), (63, 63), (64, 64), (100, 70), (130, 129),
+                 (300, 200), (520, 300), (1030, 512)):
+        A = rng.uniform(-1, 1, (m, n)).astype(DT[prec])
+        if n > 40:
+            A[:, 37] = 0
+        Q, R = g.mat_qr(A)
+        Qo, Ro = o.qr(A)
+        kappa = np.linalg.cond(A[:, np.abs(A).sum(0) > 0].astype(np.float64))
+        tol = 8 * kappa * np.sqrt(m) * eps
+        assert rel_fro(R, Ro) <= tol, (m, n, rel_fro(R, Ro), tol)
+        assert rel_fro(Q, Qo) <= tol, (m, n, rel_fro(Q, Qo), tol)
+        assert np.abs(Q.T.astype(np.float64) @ Q - np.eye(m)).max() <= 20 * m * eps, (m, n)
+        recon = Q.astype(np.float64) @ np.triu(R).astype(np.float64)
+        assert np.abs(recon - A).max() <= 20 * m * eps, (m, n)
+        assert np.abs(np.tril(R, -1)).max() <= m * (1e-6 if prec == "f32" else 1e-9), (m, n)
+        assert np.array_equal(g.mat_qr_r(A), R), (m, n)

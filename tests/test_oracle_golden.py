@@ -221,3 +221,53 @@ def test_syrk_brute_force_and_known_answer(orc, uplo, trans):
     expect = np.where(np.tril(np.ones((2, 2), bool)) if uplo == "L"
                       else np.triu(np.ones((2, 2), bool)), want, keep)
     assert np.array_equal(got, expect)
+
+
+def _qr_invariants(Q, R, A, eps):
+    m, n = A.shape
+    assert np.abs(Q.T @ Q - np.eye(m)).max() <= 20 * m * eps
+    assert np.abs(Q @ np.triu(R) - A).max() <= 20 * m * eps * max(1.0, np.abs(A).max())
+    # the reference's own is_upper_triangular tolerance (rows * eps_default)
+    assert np.abs(np.tril(R, -1)).max() <= m * (1e-6 if eps > 1e-10 else 1e-9) * max(
+        1.0, np.abs(A).max())
+
+
+def test_qr_known_answers_and_lapack(orc):
+    """The oracle's QR against what fixes it independently:
+    - the reference's known-answer inputs (tests/test_mat_qr.c:32-120): the
+      textbook 3x3 (|R| = [[14,21,14],[0,175,70],[0,0,35]]), the 4x3
+      overdetermined case, the identity;
+    - LAPACK's Householder QR (numpy), which uses the same sign rule
+      (beta = -sign(x0) ||x||), so R and the full Q agree to rounding for
+      both paths (unblocked n < 64 and blocked n >= 64);
+    - the identity, where libmat reflects every column (Q = R = -I) and
+      LAPACK does not (its tau is 0 when the tail is zero; the same rule
+      flips the last row of R of a square matrix)."""
+    dt = orc.dtype
+    eps = EPS[orc.prec]
+    A = np.array([[12, -51, 4], [6, 167, -68], [-4, 24, -41]], dt)
+    Q, R = orc.qr(A)
+    _qr_invariants(Q, R, A, eps)
+    want = np.array([[14, 21, 14], [0, 175, 70], [0, 0, 35]], np.float64)
+    assert np.abs(np.abs(np.triu(R)) - want).max() <= 1e3 * eps
+    A = np.arange(1, 13, dtype=dt).reshape(4, 3)
+    Q, R = orc.qr(A)
+    _qr_invariants(Q, R, A, eps)
+    Q, R = orc.qr(np.eye(3, dtype=dt))
+    assert np.array_equal(Q, -np.eye(3, dtype=dt)) and np.array_equal(np.triu(R), -np.eye(3, dtype=dt))
+    rng = np.random.default_rng(21)
+    for m, n in ((5, 5), (40, 25), (64, 64), (150, 100), (130, 130)):
+        A = rng.uniform(-1, 1, (m, n)).astype(dt)
+        Q, R = orc.qr(A)
+        _qr_invariants(Q, R, A, eps)
+        Ql, Rl = np.linalg.qr(A.astype(np.float64), mode="complete")
+        if m == n:
+            # a square matrix's last column is a one-element reflector:
+            # libmat reflects it (a sign flip), LAPACK takes tau = 0
+            Rl[-1, :] *= -1
+            Ql[:, -1] *= -1
+        kappa = np.linalg.cond(A.astype(np.float64))
+        tol = 8 * kappa * np.sqrt(m) * eps
+        assert rel_fro(np.triu(R), Rl) <= tol, (m, n)
+        assert rel_fro(Q, Ql) <= tol, (m, n)
+        assert np.array_equal(orc.qr_r(A), R), (m, n)

@@ -54,3 +54,21 @@ def test_syrk_bit_exact(prec, uplo, trans):
         for alpha, beta in ((1.0, 0.0), (-0.75, 1.5)):
             assert np.array_equal(o.syrk(alpha, A, beta, Cm, uplo, trans),
                                   r.syrk(alpha, A, beta, Cm, uplo, trans)), (n, k)
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_qr_bit_exact(prec):
+    """mat_qr / mat_qr_r (mat.h:3907-4192): the unblocked path (n < 64) and
+    the blocked compact-WY path (n >= 64, blocks of 16, a ragged last
+    block), tall and square, and a zero column (no reflector)."""
+    o, r = Oracle(prec), Oracle(prec, "ref")
+    rng = np.random.default_rng(12)
+    for m, n in ((1, 1), (3, 3), (4, 3), (10, 7), (63, 63), (64, 64), (100, 70), (97, 96),
+                 (150, 130)):
+        A = rng.uniform(-1, 1, (m, n)).astype(o.dtype)
+        if n > 5:
+            A[:, 5] = 0
+        Qo, Ro = o.qr(A)
+        Qr, Rr = r.qr(A)
+        assert np.array_equal(Qo, Qr) and np.array_equal(Ro, Rr), (m, n)
+        assert np.array_equal(o.qr_r(A), r.qr_r(A)), (m, n)
