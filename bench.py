@@ -57,10 +57,11 @@ sys.path.insert(0, ROOT)
 # MI355X published dense peaks
 PEAK = {"bf16": 2500.0, "fp8": 5000.0, "fp32": 157.3, "fp64": 78.6, "hbm": 8000.0}
 FAMILIES = ["panel", "rowswap", "gemm", "gemm_bf16x", "gemm_fp8", "pack", "trsm", "tri_chain",
-            "chol_leaf"]
+            "chol_leaf", "qr_panel"]
 # family -> (bound, unit, peak key for the given dtype)
 BOUND = {"panel": ("hbm", "GB/s"), "rowswap": ("hbm", "GB/s"), "pack": ("hbm", "GB/s"),
          "tri_chain": ("hbm", "GB/s"), "chol_leaf": ("hbm", "GB/s"), "trsm": ("hbm", "GB/s"),
+         "qr_panel": ("hbm", "GB/s"),
          "gemm": ("mfma", "TFLOP/s"), "gemm_bf16x": ("mfma", "TFLOP/s"),
          "gemm_fp8": ("mfma", "TFLOP/s")}
 
@@ -200,6 +201,15 @@ def cpu_chol(n, nrhs, prec, budget_s=12.0):
     return {"value": round(v, 4), "unit": "GFLOP/s", "cores": 1, "kind": kind,
             "sample": f"mat_chol + {nrhs}x(mat_solve_tril, mat_solve_trilt), {prec}, n={n}, "
                       f"median of {runs} runs ({t:.2f} s each), single thread"}
+
+
+def cpu_qr(n, prec, budget_s=12.0):
+    o, kind = _oracle(prec)
+    A = np.random.default_rng(0).uniform(-1, 1, (n, n)).astype(o.dtype)
+    v, t, runs = _cpu_loop(lambda: o.qr(A), 8.0 * n ** 3 / 3.0, budget_s, max_runs=200)
+    return {"value": round(v, 4), "unit": "GFLOP/s", "cores": 1, "kind": kind,
+            "sample": f"mat_qr (R and Q), {prec}, n={n}, median of {runs} runs ({t:.2f} s each), "
+                      "single thread"}
 
 
 # ------------------------------------------------------------ mixed-lu (default)
@@ -513,6 +523,64 @@ def run_chol(torch, libmat_amd, args):
     return out
 
 
+# ------------------------------------------------------------ qr
+def run_qr(torch, libmat_amd, args):
+    """mat_qr (A = Q R, R and the full Q) of an n x n fp32 matrix held in HBM,
+    through the device entry point mat_gpu_geqrf_f32 (the kernels mat_qr
+    runs).  A step restores A from a resident copy (2 n^2 x 4 bytes), then
+    factors it."""
+    g = libmat_amd.load("f32")
+    lib = g.lib
+    n = args.n
+    gen = torch.Generator(device="cuda").manual_seed(13)
+    A0 = (torch.rand(n, n, device="cuda", generator=gen) * 2 - 1).contiguous()
+    A = torch.empty_like(A0)
+    Q = torch.empty_like(A0)
+    s = torch.cuda.current_stream().cuda_stream
+
+    def step(q=True):
+        A.copy_(A0)
+        g.check(lib.mat_gpu_geqrf_f32(s, n, n, ptr(A), n, ptr(Q) if q else None), "geqrf")
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    g.check(lib.mat_gpu_timing_enable(1), "timing")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    g.check(lib.mat_gpu_timing_enable(0), "timing")
+    table = kernel_table(lib)
+    # column-major buffers: the factored matrix is A0^T, Q and R likewise
+    M = A0.double().T
+    Qm, Rm = Q.double().T, torch.triu(A.double().T)
+    recon = float((Qm @ Rm - M).norm() / M.norm())
+    orth = float((Qm.T @ Qm - torch.eye(n, dtype=torch.float64, device="cuda")).abs().max())
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        step(q=False)
+    torch.cuda.synchronize()
+    r_ms = (time.perf_counter() - t1) / args.steps * 1e3
+    flops = 8.0 * n ** 3 / 3.0
+    value = flops * args.steps / elapsed / 1e9
+    out = {"metric": "fp32 Householder QR GFLOP/s (mat_qr: R and Q)", "value": round(value, 2),
+           "unit": "GFLOP/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": round(elapsed / args.steps * 1e3, 3), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic U[-1,1] square matrix",
+           "config": {"workload": "fp32 Householder QR (mat_qr, full Q)", "n": n,
+                      "flops": "8/3 n^3 (R 4/3 n^3 + Q 4/3 n^3)", "parallelism": "single GPU"},
+           "reconstruction_error": recon, "orthogonality_error": orth,
+           "qr_r_ms": round(r_ms, 3), "qr_r_gflops": round(4.0 * n ** 3 / 3.0 / r_ms / 1e6, 2),
+           "kernels": breakdown(table, args.steps),
+           "roofline": roofline(table, dominant(table), "fp32", args.steps, args.profiled)}
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_qr(min(args.cpu_n, 512), "f32")
+    return out
+
+
 # ------------------------------------------------------------ batched
 def make_batch(count, seed=5, lo=3, hi=64):
     rng = np.random.default_rng(seed)
@@ -691,7 +759,7 @@ def main():
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="mixed-lu",
-                    choices=["mixed-lu", "mixed-spd", "gemm", "chol", "batched"])
+                    choices=["mixed-lu", "mixed-spd", "gemm", "chol", "batched", "qr"])
     ap.add_argument("--n", type=int, default=None)
     ap.add_argument("--nrhs", type=int, default=None)
     ap.add_argument("--kappas", default=None)
@@ -713,7 +781,8 @@ def main():
               "multi-GPU layout is a design note, not a measured path", file=sys.stderr)
         sys.exit(2)
     defaults = {"mixed-lu": (16384, 16, "1e2,1e4,1e6", 2), "mixed-spd": (32768, 4, "1e2,1e3", 1),
-                "gemm": (8192, 0, None, 10), "chol": (8192, 8, None, 5), "batched": (0, 1, None, 5)}
+                "gemm": (8192, 0, None, 10), "chol": (8192, 8, None, 5), "batched": (0, 1, None, 5),
+                "qr": (4096, 0, None, 3)}
     dn, dr, dk, ds = defaults[args.workload]
     args.n = args.n or dn
     args.nrhs = args.nrhs or dr
@@ -721,7 +790,7 @@ def main():
     args.steps = args.steps or ds
     # workload name of the committed HBM-traffic passes, when run at their sizes
     tags = {"mixed-lu": "mixed_lu_n16384", "mixed-spd": "mixed_spd_n32768", "gemm": "gemm_8192",
-            "chol": "chol_8192", "batched": "batched_200k"}
+            "chol": "chol_8192", "batched": "batched_200k", "qr": "qr_4096"}
     same = (args.n == dn and args.nrhs == dr and
             (args.workload != "batched" or args.count == 200000))
     args.profiled = tags[args.workload] if same else None
@@ -738,6 +807,8 @@ def main():
         out = run_gemm(torch, libmat_amd, args)
     elif args.workload == "chol":
         out = run_chol(torch, libmat_amd, args)
+    elif args.workload == "qr":
+        out = run_qr(torch, libmat_amd, args)
     else:
         out = run_batched(torch, libmat_amd, args)
     print(json.dumps(out), flush=True)

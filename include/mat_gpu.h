@@ -194,6 +194,11 @@ int mat_gpu_getrs_f64(void *stream, size_t n, size_t nrhs, const double *lu, siz
  * Synchronous. */
 int mat_gpu_potrf_f32(void *stream, size_t n, float *a, size_t lda);
 int mat_gpu_potrf_f64(void *stream, size_t n, double *a, size_t lda);
+/* Householder QR in place on the column-major m x n a (m >= n): a becomes
+ * mat_qr's R; q (m x m, ld m, device) receives Q unless null.  The device
+ * form of mat_qr / mat_qr_r (mat.h:3907-4192).  Synchronous. */
+int mat_gpu_geqrf_f32(void *stream, size_t m, size_t n, float *a, size_t lda, float *q);
+int mat_gpu_geqrf_f64(void *stream, size_t m, size_t n, double *a, size_t lda, double *q);
 /* X = (L L^T)^-1 B from mat_gpu_potrf's factor. */
 int mat_gpu_potrs_f32(void *stream, size_t n, size_t nrhs, const float *l, size_t ldl,
                       const float *b, size_t ldb, float *x, size_t ldx);

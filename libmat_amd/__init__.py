@@ -39,6 +39,7 @@ EXT_SYMBOLS = [
     "mat_gpu_gemm_f32", "mat_gpu_gemm_f64", "mat_gpu_gemm_lowp", "mat_gpu_getrf_f32", "mat_gpu_getrf_f64",
     "mat_gpu_getrf_tournament_f32",
     "mat_gpu_getrs_f32", "mat_gpu_getrs_f64", "mat_gpu_potrf_f32", "mat_gpu_potrf_f64",
+    "mat_gpu_geqrf_f32", "mat_gpu_geqrf_f64",
     "mat_gpu_potrs_f32", "mat_gpu_potrs_f64", "mat_gpu_gesv_f32", "mat_gpu_gesv_f64",
     "mat_gpu_posv_f32", "mat_gpu_posv_f64", "mat_gpu_ir_defaults", "mat_gpu_gesv_mixed",
     "mat_gpu_posv_mixed", "mat_gpu_batch_create_f32", "mat_gpu_batch_run",
@@ -124,6 +125,7 @@ class GpuLib:
             s(f"mat_gpu_getrf_{t}", i32, [vp, sz, vp, sz, vp, i32, C.POINTER(i32)])
             s(f"mat_gpu_getrs_{t}", i32, [vp, sz, sz, vp, sz, vp, vp, sz, vp, sz])
             s(f"mat_gpu_potrf_{t}", i32, [vp, sz, vp, sz])
+            s(f"mat_gpu_geqrf_{t}", i32, [vp, sz, sz, vp, sz, vp])
             s(f"mat_gpu_potrs_{t}", i32, [vp, sz, sz, vp, sz, vp, sz, vp, sz])
             s(f"mat_gpu_gesv_{t}", i32, [vp, sz, sz, vp, sz, vp, sz, vp, sz])
             s(f"mat_gpu_posv_{t}", i32, [vp, sz, sz, vp, sz, vp, sz, vp, sz])

@@ -123,6 +123,20 @@ int potrf_ext(void *stream, size_t n, T *a, size_t lda) {
 }
 
 template <typename T>
+int geqrf_ext(void *stream, size_t m, size_t n, T *a, size_t lda, T *q) {
+  return guarded([&] {
+    check_dims(m);
+    check_dims(n);
+    MG_CHECK(m >= n, "geqrf: needs m >= n");
+    MG_CHECK(lda >= m, "geqrf: lda below m");
+    hipStream_t s = as_stream(stream);
+    geqrf<T>(s, (int)m, (int)n, a, (long)lda, q, sizeof(T) == 4 ? (T)1e-6f : (T)1e-9);
+    sync_and_check(s);
+    return 0;
+  });
+}
+
+template <typename T>
 int potrs_ext(void *stream, size_t n, size_t nrhs, const T *l, size_t ldl, const T *b, size_t ldb,
               T *x, size_t ldx) {
   return guarded([&] {
@@ -241,6 +255,12 @@ int mat_gpu_getrs_f64(void *s, size_t n, size_t nrhs, const double *lu, size_t l
 int mat_gpu_potrf_f32(void *s, size_t n, float *a, size_t lda) { return potrf_ext<float>(s, n, a, lda); }
 int mat_gpu_potrf_f64(void *s, size_t n, double *a, size_t lda) {
   return potrf_ext<double>(s, n, a, lda);
+}
+int mat_gpu_geqrf_f32(void *s, size_t m, size_t n, float *a, size_t lda, float *q) {
+  return geqrf_ext<float>(s, m, n, a, lda, q);
+}
+int mat_gpu_geqrf_f64(void *s, size_t m, size_t n, double *a, size_t lda, double *q) {
+  return geqrf_ext<double>(s, m, n, a, lda, q);
 }
 int mat_gpu_potrs_f32(void *s, size_t n, size_t nrhs, const float *l, size_t ldl, const float *b,
                       size_t ldb, float *x, size_t ldx) {

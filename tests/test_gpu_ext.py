@@ -239,3 +239,23 @@ def test_getrf_tall_panel(gpu, prec, leaf):
     PA = A0.view(n, n).T.double()[p]
     err = float(torch.linalg.matrix_norm(L @ U - PA) / torch.linalg.matrix_norm(PA))
     assert err <= 8 * n * EPS[prec], err
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_geqrf_device_matches_dropin(gpu, prec):
+    """mat_gpu_geqrf (device pointers) runs mat_qr's kernels: R and Q bit for
+    bit those of the drop-in mat_qr, and with q = NULL the same R (the
+    mat_qr_r form); m > n with a ragged last block of reflectors."""
+    g = gpu[prec]
+    m, n = 700, 333
+    A = np.random.default_rng(4).uniform(-1, 1, (m, n))
+    Qd, Rd = g.mat_qr(A.astype(DT[prec]))
+    fn = getattr(g.lib, f"mat_gpu_geqrf_{prec}")
+    dA = to_dev(A, DT[prec])
+    dQ = to_dev(np.zeros((m, m)), DT[prec])
+    assert fn(stream(), m, n, ptr(dA), m, ptr(dQ)) == 0
+    assert np.array_equal(from_dev(dA, m, n), Rd)
+    assert np.array_equal(from_dev(dQ, m, m), Qd)
+    dA = to_dev(A, DT[prec])
+    assert fn(stream(), m, n, ptr(dA), m, None) == 0
+    assert np.array_equal(from_dev(dA, m, n), Rd)
