@@ -16,11 +16,11 @@
 //
 // MI355X design.  Blocks of 32 reflectors (the reference uses 16; fewer,
 // deeper GEMMs).  One workgroup of 1024 threads factors a block's panel
-// column by column out of L2 (norm, reflector, then the block's remaining
-// columns, each column's dot product reduced by one wavefront); a
-// one-wavefront kernel builds T from Y^T Y; the trailing columns of R and
-// the columns of Q are updated by the library's MFMA GEMM (three products
-// each).
+// column by column out of L2 (norm, reflector, then the dot products of
+// the block's remaining columns, one wavefront each, and their update); a
+// one-wavefront kernel builds T from Y^T Y; the trailing columns
+// of R and the columns of Q are updated by the library's MFMA GEMM (three
+// products each).
 #include "ops.hpp"
 
 namespace mg {
@@ -80,22 +80,39 @@ __global__ __launch_bounds__(QR_NT) void qr_panel_kernel(int m, int jb, int kb, 
     __syncthreads();
     if (tau != 0) {
       // w_c = (v . R(col:, c)) tau for the block's columns c >= j, one
-      // wavefront per column
+      // wavefront per column, eight partial sums (eight loads in flight)
       for (int c = wave; c < kb; c += QR_NW) {
         if (c < j) continue;
         const T *rc = R + (long)(jb + c) * ldr + col;
-        T d = 0;
-        for (long i = lane; i < vlen; i += 64) d += y[j + i] * rc[i];
-        d = wave_sum(d);
-        if (lane == 0) wdot[c] = d * tau;
+        const T *yv = y + j;
+        T d[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) d[u] = 0;
+        long i = lane;
+        for (; i + 7 * 64 < vlen; i += 8 * 64) {
+#pragma unroll
+          for (int u = 0; u < 8; ++u) d[u] += yv[i + u * 64] * rc[i + u * 64];
+        }
+        for (; i < vlen; i += 64) d[0] += yv[i] * rc[i];
+        T dd = ((d[0] + d[1]) + (d[2] + d[3])) + ((d[4] + d[5]) + (d[6] + d[7]));
+        dd = wave_sum(dd);
+        if (lane == 0) wdot[c] = dd * tau;
       }
       __syncthreads();
-      // R(col:, c) += (-w_c) v
+      // R(col:, c) += (-w_c) v, eight columns at a time: the eight values
+      // of a row are loaded before any is stored
       for (long i = t; i < vlen; i += QR_NT) {
         const T vi = y[j + i];
-        for (int c = j; c < kb; ++c) {
-          T *rc = R + (long)(jb + c) * ldr + col;
-          rc[i] += -wdot[c] * vi;
+        T *p = R + (long)(jb + j) * ldr + col + i;
+        for (int c = j; c < kb; c += 8) {
+          T val[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u)
+            if (c + u < kb) val[u] = p[(long)u * ldr];
+#pragma unroll
+          for (int u = 0; u < 8; ++u)
+            if (c + u < kb) p[(long)u * ldr] = val[u] + -wdot[c + u] * vi;
+          p += 8 * ldr;
         }
       }
       __syncthreads();
