@@ -18,9 +18,12 @@
 // deeper GEMMs).  One workgroup of 1024 threads factors a block's panel
 // column by column out of L2 (norm, reflector, then the dot products of
 // the block's remaining columns, one wavefront each, and their update); a
-// one-wavefront kernel builds T from Y^T Y; the trailing columns
-// of R and the columns of Q are updated by the library's MFMA GEMM (three
-// products each).
+// one-wavefront kernel builds T from Y^T Y; the trailing columns of R and
+// the columns of Q are updated by the library's MFMA GEMM (three products
+// each).  Q's updates run on the library's side stream, so block k's Q
+// update overlaps block k+1's panel, which occupies one CU.
+#include <mutex>
+
 #include "ops.hpp"
 
 namespace mg {
@@ -156,23 +159,42 @@ void geqrf(hipStream_t s, int m, int n, T *R, long ldr, T *Q, T eps) {
   MG_CHECK(m >= n && n >= 1, "geqrf: needs m >= n >= 1");
   MG_CHECK(ldr >= m, "geqrf: leading dimension below m");
   Workspace &ws = workspace(s);
-  T *Y = (T *)ws.get("qr.Y", (size_t)m * QR_NB * sizeof(T));
+  // Y and T in two slots: with Q, block k's Q update (side stream) still
+  // reads its slot while block k+1 factors into the other one
+  T *Ys[2] = {(T *)ws.get("qr.Y0", (size_t)m * QR_NB * sizeof(T)),
+              Q ? (T *)ws.get("qr.Y1", (size_t)m * QR_NB * sizeof(T)) : nullptr};
+  T *Ts[2] = {(T *)ws.get("qr.T0", QR_NB * QR_NB * sizeof(T)),
+              Q ? (T *)ws.get("qr.T1", QR_NB * QR_NB * sizeof(T)) : nullptr};
   T *tau = (T *)ws.get("qr.tau", QR_NB * sizeof(T));
   T *G = (T *)ws.get("qr.G", QR_NB * QR_NB * sizeof(T));
-  T *Tm = (T *)ws.get("qr.T", QR_NB * QR_NB * sizeof(T));
   T *Ck = (T *)ws.get("qr.Ck", (size_t)QR_NB * n * sizeof(T));
   T *Ck2 = (T *)ws.get("qr.Ck2", (size_t)QR_NB * n * sizeof(T));
   T *Cm = Q ? (T *)ws.get("qr.Cm", (size_t)m * QR_NB * sizeof(T)) : nullptr;
   T *Cm2 = Q ? (T *)ws.get("qr.Cm2", (size_t)m * QR_NB * sizeof(T)) : nullptr;
+  const int nblk = (n + QR_NB - 1) / QR_NB;
+  // Q = I, then Q(:, jb:) -= ((Q(:, jb:) Y) T) Y^T per block, all on the
+  // side stream (events: 2k = block k's Y and T ready, 2k+1 = block k's Q
+  // update done with them)
+  std::unique_lock<std::mutex> lk;
+  hipStream_t u = nullptr;
   if (Q) {
+    lk = std::unique_lock<std::mutex>(lookahead_mutex());
+    u = side_stream();
+    hipEvent_t start = pooled_event(2 * nblk);
+    MG_HIP(hipEventRecord(start, s));
+    MG_HIP(hipStreamWaitEvent(u, start, 0));
     const long total = (long)m * m;
     const unsigned blocks = (unsigned)std::min<long>((total + 255) / 256, 8192);
-    hipLaunchKernelGGL((qr_eye_kernel<T>), dim3(blocks), dim3(256), 0, s, (long)m, Q);
+    hipLaunchKernelGGL((qr_eye_kernel<T>), dim3(blocks), dim3(256), 0, u, (long)m, Q);
     MG_HIP(hipGetLastError());
   }
-  for (int jb = 0; jb < n; jb += QR_NB) {
+  for (int k = 0; k < nblk; ++k) {
+    const int jb = k * QR_NB;
     const int kb = std::min(QR_NB, n - jb);
     const long len = (long)m - jb;
+    T *Y = Ys[Q ? (k & 1) : 0];
+    T *Tm = Ts[Q ? (k & 1) : 0];
+    if (Q && k >= 2) MG_HIP(hipStreamWaitEvent(s, pooled_event(2 * (k - 2) + 1), 0));
     {
       KScope ks(s, "qr_panel", 2.0 * len * kb * sizeof(T));
       hipLaunchKernelGGL((qr_panel_kernel<T>), dim3(1), dim3(QR_NT), 0, s, m, jb, kb, R, ldr, Y,
@@ -185,6 +207,15 @@ void geqrf(hipStream_t s, int m, int n, T *R, long ldr, T *Q, T eps) {
     hipLaunchKernelGGL((qr_tfactor_kernel<T>), dim3(1), dim3(64), 0, s, kb, G, tau, Tm);
     MG_HIP(hipGetLastError());
     const View<const T> tm{Tm, 1, kb};
+    if (Q) {
+      MG_HIP(hipEventRecord(pooled_event(2 * k), s));
+      MG_HIP(hipStreamWaitEvent(u, pooled_event(2 * k), 0));
+      T *Qt = Q + (long)jb * m;
+      gemm<T>(u, m, kb, len, T(1), View<const T>{Qt, 1, m}, y, T(0), View<T>{Cm, 1, m});
+      gemm<T>(u, m, kb, kb, T(1), View<const T>{Cm, 1, m}, tm, T(0), View<T>{Cm2, 1, m});
+      gemm<T>(u, m, len, kb, T(-1), View<const T>{Cm2, 1, m}, y.t(), T(1), View<T>{Qt, 1, m});
+      MG_HIP(hipEventRecord(pooled_event(2 * k + 1), u));
+    }
     if (jb + kb < n) {
       // R(jb:, jb+kb:) -= Y (T^T (Y^T R(jb:, jb+kb:)))
       const int tc = n - (jb + kb);
@@ -193,13 +224,11 @@ void geqrf(hipStream_t s, int m, int n, T *R, long ldr, T *Q, T eps) {
       gemm<T>(s, kb, tc, kb, T(1), tm.t(), View<const T>{Ck, 1, kb}, T(0), View<T>{Ck2, 1, kb});
       gemm<T>(s, len, tc, kb, T(-1), y, View<const T>{Ck2, 1, kb}, T(1), View<T>{Rs, 1, ldr});
     }
-    if (Q) {
-      // Q(:, jb:) -= ((Q(:, jb:) Y) T) Y^T
-      T *Qt = Q + (long)jb * m;
-      gemm<T>(s, m, kb, len, T(1), View<const T>{Qt, 1, m}, y, T(0), View<T>{Cm, 1, m});
-      gemm<T>(s, m, kb, kb, T(1), View<const T>{Cm, 1, m}, tm, T(0), View<T>{Cm2, 1, m});
-      gemm<T>(s, m, len, kb, T(-1), View<const T>{Cm2, 1, m}, y.t(), T(1), View<T>{Qt, 1, m});
-    }
+  }
+  if (Q) {
+    hipEvent_t done = pooled_event(2 * nblk + 1);
+    MG_HIP(hipEventRecord(done, u));
+    MG_HIP(hipStreamWaitEvent(s, done, 0));
   }
 }
 
