@@ -125,20 +125,26 @@ __global__ __launch_bounds__(QR_NT) void qr_panel_kernel(int m, int jb, int kb, 
 
 // T (kb x kb, upper, ld kb) of the compact-WY form from G = Y^T Y (ld kb)
 // and tau: T(j,j) = tau_j, T(i,j) = -tau_j sum_{p=i}^{j-1} T(i,p) G(p,j)
-// (mat.h:3810-3823).  Lane i builds row i, which only reads its own row.
+// (mat.h:3810-3823).  G and tau are staged in LDS first (all loads in
+// flight at once); lane i then builds row i, which only reads its own row.
 template <typename T>
 __global__ __launch_bounds__(64) void qr_tfactor_kernel(int kb, const T *G, const T *tau, T *Tm) {
   __shared__ T Ts[QR_NB][QR_NB + 1];
+  __shared__ T Gs[QR_NB][QR_NB + 1];
+  __shared__ T taus[QR_NB];
   const int i = threadIdx.x;
+  for (int x = i; x < kb * kb; x += 64) Gs[x % kb][x / kb] = G[x];
+  if (i < kb) taus[i] = tau[i];
+  __syncthreads();
   if (i >= kb) return;
   for (int c = 0; c < kb; ++c) Ts[i][c] = 0;
   for (int j = 0; j < kb; ++j) {
     if (i < j) {
       T sum = 0;
-      for (int p = i; p < j; ++p) sum += Ts[i][p] * G[p + (long)j * kb];
-      Ts[i][j] = -tau[j] * sum;
+      for (int p = i; p < j; ++p) sum += Ts[i][p] * Gs[p][j];
+      Ts[i][j] = -taus[j] * sum;
     } else if (i == j) {
-      Ts[i][j] = tau[j];
+      Ts[i][j] = taus[j];
     }
   }
   for (int c = 0; c < kb; ++c) Tm[i + (long)c * kb] = Ts[i][c];
