@@ -20,8 +20,10 @@
 // the block's remaining columns, one wavefront each, and their update); a
 // one-wavefront kernel builds T from Y^T Y; the trailing columns of R and
 // the columns of Q are updated by the library's MFMA GEMM (three products
-// each).  Q's updates run on the library's side stream, so block k's Q
-// update overlaps block k+1's panel, which occupies one CU.
+// each).  Look-ahead: the caller's stream applies a block only to the next
+// block's columns and factors that panel, while the library's side stream
+// applies it to the rest of R and to Q, so the wide updates overlap the
+// panel, which occupies one CU.
 #include <mutex>
 
 #include "ops.hpp"
@@ -160,47 +162,60 @@ __global__ void qr_eye_kernel(long m, T *Q) {
 
 }  // namespace
 
+// Schedule (k = block of 32 reflectors).  Caller's stream: panel k, then
+// T_k, then block k applied to block k+1's columns only, then panel k+1.
+// Side stream, in order: block k applied to every column right of block
+// k+1, then to Q.  The two never touch the same columns at once; a block's
+// columns receive the blocks before it in order (the caller's update of
+// block k+1's columns waits for the side stream's update of block k-1);
+// Y and T alternate between two slots, and panel k+2 waits until the side
+// stream is done with block k's slot.
 template <typename T>
 void geqrf(hipStream_t s, int m, int n, T *R, long ldr, T *Q, T eps) {
   MG_CHECK(m >= n && n >= 1, "geqrf: needs m >= n >= 1");
   MG_CHECK(ldr >= m, "geqrf: leading dimension below m");
   Workspace &ws = workspace(s);
-  // Y and T in two slots: with Q, block k's Q update (side stream) still
-  // reads its slot while block k+1 factors into the other one
   T *Ys[2] = {(T *)ws.get("qr.Y0", (size_t)m * QR_NB * sizeof(T)),
-              Q ? (T *)ws.get("qr.Y1", (size_t)m * QR_NB * sizeof(T)) : nullptr};
+              (T *)ws.get("qr.Y1", (size_t)m * QR_NB * sizeof(T))};
   T *Ts[2] = {(T *)ws.get("qr.T0", QR_NB * QR_NB * sizeof(T)),
-              Q ? (T *)ws.get("qr.T1", QR_NB * QR_NB * sizeof(T)) : nullptr};
+              (T *)ws.get("qr.T1", QR_NB * QR_NB * sizeof(T))};
   T *tau = (T *)ws.get("qr.tau", QR_NB * sizeof(T));
   T *G = (T *)ws.get("qr.G", QR_NB * QR_NB * sizeof(T));
-  T *Ck = (T *)ws.get("qr.Ck", (size_t)QR_NB * n * sizeof(T));
+  T *Cn = (T *)ws.get("qr.Cn", QR_NB * QR_NB * sizeof(T));    // caller's stream
+  T *Cn2 = (T *)ws.get("qr.Cn2", QR_NB * QR_NB * sizeof(T));
+  T *Ck = (T *)ws.get("qr.Ck", (size_t)QR_NB * n * sizeof(T));  // side stream
   T *Ck2 = (T *)ws.get("qr.Ck2", (size_t)QR_NB * n * sizeof(T));
   T *Cm = Q ? (T *)ws.get("qr.Cm", (size_t)m * QR_NB * sizeof(T)) : nullptr;
   T *Cm2 = Q ? (T *)ws.get("qr.Cm2", (size_t)m * QR_NB * sizeof(T)) : nullptr;
   const int nblk = (n + QR_NB - 1) / QR_NB;
-  // Q = I, then Q(:, jb:) -= ((Q(:, jb:) Y) T) Y^T per block, all on the
-  // side stream (events: 2k = block k's Y and T ready, 2k+1 = block k's Q
-  // update done with them)
-  std::unique_lock<std::mutex> lk;
-  hipStream_t u = nullptr;
+  // events: 3k = Y_k and T_k ready; 3k+1 = side stream done with block k's
+  // columns; 3k+2 = side stream done with block k (its slot is free)
+  std::lock_guard<std::mutex> lk(lookahead_mutex());
+  hipStream_t u = side_stream();
+  hipEvent_t start = pooled_event(3 * nblk);
+  MG_HIP(hipEventRecord(start, s));
+  MG_HIP(hipStreamWaitEvent(u, start, 0));
   if (Q) {
-    lk = std::unique_lock<std::mutex>(lookahead_mutex());
-    u = side_stream();
-    hipEvent_t start = pooled_event(2 * nblk);
-    MG_HIP(hipEventRecord(start, s));
-    MG_HIP(hipStreamWaitEvent(u, start, 0));
     const long total = (long)m * m;
     const unsigned blocks = (unsigned)std::min<long>((total + 255) / 256, 8192);
     hipLaunchKernelGGL((qr_eye_kernel<T>), dim3(blocks), dim3(256), 0, u, (long)m, Q);
     MG_HIP(hipGetLastError());
   }
+  // R(jb:, c0:c0+w) -= Y (T^T (Y^T R(jb:, c0:c0+w)))
+  auto apply_r = [&](hipStream_t st, const View<const T> &y, const View<const T> &tm, int kb,
+                     long len, int jb, int c0, int w, T *C1, T *C2) {
+    T *Rs = R + (long)c0 * ldr + jb;
+    gemm<T>(st, kb, w, len, T(1), y.t(), View<const T>{Rs, 1, ldr}, T(0), View<T>{C1, 1, kb});
+    gemm<T>(st, kb, w, kb, T(1), tm.t(), View<const T>{C1, 1, kb}, T(0), View<T>{C2, 1, kb});
+    gemm<T>(st, len, w, kb, T(-1), y, View<const T>{C2, 1, kb}, T(1), View<T>{Rs, 1, ldr});
+  };
   for (int k = 0; k < nblk; ++k) {
     const int jb = k * QR_NB;
     const int kb = std::min(QR_NB, n - jb);
     const long len = (long)m - jb;
-    T *Y = Ys[Q ? (k & 1) : 0];
-    T *Tm = Ts[Q ? (k & 1) : 0];
-    if (Q && k >= 2) MG_HIP(hipStreamWaitEvent(s, pooled_event(2 * (k - 2) + 1), 0));
+    T *Y = Ys[k & 1];
+    T *Tm = Ts[k & 1];
+    if (k >= 2) MG_HIP(hipStreamWaitEvent(s, pooled_event(3 * (k - 2) + 2), 0));
     {
       KScope ks(s, "qr_panel", 2.0 * len * kb * sizeof(T));
       hipLaunchKernelGGL((qr_panel_kernel<T>), dim3(1), dim3(QR_NT), 0, s, m, jb, kb, R, ldr, Y,
@@ -213,29 +228,32 @@ void geqrf(hipStream_t s, int m, int n, T *R, long ldr, T *Q, T eps) {
     hipLaunchKernelGGL((qr_tfactor_kernel<T>), dim3(1), dim3(64), 0, s, kb, G, tau, Tm);
     MG_HIP(hipGetLastError());
     const View<const T> tm{Tm, 1, kb};
+    MG_HIP(hipEventRecord(pooled_event(3 * k), s));
+    const int c1 = jb + kb;                      // block k+1's first column
+    const int w1 = std::min(QR_NB, n - c1);      // its width (<= 0: none)
+    const int c2 = c1 + std::max(w1, 0);         // the rest
+    // side stream: block k on the columns right of block k+1, then on Q
+    MG_HIP(hipStreamWaitEvent(u, pooled_event(3 * k), 0));
+    if (c2 < n) apply_r(u, y, tm, kb, len, jb, c2, n - c2, Ck, Ck2);
+    MG_HIP(hipEventRecord(pooled_event(3 * k + 1), u));
     if (Q) {
-      MG_HIP(hipEventRecord(pooled_event(2 * k), s));
-      MG_HIP(hipStreamWaitEvent(u, pooled_event(2 * k), 0));
+      // Q(:, jb:) -= ((Q(:, jb:) Y) T) Y^T
       T *Qt = Q + (long)jb * m;
       gemm<T>(u, m, kb, len, T(1), View<const T>{Qt, 1, m}, y, T(0), View<T>{Cm, 1, m});
       gemm<T>(u, m, kb, kb, T(1), View<const T>{Cm, 1, m}, tm, T(0), View<T>{Cm2, 1, m});
       gemm<T>(u, m, len, kb, T(-1), View<const T>{Cm2, 1, m}, y.t(), T(1), View<T>{Qt, 1, m});
-      MG_HIP(hipEventRecord(pooled_event(2 * k + 1), u));
     }
-    if (jb + kb < n) {
-      // R(jb:, jb+kb:) -= Y (T^T (Y^T R(jb:, jb+kb:)))
-      const int tc = n - (jb + kb);
-      T *Rs = R + (long)(jb + kb) * ldr + jb;
-      gemm<T>(s, kb, tc, len, T(1), y.t(), View<const T>{Rs, 1, ldr}, T(0), View<T>{Ck, 1, kb});
-      gemm<T>(s, kb, tc, kb, T(1), tm.t(), View<const T>{Ck, 1, kb}, T(0), View<T>{Ck2, 1, kb});
-      gemm<T>(s, len, tc, kb, T(-1), y, View<const T>{Ck2, 1, kb}, T(1), View<T>{Rs, 1, ldr});
+    MG_HIP(hipEventRecord(pooled_event(3 * k + 2), u));
+    // caller's stream: block k on block k+1's columns, after block k-1's
+    // update of them (side stream) is done
+    if (w1 > 0) {
+      if (k >= 1) MG_HIP(hipStreamWaitEvent(s, pooled_event(3 * (k - 1) + 1), 0));
+      apply_r(s, y, tm, kb, len, jb, c1, w1, Cn, Cn2);
     }
   }
-  if (Q) {
-    hipEvent_t done = pooled_event(2 * nblk + 1);
-    MG_HIP(hipEventRecord(done, u));
-    MG_HIP(hipStreamWaitEvent(s, done, 0));
-  }
+  hipEvent_t done = pooled_event(3 * nblk + 1);
+  MG_HIP(hipEventRecord(done, u));
+  MG_HIP(hipStreamWaitEvent(s, done, 0));
 }
 
 template void geqrf<float>(hipStream_t, int, int, float *, long, float *, float);
