@@ -31,6 +31,7 @@
 // every later row move of this leaf reuses.  Every poll is bounded and
 // reports a timeout instead of hanging.
 #include <algorithm>
+#include <atomic>
 #include <cstdint>
 
 #include "ops.hpp"
@@ -699,7 +700,10 @@ template <typename T, int WMAX>
 static void lu_leaf_w(hipStream_t s, int n, T *W, long ld, int c0, int w, int *ipiv, int *pairs,
                       T eps) {
   constexpr int REC = 4 + WMAX * Words<T>::N;
-  static unsigned epoch = 0;  // tags of the exchange words; never 0
+  // tags of the exchange words: distinct per launch, never 0, below 2^25
+  // (tag = epoch * 64 + step + 1 stays within 31 bits); atomic so that
+  // host threads factoring on different streams draw distinct values
+  static std::atomic<unsigned> epoch{0};
   Workspace &ws = workspace(s);
   const int m = n - c0;
   PanelP<T> p{};
@@ -719,8 +723,7 @@ static void lu_leaf_w(hipStream_t s, int n, T *W, long ld, int c0, int w, int *i
   p.krec = (uint64_t *)ws.get("panel.krec", 2L * 64 * Words<T>::N * 8);
   (void)REC;
   p.err = sync_error_word(s);
-  p.epoch = ++epoch;
-  if (p.epoch >= (1u << 25)) epoch = 0, p.epoch = ++epoch;
+  p.epoch = epoch.fetch_add(1, std::memory_order_relaxed) % ((1u << 25) - 1) + 1;
   // one row per thread where 64 workgroups suffice (a column step then
   // costs ~40% less than with two rows per thread), else two
   const int rpt = ceil_div(m, PanelNT<T>::value) > maxwg ? 2 : 1;

@@ -31,6 +31,7 @@
 // matrix before all have read it: step 3 reads the published copies, and
 // the rows a workgroup overwrites in step 4 are its own or the window's.
 #include <algorithm>
+#include <atomic>
 #include <cstdint>
 
 #include "ops.hpp"
@@ -626,7 +627,9 @@ void tslu_leaf(hipStream_t s, int n, float *W, long ld, int c0, int w, int *ipiv
   MG_CHECK(w >= 1 && w <= TS_W, "tslu_leaf: leaf width must be in [1, 32]");
   const int m = n - c0;
   MG_CHECK(m >= w && m <= tslu_max_rows(), "tslu_leaf: panel height out of range");
-  static unsigned epoch = 0;  // tags; never 0
+  // tags: distinct per launch, never 0; atomic so that host threads
+  // factoring on different streams draw distinct values
+  static std::atomic<unsigned> epoch{0};
   Workspace &ws = workspace(s);
   TsluP p{};
   p.W = W;
@@ -640,8 +643,9 @@ void tslu_leaf(hipStream_t s, int n, float *W, long ld, int c0, int w, int *ipiv
   p.pairs = pairs + (long)c0 * PAIR_STRIDE;
   p.err = sync_error_word(s);
   p.rec = (uint64_t *)ws.get("tslu.rec", (size_t)32 * TS_W * TS_REC * 8);
-  if (++epoch == 0) epoch = 1;
-  p.tag = epoch;
+  unsigned tag = epoch.fetch_add(1, std::memory_order_relaxed) + 1;
+  if (tag == 0) tag = epoch.fetch_add(1, std::memory_order_relaxed) + 1;
+  p.tag = tag;
   const int rpt0 = m > 32 * TS_NT ? 2 : 1;
   p.G = ceil_div(m, TS_NT * rpt0);
   const int K = (p.G + 3) / 4;
