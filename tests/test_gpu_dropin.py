@@ -258,7 +258,7 @@ def test_qr_against_oracle(gpu, prec):
     eps = EPS[prec]
     rng = np.random.default_rng(31)
     for m, n in ((1, 1), (3, 3), (4, 3), (10, 7), (63, 63), (64, 64), (100, 70), (130, 129),
-                 (300, 200), (520, 300), (1030, 512)):
+                 (300, 200), (520, 300), (1030, 512), (3000, 40)):
         A = rng.uniform(-1, 1, (m, n)).astype(DT[prec])
         if n > 40:
             A[:, 37] = 0

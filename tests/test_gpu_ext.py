@@ -259,3 +259,28 @@ def test_geqrf_device_matches_dropin(gpu, prec):
     dA = to_dev(A, DT[prec])
     assert fn(stream(), m, n, ptr(dA), m, None) == 0
     assert np.array_equal(from_dev(dA, m, n), Rd)
+
+
+def test_geqrf_full_size_properties(gpu):
+    """QR at the benchmark's size (fp32 4096 x 4096, through the device
+    entry point): Q^T Q = I and A = Q R checked in fp64 on the GPU, the
+    residue below R's diagonal within the reference test's m * eps bound,
+    and the R-only call (q = NULL) giving the same R bit for bit."""
+    g = gpu["f32"]
+    n = 4096
+    gen = torch.Generator(device="cuda").manual_seed(17)
+    A0 = (torch.rand(n * n, device="cuda", generator=gen) * 2 - 1).contiguous()
+    A = A0.clone()
+    Q = torch.empty_like(A0)
+    assert g.lib.mat_gpu_geqrf_f32(stream(), n, n, ptr(A), n, ptr(Q)) == 0
+    M = A0.view(n, n).double().T
+    Qm = Q.view(n, n).double().T
+    Rm = A.view(n, n).double().T
+    eye = torch.eye(n, dtype=torch.float64, device="cuda")
+    assert float((Qm.T @ Qm - eye).abs().max()) <= 20 * n * EPS["f32"]
+    assert float((Qm @ torch.triu(Rm) - M).norm() / M.norm()) <= 20 * n * EPS["f32"]
+    assert float(torch.tril(Rm, -1).abs().max()) <= n * 1e-6
+    A2 = A0.clone()
+    assert g.lib.mat_gpu_geqrf_f32(stream(), n, n, ptr(A2), n, None) == 0
+    torch.cuda.synchronize()
+    assert torch.equal(A2, A)
