@@ -59,8 +59,14 @@ __global__ __launch_bounds__(QR_NT) void qr_panel_kernel(int m, int jb, int kb, 
     T *y = Y + (long)j * ldy;
     // ||x||^2: every thread sums the wavefront partials in the same order,
     // so every thread holds the same norm and takes the same branch
-    T s = 0;
-    for (long i = t; i < vlen; i += QR_NT) s += x[i] * x[i];
+    T sq[4] = {0, 0, 0, 0};
+    long i = t;
+    for (; i + 3L * QR_NT < vlen; i += 4L * QR_NT) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) sq[r] += x[i + (long)r * QR_NT] * x[i + (long)r * QR_NT];
+    }
+    for (; i < vlen; i += QR_NT) sq[0] += x[i] * x[i];
+    T s = (sq[0] + sq[1]) + (sq[2] + sq[3]);
     s = wave_sum(s);
     if (lane == 0) red[wave] = s;
     __syncthreads();
@@ -104,19 +110,32 @@ __global__ __launch_bounds__(QR_NT) void qr_panel_kernel(int m, int jb, int kb, 
         if (lane == 0) wdot[c] = dd * tau;
       }
       __syncthreads();
-      // R(col:, c) += (-w_c) v, eight columns at a time: the eight values
-      // of a row are loaded before any is stored
-      for (long i = t; i < vlen; i += QR_NT) {
-        const T vi = y[j + i];
-        T *p = R + (long)(jb + j) * ldr + col + i;
+      // R(col:, c) += (-w_c) v, in tiles of RG rows (RG per thread, QR_NT
+      // apart) by eight columns: a tile's values are all loaded before any
+      // is stored, one memory round trip per tile
+      constexpr int RG = sizeof(T) == 4 ? 4 : 2;
+      for (long i0 = t; i0 < vlen; i0 += (long)RG * QR_NT) {
+        T vi[RG];
+        bool ok[RG];
+#pragma unroll
+        for (int r = 0; r < RG; ++r) {
+          ok[r] = i0 + (long)r * QR_NT < vlen;
+          vi[r] = ok[r] ? y[j + i0 + (long)r * QR_NT] : T(0);
+        }
+        T *p = R + (long)(jb + j) * ldr + col + i0;
         for (int c = j; c < kb; c += 8) {
-          T val[8];
+          T val[RG][8];
 #pragma unroll
-          for (int u = 0; u < 8; ++u)
-            if (c + u < kb) val[u] = p[(long)u * ldr];
+          for (int r = 0; r < RG; ++r)
 #pragma unroll
-          for (int u = 0; u < 8; ++u)
-            if (c + u < kb) p[(long)u * ldr] = val[u] + -wdot[c + u] * vi;
+            for (int u = 0; u < 8; ++u)
+              if (ok[r] && c + u < kb) val[r][u] = p[(long)u * ldr + (long)r * QR_NT];
+#pragma unroll
+          for (int r = 0; r < RG; ++r)
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+              if (ok[r] && c + u < kb)
+                p[(long)u * ldr + (long)r * QR_NT] = val[r][u] + -wdot[c + u] * vi[r];
           p += 8 * ldr;
         }
       }
