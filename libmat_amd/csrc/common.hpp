@@ -80,14 +80,15 @@ void timing_query(const char *family, long *launches, double *ms, double *work);
 // Library-owned default stream for the drop-in (synchronous) entry points.
 hipStream_t default_stream();
 
-// Second library-owned stream per device for the look-ahead LU's trailing
-// updates.  With MATGPU_PANEL_CUS=k (default 0: no mask) and a runtime that
+// Second library-owned stream per device for the look-ahead
+// factorizations' trailing updates (LU, Cholesky, QR).  With MATGPU_PANEL_CUS=k (default 0: no mask) and a runtime that
 // supports CU masks, it leaves k compute units to the caller's stream.
 hipStream_t side_stream();
 // Reusable events (timing disabled) of the current device, by index.
 hipEvent_t pooled_event(int i);
-// Held by a look-ahead factorization while it enqueues: the side stream and
-// the event pool are shared by every factorization on a device.
+// Held by a look-ahead factorization (LU, Cholesky, QR) while it enqueues:
+// the side stream and the event pool are shared by every factorization on
+// a device.
 std::mutex &lookahead_mutex();
 // Look-ahead block width of the LU (MATGPU_LU_NB, default 256; 0 = none).
 int lookahead_block();
