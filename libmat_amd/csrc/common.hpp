@@ -81,8 +81,9 @@ void timing_query(const char *family, long *launches, double *ms, double *work);
 hipStream_t default_stream();
 
 // Second library-owned stream per device for the look-ahead
-// factorizations' trailing updates (LU, Cholesky, QR).  With MATGPU_PANEL_CUS=k (default 0: no mask) and a runtime that
-// supports CU masks, it leaves k compute units to the caller's stream.
+// factorizations' trailing updates (LU, Cholesky, QR).  With
+// MATGPU_PANEL_CUS=k (default 0: no mask) and a runtime that supports CU
+// masks, it leaves k compute units to the caller's stream.
 hipStream_t side_stream();
 // Reusable events (timing disabled) of the current device, by index.
 hipEvent_t pooled_event(int i);
