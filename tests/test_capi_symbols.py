@@ -101,3 +101,37 @@ print("returned")
     r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
     assert r.returncode != 0 and "returned" not in r.stdout
     assert "assertion failed" in r.stderr and fn in r.stderr
+
+
+def test_qr_shape_contract():
+    """mat_qr asserts m >= n (mat.h:3914); the drop-in aborts with a message
+    on a wide matrix before any GPU call, so this runs without a GPU."""
+    import subprocess
+    import sys
+    code = f"""
+import ctypes as C, sys
+sys.path.insert(0, {libmat_amd.ROOT!r})
+import libmat_amd
+g = libmat_amd.load("f32")
+buf = (C.c_float * 64)()
+a = g.Mat(2, 3, C.cast(buf, C.POINTER(C.c_float)))
+q = g.Mat(2, 2, C.cast(buf, C.POINTER(C.c_float)))
+g.lib.mat_qr(C.byref(a), C.byref(q), C.byref(a))
+print("returned")
+"""
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
+    assert r.returncode != 0 and "returned" not in r.stdout
+    assert "assertion failed" in r.stderr and "mat_qr" in r.stderr
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_geqrf_rejects_bad_shapes(prec):
+    """The device entry point checks its shapes on the host before it touches
+    the device: m < n, n = 0 and lda < m return a negative code with a
+    message (no GPU needed)."""
+    g = libmat_amd.load(prec)
+    fn = getattr(g.lib, f"mat_gpu_geqrf_{prec}")
+    g.lib.mat_gpu_last_error.restype = C.c_char_p
+    for m, n, lda, what in ((2, 3, 2, b"m >= n"), (3, 0, 3, b"m >= n >= 1"), (4, 2, 3, b"lda")):
+        assert fn(None, m, n, None, lda, None) < 0, (m, n, lda)
+        assert what in g.lib.mat_gpu_last_error(), (m, n, lda)
