@@ -171,6 +171,50 @@ __global__ __launch_bounds__(64) void qr_tfactor_kernel(int kb, const T *G, cons
   for (int c = 0; c < kb; ++c) Tm[i + (long)c * kb] = Ts[i][c];
 }
 
+// C (kb x N, ld ldc) = Y^T B for Y (len x kb, ld ldy) and B (len x N, ld
+// ldb), kb, N <= 32, K = len long: split over chunks of QR_CH rows (one
+// workgroup each, both tiles staged in LDS) into partials P[chunk][32*32],
+// then summed in chunk order (deterministic, no atomics).  A GEMM tile
+// would walk all of K alone.
+constexpr int QR_CH = 64;
+template <typename T>
+__global__ __launch_bounds__(256) void qr_ytb_partial_kernel(long len, int kb, int N, const T *Y,
+                                                             long ldy, const T *B, long ldb, T *P) {
+  __shared__ T Ys[QR_CH][QR_NB + 1];
+  __shared__ T Bs[QR_CH][QR_NB + 1];
+  const long r0 = (long)blockIdx.x * QR_CH;
+  const int t = threadIdx.x;
+  for (int x = t; x < QR_CH * kb; x += 256) {
+    const int r = x % QR_CH, c = x / QR_CH;
+    Ys[r][c] = r0 + r < len ? Y[r0 + r + (long)c * ldy] : T(0);
+  }
+  for (int x = t; x < QR_CH * N; x += 256) {
+    const int r = x % QR_CH, c = x / QR_CH;
+    Bs[r][c] = r0 + r < len ? B[r0 + r + (long)c * ldb] : T(0);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < (QR_NB * QR_NB) / 256; ++q) {
+    const int idx = t + 256 * q, i = idx % QR_NB, j = idx / QR_NB;
+    if (i < kb && j < N) {
+      T acc = 0;
+#pragma unroll 8
+      for (int r = 0; r < QR_CH; ++r) acc += Ys[r][i] * Bs[r][j];
+      P[(long)blockIdx.x * (QR_NB * QR_NB) + idx] = acc;
+    }
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void qr_ytb_reduce_kernel(int nch, int kb, int N, const T *P,
+                                                            T *C, long ldc) {
+  const int idx = blockIdx.x * 256 + threadIdx.x, i = idx % QR_NB, j = idx / QR_NB;
+  if (idx >= QR_NB * QR_NB || i >= kb || j >= N) return;
+  T acc = 0;
+  for (int c = 0; c < nch; ++c) acc += P[(long)c * (QR_NB * QR_NB) + idx];
+  C[i + (long)j * ldc] = acc;
+}
+
 template <typename T>
 __global__ void qr_eye_kernel(long m, T *Q) {
   const long total = m * m;
@@ -207,6 +251,17 @@ void geqrf(hipStream_t s, int m, int n, T *R, long ldr, T *Q, T eps) {
   T *Cm = Q ? (T *)ws.get("qr.Cm", (size_t)m * QR_NB * sizeof(T)) : nullptr;
   T *Cm2 = Q ? (T *)ws.get("qr.Cm2", (size_t)m * QR_NB * sizeof(T)) : nullptr;
   const int nblk = (n + QR_NB - 1) / QR_NB;
+  T *P = (T *)ws.get("qr.P", ((size_t)m / QR_CH + 1) * QR_NB * QR_NB * sizeof(T));
+  // C (kb x N, ld kb) = Y^T B with B len x N (ld ldb), N <= QR_NB
+  auto ytb = [&](const T *Yp, long len, int kb, const T *B, long ldb, int N, T *Cp) {
+    const int nch = (int)((len + QR_CH - 1) / QR_CH);
+    hipLaunchKernelGGL((qr_ytb_partial_kernel<T>), dim3(nch), dim3(256), 0, s, len, kb, N, Yp, len,
+                       B, ldb, P);
+    MG_HIP(hipGetLastError());
+    hipLaunchKernelGGL((qr_ytb_reduce_kernel<T>), dim3((QR_NB * QR_NB + 255) / 256), dim3(256), 0,
+                       s, nch, kb, N, P, Cp, (long)kb);
+    MG_HIP(hipGetLastError());
+  };
   // events: 3k = Y_k and T_k ready; 3k+1 = side stream done with block k's
   // columns; 3k+2 = side stream done with block k (its slot is free)
   std::lock_guard<std::mutex> lk(lookahead_mutex());
@@ -243,7 +298,7 @@ void geqrf(hipStream_t s, int m, int n, T *R, long ldr, T *Q, T eps) {
     }
     const View<const T> y{Y, 1, len};
     // G = Y^T Y, then T
-    gemm<T>(s, kb, kb, len, T(1), y.t(), y, T(0), View<T>{G, 1, kb});
+    ytb(Y, len, kb, Y, len, kb, G);
     hipLaunchKernelGGL((qr_tfactor_kernel<T>), dim3(1), dim3(64), 0, s, kb, G, tau, Tm);
     MG_HIP(hipGetLastError());
     const View<const T> tm{Tm, 1, kb};
@@ -267,7 +322,10 @@ void geqrf(hipStream_t s, int m, int n, T *R, long ldr, T *Q, T eps) {
     // update of them (side stream) is done
     if (w1 > 0) {
       if (k >= 1) MG_HIP(hipStreamWaitEvent(s, pooled_event(3 * (k - 1) + 1), 0));
-      apply_r(s, y, tm, kb, len, jb, c1, w1, Cn, Cn2);
+      T *Rn = R + (long)c1 * ldr + jb;
+      ytb(Y, len, kb, Rn, ldr, w1, Cn);
+      gemm<T>(s, kb, w1, kb, T(1), tm.t(), View<const T>{Cn, 1, kb}, T(0), View<T>{Cn2, 1, kb});
+      gemm<T>(s, len, w1, kb, T(-1), y, View<const T>{Cn2, 1, kb}, T(1), View<T>{Rn, 1, ldr});
     }
   }
   hipEvent_t done = pooled_event(3 * nblk + 1);
