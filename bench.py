@@ -575,7 +575,10 @@ def run_qr(torch, libmat_amd, args):
            "reconstruction_error": recon, "orthogonality_error": orth,
            "qr_r_ms": round(r_ms, 3), "qr_r_gflops": round(4.0 * n ** 3 / 3.0 / r_ms / 1e6, 2),
            "kernels": breakdown(table, args.steps),
-           "roofline": roofline(table, dominant(table), "fp32", args.steps, args.profiled)}
+           # the panel is the critical path (the caller's stream); the GEMMs,
+           # about as long in sum, run mostly on the side stream beside it
+           "roofline": roofline(table, "qr_panel", "fp32", args.steps, args.profiled),
+           "roofline_updates": roofline(table, "gemm", "fp32", args.steps, args.profiled)}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_qr(min(args.cpu_n, 512), "f32")
     return out
