@@ -43,11 +43,18 @@ const char *last_error();
 // Device scratch owned by the library, one set per (device, stream):
 // named slots that grow on demand and are reused by later calls on the
 // same stream (stream order makes reuse safe).  Freed at process exit.
+// The slot map is locked, so host threads may look up slots of one stream
+// at once; a pointer stays valid until its slot grows, so calls that
+// enqueue work on one stream from several threads must still be ordered by
+// the caller (the drop-in entry points use a stream per host thread).
 struct Workspace {
+  std::mutex mu;
   std::unordered_map<std::string, std::pair<void *, size_t>> slots;
   // fresh (optional) is set when the slot was (re)allocated: its contents
   // are undefined
   void *get(const std::string &name, size_t bytes, bool *fresh = nullptr);
+  // the slot's buffer, or null when it was never allocated
+  void *find(const std::string &name);
   ~Workspace();
 };
 Workspace &workspace(hipStream_t s);
@@ -77,7 +84,10 @@ void timing_enable(bool on);
 // on the recorded events)
 void timing_query(const char *family, long *launches, double *ms, double *work);
 
-// Library-owned default stream for the drop-in (synchronous) entry points.
+// Library-owned stream of the calling host thread on the current device,
+// for the drop-in (synchronous) entry points: threads never share one, so
+// their uploads, workspaces and kernels stay apart.  A thread's streams go
+// back to a per-device pool when it exits and are handed to later threads.
 hipStream_t default_stream();
 
 // Second library-owned stream per device for the look-ahead

@@ -18,6 +18,7 @@ const char *last_error() { return g_last_error.c_str(); }
 void *Workspace::get(const std::string &name, size_t bytes, bool *fresh) {
   if (bytes == 0) bytes = 1;
   if (fresh) *fresh = false;
+  std::lock_guard<std::mutex> lk(mu);
   auto it = slots.find(name);
   if (it != slots.end() && it->second.second >= bytes) return it->second.first;
   if (fresh) *fresh = true;
@@ -32,6 +33,12 @@ void *Workspace::get(const std::string &name, size_t bytes, bool *fresh) {
   MG_HIP(hipMalloc(&p, rounded));
   slots[name] = {p, rounded};
   return p;
+}
+
+void *Workspace::find(const std::string &name) {
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = slots.find(name);
+  return it == slots.end() ? nullptr : it->second.first;
 }
 
 Workspace::~Workspace() {
@@ -55,25 +62,29 @@ Workspace &workspace(hipStream_t s) {
 
 int *sync_error_word(hipStream_t s) {
   Workspace &ws = workspace(s);
+  std::lock_guard<std::mutex> lk(ws.mu);
   auto it = ws.slots.find("sync.err");
   if (it != ws.slots.end()) return (int *)it->second.first;
-  int *p = (int *)ws.get("sync.err", sizeof(int));
-  MG_HIP(hipMemset(p, 0, sizeof(int)));  // once, at creation (never inside a capture)
-  return p;
+  void *p = nullptr;
+  MG_HIP(hipMalloc(&p, 256));
+  // once, at creation (never inside a capture), before any kernel can see it
+  MG_HIP(hipMemset(p, 0, sizeof(int)));
+  MG_HIP(hipStreamSynchronize(nullptr));
+  ws.slots["sync.err"] = {p, 256};
+  return (int *)p;
 }
 
 void sync_and_check(hipStream_t s) {
-  Workspace &ws = workspace(s);
-  auto it = ws.slots.find("sync.err");
-  if (it == ws.slots.end()) {
+  int *word = (int *)workspace(s).find("sync.err");
+  if (!word) {
     MG_HIP(hipStreamSynchronize(s));
     return;
   }
   int h = 0;
-  MG_HIP(hipMemcpyAsync(&h, it->second.first, sizeof(int), hipMemcpyDeviceToHost, s));
+  MG_HIP(hipMemcpyAsync(&h, word, sizeof(int), hipMemcpyDeviceToHost, s));
   MG_HIP(hipStreamSynchronize(s));
   if (h != 0) {
-    MG_HIP(hipMemsetAsync(it->second.first, 0, sizeof(int), s));
+    MG_HIP(hipMemsetAsync(word, 0, sizeof(int), s));
     MG_HIP(hipStreamSynchronize(s));
     throw Error{-5, "a multi-workgroup kernel timed out waiting for another workgroup"};
   }
@@ -205,17 +216,38 @@ int lookahead_block() {
   return nb;
 }
 
+namespace {
+// Streams of exited host threads, per device, for later threads to reuse
+// (with their workspaces).  Leaked at process exit, like the workspaces.
+std::mutex g_pool_mu;
+std::map<int, std::vector<hipStream_t>> *g_pool = new std::map<int, std::vector<hipStream_t>>();
+
+struct ThreadStreams {
+  std::map<int, hipStream_t> by_dev;
+  ~ThreadStreams() {
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    for (auto &kv : by_dev) (*g_pool)[kv.first].push_back(kv.second);
+  }
+};
+thread_local ThreadStreams t_streams;
+}  // namespace
+
 hipStream_t default_stream() {
-  static std::mutex m;
-  static std::map<int, hipStream_t> streams;
   int dev = 0;
   MG_HIP(hipGetDevice(&dev));
-  std::lock_guard<std::mutex> lk(m);
-  auto it = streams.find(dev);
-  if (it != streams.end()) return it->second;
-  hipStream_t s;
-  MG_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-  streams[dev] = s;
+  auto it = t_streams.by_dev.find(dev);
+  if (it != t_streams.by_dev.end()) return it->second;
+  hipStream_t s = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    auto &free_list = (*g_pool)[dev];
+    if (!free_list.empty()) {
+      s = free_list.back();
+      free_list.pop_back();
+    }
+  }
+  if (!s) MG_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  t_streams.by_dev[dev] = s;
   return s;
 }
 

@@ -389,6 +389,83 @@ void ORC(solve_multi)(size_t n, size_t nrhs, const elem *A, const elem *B,
   free(z);
 }
 
+/* Cholesky once, then per right-hand side L y = b and L^T x = y: what
+ * mat_solve_spd does internally per call (mat.h:4854-4865).  B and X are
+ * n x nrhs, column-major.  Returns mat_chol's result; X is untouched when
+ * it is -1.  Used as the multi-RHS Cholesky CPU baseline. */
+int ORC(chol_solve_multi)(size_t n, size_t nrhs, const elem *A, const elem *B,
+                          elem *X) {
+  elem *L = (elem *)malloc(n * n * sizeof(elem));
+  elem *y = (elem *)calloc(n, sizeof(elem));
+  int r = ORC(chol)(n, A, L);
+  if (r == 0) {
+    for (size_t c = 0; c < nrhs; c++) {
+      ORC(solve_tril)(n, L, &B[c * n], y);
+      ORC(solve_trilt)(n, L, y, &X[c * n]);
+    }
+  }
+  free(L);
+  free(y);
+  return r;
+}
+
+/* A batch of independent small systems, one mat_solve (kind 0) or
+ * mat_solve_spd (kind 1) call per matrix, in order: the per-matrix CPU
+ * loop the batched GPU solver replaces.  Matrix i is n[i] x n[i] at
+ * A + a_off[i]; its right-hand side and solution at b / x + v_off[i];
+ * info[i] gets mat_solve_spd's result (0 for kind 0). */
+void ORC(batch_solve)(size_t count, const int *n, const signed char *kind,
+                      const elem *A, const long long *a_off, const elem *b,
+                      elem *x, const long long *v_off, int *info) {
+  for (size_t i = 0; i < count; i++) {
+    const size_t ni = (size_t)n[i];
+    if (kind[i] == 0) {
+      ORC(solve)(ni, A + a_off[i], b + v_off[i], x + v_off[i]);
+      info[i] = 0;
+    } else {
+      info[i] = ORC(solve_spd)(ni, A + a_off[i], b + v_off[i], x + v_off[i]);
+    }
+  }
+}
+
+/* mat_inv (mat.h:6166-6225): blocked PLU of a copy, out = the permuted
+ * identity (column i holds a one at row p[i]), then in place the
+ * unit-lower forward elimination column j by column j over every column
+ * of out (zero entries skipped), then the upper back substitution from
+ * the last column, each row scaled by 1/U(j,j) before it updates the rows
+ * above. */
+void ORC(inv)(size_t n, const elem *A, elem *out) {
+  elem *m = (elem *)malloc(n * n * sizeof(elem));
+  size_t *p = (size_t *)malloc(n * sizeof(size_t));
+  memcpy(m, A, n * n * sizeof(elem));
+  for (size_t i = 0; i < n; i++) p[i] = i;
+  plu_inplace(n, m, p);
+  memset(out, 0, n * n * sizeof(elem));
+  for (size_t i = 0; i < n; i++) out[i * n + p[i]] = 1.0f;
+  for (size_t j = 0; j < n; j++) {
+    const elem *Lj = &m[j * n];
+    for (size_t col = 0; col < n; col++) {
+      elem *oc = &out[col * n];
+      const elem ojc = oc[j];
+      if (ojc == 0) continue;
+      for (size_t i = j + 1; i < n; i++) oc[i] -= ojc * Lj[i];
+    }
+  }
+  for (size_t jj = n; jj > 0; jj--) {
+    const size_t j = jj - 1;
+    const elem *Uj = &m[j * n];
+    const elem dinv = 1.0f / Uj[j];
+    for (size_t col = 0; col < n; col++) {
+      elem *oc = &out[col * n];
+      oc[j] *= dinv;
+      const elem ojc = oc[j];
+      for (size_t i = 0; i < j; i++) oc[i] -= ojc * Uj[i];
+    }
+  }
+  free(m);
+  free(p);
+}
+
 /* mat_det (mat.h:6227-6249): (-1)^swaps * prod diag(U). */
 elem ORC(det)(size_t n, const elem *A) {
   elem *L = (elem *)malloc(n * n * sizeof(elem));

@@ -69,6 +69,9 @@ class Oracle:
         f("solve_spd", I, [sz, P, P, P])
         f("solve_multi", None, [sz, sz, P, P, P])
         f("det", self.ct, [sz, P])
+        f("inv", None, [sz, P, P])
+        f("chol_solve_multi", I, [sz, sz, P, P, P])
+        f("batch_solve", None, [sz, P, P, P, P, P, P, P, P])
         f("qr", None, [sz, sz, P, P, P])
         f("qr_r", None, [sz, sz, P, P])
 
@@ -190,6 +193,36 @@ class Oracle:
         X = np.zeros_like(B, order="F")
         self._solve_multi(A.shape[0], B.shape[1], self._p(A), self._p(B), self._p(X))
         return X
+
+    def inv(self, A):
+        A = self._cm(A)
+        out = np.zeros_like(A, order="F")
+        self._inv(A.shape[0], self._p(A), self._p(out))
+        return out
+
+    def chol_solve_multi(self, A, B):
+        """mat_chol once + per column mat_solve_tril, mat_solve_trilt;
+        returns (X, mat_chol's result)."""
+        A, B = self._cm(A), self._cm(B)
+        X = np.zeros_like(B, order="F")
+        r = self._chol_solve_multi(A.shape[0], B.shape[1], self._p(A), self._p(B), self._p(X))
+        return X, r
+
+    def batch_solve(self, n, kind, A, a_off, b, v_off):
+        """mat_solve / mat_solve_spd per matrix over a packed batch (column-
+        major matrices at A[a_off[i]:], vectors at b[v_off[i]:]); returns
+        (x, info)."""
+        n = np.ascontiguousarray(n, np.int32)
+        kind = np.ascontiguousarray(kind, np.int8)
+        A = np.ascontiguousarray(A, self.dtype)
+        b = np.ascontiguousarray(b, self.dtype)
+        a_off = np.ascontiguousarray(a_off, np.int64)
+        v_off = np.ascontiguousarray(v_off, np.int64)
+        x = np.zeros_like(b)
+        info = np.zeros(n.size, np.int32)
+        self._batch_solve(n.size, self._p(n), self._p(kind), self._p(A), self._p(a_off),
+                          self._p(b), self._p(x), self._p(v_off), self._p(info))
+        return x, info
 
     def det(self, A):
         A = self._cm(A)

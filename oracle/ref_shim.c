@@ -117,6 +117,48 @@ void REF(solve_multi)(size_t n, size_t nrhs, const elem *A, const elem *B,
   mat_free_mat(z);
 }
 
+/* Cholesky once, then per right-hand side mat_solve_tril +
+ * mat_solve_trilt: mat_solve_spd's internals (mat.h:4854-4865). */
+int REF(chol_solve_multi)(size_t n, size_t nrhs, const elem *A, const elem *B,
+                          elem *X) {
+  Mat *L = mat_mat(n, n);
+  Vec *y = mat_vec(n);
+  Mat a = wrap(n, n, A);
+  int r = mat_chol(&a, L);
+  if (r == 0) {
+    for (size_t c = 0; c < nrhs; c++) {
+      Mat bb = wrap(n, 1, &B[c * n]), xx = wrap(n, 1, &X[c * n]);
+      mat_solve_tril(y, L, &bb);
+      mat_solve_trilt(&xx, L, y);
+    }
+  }
+  mat_free_mat(L);
+  mat_free_mat(y);
+  return r;
+}
+
+/* One mat_solve (kind 0) or mat_solve_spd (kind 1) per matrix, in order. */
+void REF(batch_solve)(size_t count, const int *n, const signed char *kind,
+                      const elem *A, const long long *a_off, const elem *b,
+                      elem *x, const long long *v_off, int *info) {
+  for (size_t i = 0; i < count; i++) {
+    const size_t ni = (size_t)n[i];
+    Mat a = wrap(ni, ni, A + a_off[i]), bb = wrap(ni, 1, b + v_off[i]),
+        xx = wrap(ni, 1, x + v_off[i]);
+    if (kind[i] == 0) {
+      mat_solve(&xx, &a, &bb);
+      info[i] = 0;
+    } else {
+      info[i] = mat_solve_spd(&xx, &a, &bb);
+    }
+  }
+}
+
+void REF(inv)(size_t n, const elem *A, elem *out) {
+  Mat a = wrap(n, n, A), o = wrap(n, n, out);
+  mat_inv(&o, &a);
+}
+
 elem REF(det)(size_t n, const elem *A) {
   Mat a = wrap(n, n, A);
   return mat_det(&a);

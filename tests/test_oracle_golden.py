@@ -271,3 +271,29 @@ def test_qr_known_answers_and_lapack(orc):
         assert rel_fro(np.triu(R), Rl) <= tol, (m, n)
         assert rel_fro(Q, Ql) <= tol, (m, n)
         assert np.array_equal(orc.qr_r(A), R), (m, n)
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_inv_closed_form(prec):
+    """mat_inv (mat.h:6166-6225) seeds its right-hand sides with the
+    identity permuted by p^-1 (column i holds a one at row p[i]) where
+    A^-1 = U^-1 L^-1 P needs P, so it returns A^-1 (P^T)^2: A^-1 itself
+    exactly when the row permutation is its own inverse (no swaps, or
+    disjoint transpositions), as for the reference's own diagonally
+    dominant test matrices (tests/test_mat_inv.c:82-118).  Pinned here
+    against numpy's inverse (LAPACK getri) in both cases."""
+    o = Oracle(prec)
+    rng = np.random.default_rng(21)
+    tol = 1e-3 if prec == "f32" else 1e-10
+    for n in (1, 2, 3, 8, 25, 60):
+        A = rng.uniform(-1, 1, (n, n)).astype(o.dtype)
+        X = o.inv(A).astype(np.float64)
+        _, _, p, _ = o.plu(A)
+        Pt = np.zeros((n, n))
+        Pt[np.asarray(p), np.arange(n)] = 1.0  # column i: a one at row p[i]
+        Ainv = np.linalg.inv(A.astype(np.float64))
+        want = Ainv @ Pt @ Pt
+        assert np.abs(X - want).max() <= tol * max(1.0, np.abs(want).max()), n
+        D = (A + n * np.eye(n)).astype(o.dtype)  # diagonally dominant: no swaps
+        XD = o.inv(D).astype(np.float64)
+        assert np.abs(D.astype(np.float64) @ XD - np.eye(n)).max() <= tol, n

@@ -72,3 +72,38 @@ def test_qr_bit_exact(prec):
         Qr, Rr = r.qr(A)
         assert np.array_equal(Qo, Qr) and np.array_equal(Ro, Rr), (m, n)
         assert np.array_equal(o.qr_r(A), r.qr_r(A)), (m, n)
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_inv_and_multi_rhs_bit_exact(prec):
+    """mat_inv (mat.h:6166-6225), the multi-RHS Cholesky solve
+    (mat.h:4854-4865 per column) and the per-matrix batch loop."""
+    o, r = Oracle(prec), Oracle(prec, "ref")
+    rng = np.random.default_rng(13)
+    dt = o.dtype
+    for n in SIZES:
+        A = rng.uniform(-1, 1, (n, n)).astype(dt)
+        assert np.array_equal(o.inv(A), r.inv(A)), n
+        S = (A @ A.T + n * np.eye(n)).astype(dt)
+        B = rng.uniform(-1, 1, (n, 3)).astype(dt)
+        Xo, ro = o.chol_solve_multi(S, B)
+        Xr, rr = r.chol_solve_multi(S, B)
+        assert ro == rr == 0 and np.array_equal(Xo, Xr), n
+    # a batch with general, SPD and one non-SPD member
+    ns = np.array([1, 3, 7, 24, 25, 5], np.int32)
+    kinds = np.array([0, 1, 0, 1, 0, 1], np.int8)
+    mats, vecs = [], []
+    for n, k in zip(ns, kinds):
+        M = rng.uniform(-1, 1, (n, n))
+        M = M + n * np.eye(n) if k == 0 else M @ M.T + np.eye(n)
+        mats.append(M.astype(dt))
+        vecs.append(rng.uniform(-1, 1, n).astype(dt))
+    mats[-1] = -mats[-1]
+    a_off = np.concatenate([[0], np.cumsum(ns.astype(np.int64) ** 2)[:-1]])
+    v_off = np.concatenate([[0], np.cumsum(ns.astype(np.int64))[:-1]])
+    Ap = np.concatenate([m.ravel(order="F") for m in mats])
+    bp = np.concatenate(vecs)
+    xo, io = o.batch_solve(ns, kinds, Ap, a_off, bp, v_off)
+    xr, ir = r.batch_solve(ns, kinds, Ap, a_off, bp, v_off)
+    assert np.array_equal(xo, xr) and np.array_equal(io, ir)
+    assert io.tolist() == [0, 0, 0, 0, 0, -1]
