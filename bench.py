@@ -1,6 +1,14 @@
 """Benchmarks of the dense factor-and-solve path on one MI355X.
 
-Default workload (the headline metric): mixed-precision general solve.
+With no --workload, one run measures every single-GPU configuration and
+prints ONE JSON line: the headline (mixed-precision general solve, below)
+at the top level and the others under "configs" (fp32 GEMM 8192, fp32
+Cholesky 8192 + 8 RHS, mixed-precision SPD 32768, batched 200k), each with
+its value, ms_per_step, roofline and cpu_baseline.  Every timed loop runs
+with the library's per-launch timing off; the kernel table and the roofline
+come from a separate instrumented pass of the same launches afterwards.
+
+Headline workload: mixed-precision general solve.
 N = 16384 dense system, 16 right-hand sides ~ U[-1,1], A = U diag(sigma) V^T
 with U, V the Q factors of seeded Gaussian matrices and sigma geometrically
 spaced for condition numbers kappa in {1e2, 1e4, 1e6}.  One step = solving
@@ -30,13 +38,16 @@ Other workloads (--workload):
 Every line carries "roofline" for the workload's dominant kernel family:
 achieved = algorithmic bytes (or flops) of its launches / their summed
 durations, measured with HIP events recorded by the library on the launch
-stream during the timed steps, with "traffic" = HBM bytes per launch from
+stream during an instrumented pass after the timed steps, with "traffic" =
+HBM bytes per launch from
 the rocprofv3 FETCH_SIZE / WRITE_SIZE passes committed under profiles/
 (hbm_traffic__*.md; null for a workload run at other sizes or not
 profiled); and "cpu_baseline": the reference's x86
 scalar path (oracle/_ref, compiled from the reference sources; the
-clean-room oracle restatement when that is absent) timed on one host core
-on a bounded sample.
+clean-room oracle restatement when that is absent) timed in a child process
+pinned to one host core while the GPU runs: 1 s warm-up, then the median of
+10 samples at three sizes, fitted as a n^3 and reported as extrapolated to
+the configuration's size (the batched loop runs at full size).
 
 Prints one JSON line.
 """
@@ -143,73 +154,205 @@ def breakdown(table, steps):
             for f, r in sorted(table.items(), key=lambda x: -x[1]["ms"])}
 
 
+def instrumented_pass(torch, g, step, reps):
+    """Run `step` reps times with the library's per-launch HIP-event timing
+    on (after the timed loop, which runs with it off) and return the kernel
+    table of those launches."""
+    torch.cuda.synchronize()
+    g.check(g.lib.mat_gpu_timing_enable(1), "timing")
+    for _ in range(reps):
+        step()
+    torch.cuda.synchronize()
+    g.check(g.lib.mat_gpu_timing_enable(0), "timing")
+    return kernel_table(g.lib)
+
+
+def timed(torch, step, steps, warmup, label):
+    """Warm-up, then `steps` steps bracketed by device synchronizations;
+    returns the wall seconds of the timed steps."""
+    for w in range(warmup):
+        step()
+        log(f"{label}: warmup {w} done")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        step()
+        if steps <= 25:
+            log(f"{label}: step {k} done")
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
+INSTR_NOTE = ("kernel durations from HIP events the library records on each launch's stream, "
+              "in an instrumented pass of {} step(s) after the timed loop")
+
+
 # ------------------------------------------------------------ CPU baselines
+# Run in a child process (`bench.py --cpu-worker`) pinned to one host core,
+# started before this process touches the GPU and timed while the GPU runs.
+# Method of the reference's benchmark harness (zap.h:792-804, 1133-1226):
+# monotonic clock, 1 s warm-up, median of 10 samples.  The scalar path is
+# far too slow for the configurations' sizes (GEMM 8192 alone would take
+# over half an hour), so each op is sampled at three sizes, fitted as
+# t = a n^3 and reported as extrapolated to the configuration's size; the
+# batched per-matrix loop runs at full size.
+CPU_SAMPLES = 10
+
+
+def _host_cpu():
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model
+
+
 def _oracle(prec):
     from oracle import oracle as orc
     kind = "reference" if orc.available(prec, "ref") else "port"
     return orc.Oracle(prec, kind="ref" if kind == "reference" else "port"), kind
 
 
-def _cpu_loop(fn, flops, budget_s, max_runs=5):
-    times = []
-    t_start = time.perf_counter()
-    while True:
+def _median_s(fn, samples):
+    ts = []
+    for _ in range(samples):
         t0 = time.perf_counter()
         fn()
-        times.append(time.perf_counter() - t0)
-        if time.perf_counter() - t_start > budget_s or len(times) >= max_runs:
-            break
-    t = float(np.median(times))
-    return flops / t / 1e9, t, len(times)
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts))
 
 
-def cpu_solve_multi(n, nrhs=16, budget_s=12.0):
-    o, kind = _oracle("f64")
+def _warm_up(fn, secs=1.0):
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < secs:
+        fn()
+
+
+def _cpu_sweep(make, flops, sizes, config_n, what):
+    """make(n) -> a no-argument callable doing one sample's work at order n."""
+    _warm_up(make(sizes[0]))
+    points = []
+    for n in sizes:
+        fn = make(n)
+        fn()
+        t = _median_s(fn, CPU_SAMPLES)
+        points.append({"n": n, "median_s": round(t, 6), "gflops": round(flops(n) / t / 1e9, 4),
+                       "samples": CPU_SAMPLES})
+    ln_n = np.log(np.array(sizes, dtype=np.float64))
+    ln_t = np.log(np.array([p["median_s"] for p in points]))
+    a = float(np.exp(np.mean(ln_t - 3.0 * ln_n)))
+    t_cfg = a * config_n ** 3
+    return {"value": round(flops(config_n) / t_cfg / 1e9, 4), "unit": "GFLOP/s",
+            "extrapolated": True, "config_n": config_n,
+            "extrapolated_s_at_config": round(t_cfg, 2),
+            "fit": "t = a n^3 over the sampled sizes",
+            "fit_free_exponent": round(float(np.polyfit(ln_n, ln_t, 1)[0]), 3),
+            "points": points, "sample": what}
+
+
+def cpu_op(name):
     rng = np.random.default_rng(0)
-    A = rng.uniform(-1, 1, (n, n))
-    B = rng.uniform(-1, 1, (n, nrhs))
-    v, t, runs = _cpu_loop(lambda: o.solve_multi(A, B), 2.0 / 3.0 * n ** 3 + 2.0 * n ** 2 * nrhs,
-                           budget_s, max_runs=200)
-    return {"value": round(v, 4), "unit": "GFLOP/s", "cores": 1, "kind": kind,
-            "sample": f"mat_plu + {nrhs}x(mat_solve_tril_unit, mat_solve_triu), fp64, n={n}, "
-                      f"median of {runs} runs ({t:.2f} s each), single thread"}
+    if name == "mixed-lu":
+        o, kind = _oracle("f64")
+        nrhs = 16
+
+        def make(n):
+            A, B = rng.uniform(-1, 1, (n, n)), rng.uniform(-1, 1, (n, nrhs))
+            return lambda: o.solve_multi(A, B)
+        r = _cpu_sweep(make, lambda n: 2.0 / 3.0 * n ** 3 + 2.0 * n ** 2 * nrhs, [512, 1024, 2048],
+                       16384, "mat_plu + 16 x (permute, mat_solve_tril_unit, mat_solve_triu), "
+                              "fp64, U[-1,1] systems (a direct LU's time does not depend on kappa)")
+    elif name == "gemm":
+        o, kind = _oracle("f32")
+
+        def make(n):
+            A = rng.uniform(-1, 1, (n, n)).astype(np.float32)
+            B = rng.uniform(-1, 1, (n, n)).astype(np.float32)
+            return lambda: o.mul(A, B)
+        r = _cpu_sweep(make, lambda n: 2.0 * n ** 3, [256, 384, 512], 8192,
+                       "mat_mul fp32, U[-1,1]")
+    elif name in ("chol", "mixed-spd"):
+        prec, nrhs, cfg = ("f32", 8, 8192) if name == "chol" else ("f64", 4, 32768)
+        o, kind = _oracle(prec)
+
+        def make(n):
+            G = rng.standard_normal((n, n))
+            A = (G @ G.T / n + np.eye(n)).astype(o.dtype)
+            B = rng.uniform(-1, 1, (n, nrhs)).astype(o.dtype)
+            return lambda: o.chol_solve_multi(A, B)
+        r = _cpu_sweep(make, lambda n: n ** 3 / 3.0 + 2.0 * n ** 2 * nrhs, [512, 768, 1024], cfg,
+                       f"mat_chol + {nrhs} x (mat_solve_tril, mat_solve_trilt), {prec}, "
+                       "SPD G G^T / n + I")
+    elif name == "batched":
+        o, kind = _oracle("f32")
+        ns, kinds, A, b, aoff, voff = make_batch(200000)
+        _warm_up(lambda: o.batch_solve(ns[:2000], kinds[:2000], A, aoff[:2000], b, voff[:2000]))
+        t = _median_s(lambda: o.batch_solve(ns, kinds, A, aoff, b, voff), CPU_SAMPLES)
+        flops = batch_flops(ns, kinds)
+        r = {"value": round(flops / t / 1e9, 4), "unit": "GFLOP/s",
+             "matrices_per_s": round(ns.size / t, 1), "extrapolated": False,
+             "median_s": round(t, 4), "samples": CPU_SAMPLES,
+             "sample": "all 200000 systems, one mat_solve / mat_solve_spd per matrix in a C loop"}
+    elif name == "qr":
+        o, kind = _oracle("f32")
+
+        def make(n):
+            A = rng.uniform(-1, 1, (n, n)).astype(np.float32)
+            return lambda: o.qr(A)
+        r = _cpu_sweep(make, lambda n: 8.0 * n ** 3 / 3.0, [256, 384, 512], 4096,
+                       "mat_qr (R and Q), fp32, U[-1,1]")
+    else:
+        raise ValueError(name)
+    r["kind"] = kind
+    return r
 
 
-def cpu_gemm(n, budget_s=20.0):
-    o, kind = _oracle("f32")
-    rng = np.random.default_rng(0)
-    A = rng.uniform(-1, 1, (n, n)).astype(np.float32)
-    B = rng.uniform(-1, 1, (n, n)).astype(np.float32)
-    v, t, runs = _cpu_loop(lambda: o.mul(A, B), 2.0 * n ** 3, budget_s, max_runs=3)
-    return {"value": round(v, 4), "unit": "GFLOP/s", "cores": 1, "kind": kind,
-            "sample": f"mat_mul fp32, n={n}, median of {runs} runs ({t:.2f} s each), single thread"}
+def cpu_worker(ops, core):
+    """Child-process entry: pin to one core, time the ops, print one JSON
+    object {op: baseline} on stdout.  Never touches the GPU."""
+    if core is not None:
+        os.sched_setaffinity(0, {core})
+    host = {"cpu_model": _host_cpu(), "nproc": os.cpu_count(), "cores_used": 1,
+            "pinned_core": core}
+    out = {}
+    for op in ops:
+        t0 = time.perf_counter()
+        try:
+            out[op] = cpu_op(op)
+        except Exception as e:  # reported in the JSON, the GPU results still stand
+            out[op] = {"error": repr(e)}
+        out[op].update({"cores": 1, "host": host})
+        log(f"cpu baseline {op} done in {time.perf_counter() - t0:.1f} s")
+    print(json.dumps(out), flush=True)
 
 
-def cpu_chol(n, nrhs, prec, budget_s=12.0):
-    o, kind = _oracle(prec)
-    rng = np.random.default_rng(0)
-    G = rng.standard_normal((n, n))
-    A = (G @ G.T / n + np.eye(n)).astype(o.dtype)
-    bs = [rng.uniform(-1, 1, n).astype(o.dtype) for _ in range(nrhs)]
+class CpuBaselines:
+    """The CPU-baseline child process, started before the GPU is touched."""
 
-    def run():
-        L, r = o.chol(A)
-        for b in bs:
-            o.solve_trilt(L, o.solve_tril(L, b))
+    def __init__(self, ops):
+        import subprocess
+        self.proc = None
+        if not ops:
+            return
+        cores = sorted(os.sched_getaffinity(0))
+        core = cores[-1]
+        self.proc = subprocess.Popen(
+            [sys.executable, os.path.abspath(__file__), "--cpu-worker", ",".join(ops),
+             "--cpu-core", str(core)], stdout=subprocess.PIPE, text=True)
 
-    v, t, runs = _cpu_loop(run, n ** 3 / 3.0 + 2.0 * n ** 2 * nrhs, budget_s, max_runs=200)
-    return {"value": round(v, 4), "unit": "GFLOP/s", "cores": 1, "kind": kind,
-            "sample": f"mat_chol + {nrhs}x(mat_solve_tril, mat_solve_trilt), {prec}, n={n}, "
-                      f"median of {runs} runs ({t:.2f} s each), single thread"}
-
-
-def cpu_qr(n, prec, budget_s=12.0):
-    o, kind = _oracle(prec)
-    A = np.random.default_rng(0).uniform(-1, 1, (n, n)).astype(o.dtype)
-    v, t, runs = _cpu_loop(lambda: o.qr(A), 8.0 * n ** 3 / 3.0, budget_s, max_runs=200)
-    return {"value": round(v, 4), "unit": "GFLOP/s", "cores": 1, "kind": kind,
-            "sample": f"mat_qr (R and Q), {prec}, n={n}, median of {runs} runs ({t:.2f} s each), "
-                      "single thread"}
+    def results(self, timeout=900):
+        if self.proc is None:
+            return {}
+        out, _ = self.proc.communicate(timeout=timeout)
+        try:
+            return json.loads(out.strip().splitlines()[-1])
+        except (ValueError, IndexError):
+            return {"error": f"cpu worker exit {self.proc.returncode}"}
 
 
 # ------------------------------------------------------------ mixed-lu (default)
@@ -290,19 +433,9 @@ def run_mixed(torch, libmat_amd, args, spd):
             g.check(solve(stream, n, nrhs, ptr(A), n, ptr(B), n, ptr(X), n, C.byref(opts),
                           C.byref(status[i])), "mixed solve")
 
-    for w in range(args.warmup):
-        step()
-        log(f"warmup {w} done")
-    torch.cuda.synchronize()
-    g.check(lib.mat_gpu_timing_enable(1), "timing")
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        step()
-        log(f"step {k} done")
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    g.check(lib.mat_gpu_timing_enable(0), "timing")
-    table = kernel_table(lib)
+    label = "mixed-spd" if spd else "mixed-lu"
+    elapsed = timed(torch, step, args.steps, args.warmup, label)
+    table = instrumented_pass(torch, g, step, 1)
     ms_per_step = elapsed / args.steps * 1e3
     if spd:
         flops_per_solve = n ** 3 / 3.0 + 2.0 * n ** 2 * nrhs
@@ -369,7 +502,7 @@ def run_mixed(torch, libmat_amd, args, spd):
                                         "split MFMA trailing updates"),
                       "refinement": "fp64 residual + fp64 update, fallback to fp64 on stall",
                       "parallelism": "single GPU"},
-           "per_kappa": per, "kernels": breakdown(table, args.steps)}
+           "per_kappa": per, "kernels": breakdown(table, 1)}
     if not args.no_plain:
         plain_gflops = flops_per_solve * len(kappas) / (plain_ms_total * 1e-3) / 1e9
         out["plain_fp64"] = {"value": round(plain_gflops, 2), "unit": "GFLOP/s",
@@ -383,14 +516,11 @@ def run_mixed(torch, libmat_amd, args, spd):
     del mats
     torch.cuda.empty_cache()
     dom = dominant(table)
-    out["roofline"] = roofline(table, dom, steps=args.steps, workload=args.profiled)
+    out["roofline"] = roofline(table, dom, steps=1, workload=args.profiled)
+    out["roofline"]["measured_in"] = INSTR_NOTE.format(1)
     gem = "gemm_fp8" if spd else "gemm_bf16x"
     if gem in table and gem != dom:
-        out["roofline_trailing_update"] = roofline(table, gem, steps=args.steps,
-                                                   workload=args.profiled)
-    if not args.no_cpu:
-        out["cpu_baseline"] = (cpu_chol(args.cpu_n, nrhs, "f64") if spd else
-                               cpu_solve_multi(args.cpu_n, nrhs))
+        out["roofline_trailing_update"] = roofline(table, gem, steps=1, workload=args.profiled)
     return out
 
 
@@ -442,17 +572,8 @@ def run_gemm(torch, libmat_amd, args):
         g.check(lib.mat_gpu_gemm_f32(s, 0, 0, n, n, n, 1.0, ptr(A), n, ptr(B), n, 0.0, ptr(Cc), n),
                 "gemm")
 
-    for w in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    g.check(lib.mat_gpu_timing_enable(1), "timing")
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    g.check(lib.mat_gpu_timing_enable(0), "timing")
-    table = kernel_table(lib)
+    elapsed = timed(torch, step, args.steps, args.warmup, "gemm")
+    table = instrumented_pass(torch, g, step, 3)
     # sampled check against an fp64 product of the same operands
     rows = torch.randint(0, n, (64,), device="cuda", generator=gen)
     Am = A.view(n, n).T  # column-major buffers
@@ -467,10 +588,9 @@ def run_gemm(torch, libmat_amd, args):
            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic U[-1,1]",
            "config": {"workload": "fp32 GEMM (mat_gemm)", "m": n, "n": n, "k": n,
                       "parallelism": "single GPU"},
-           "max_rel_err_vs_fp64_sampled_rows": rel, "kernels": breakdown(table, args.steps),
-           "roofline": roofline(table, "gemm", "fp32", args.steps, args.profiled)}
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_gemm(min(args.cpu_n, 1024))
+           "max_rel_err_vs_fp64_sampled_rows": rel, "kernels": breakdown(table, 3),
+           "roofline": roofline(table, "gemm", "fp32", 3, args.profiled)}
+    out["roofline"]["measured_in"] = INSTR_NOTE.format(3)
     return out
 
 
@@ -492,17 +612,8 @@ def run_chol(torch, libmat_amd, args):
         g.check(rc, "posv")
         assert rc == 0, "matrix not positive definite"
 
-    for w in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    g.check(lib.mat_gpu_timing_enable(1), "timing")
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    g.check(lib.mat_gpu_timing_enable(0), "timing")
-    table = kernel_table(lib)
+    elapsed = timed(torch, step, args.steps, args.warmup, "chol")
+    table = instrumented_pass(torch, g, step, 3)
     Ad = A.double()
     R = Ad @ X.double().T - B.double().T
     berr = float((R.abs().amax(dim=0) / (Ad.abs().sum(dim=1).max() * X.double().abs().amax(dim=1)
@@ -516,10 +627,9 @@ def run_chol(torch, libmat_amd, args):
            "data": "synthetic SPD G G^T / N + I, G ~ N(0,1)",
            "config": {"workload": "fp32 Cholesky solve (mat_chol + mat_solve_spd)", "n": n,
                       "nrhs": nrhs, "parallelism": "single GPU"},
-           "backward_error": berr, "kernels": breakdown(table, args.steps),
-           "roofline": roofline(table, dominant(table), "fp32", args.steps, args.profiled)}
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_chol(min(args.cpu_n, 1024), nrhs, "f32")
+           "backward_error": berr, "kernels": breakdown(table, 3),
+           "roofline": roofline(table, dominant(table), "fp32", 3, args.profiled)}
+    out["roofline"]["measured_in"] = INSTR_NOTE.format(3)
     return out
 
 
@@ -542,17 +652,8 @@ def run_qr(torch, libmat_amd, args):
         A.copy_(A0)
         g.check(lib.mat_gpu_geqrf_f32(s, n, n, ptr(A), n, ptr(Q) if q else None), "geqrf")
 
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    g.check(lib.mat_gpu_timing_enable(1), "timing")
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    g.check(lib.mat_gpu_timing_enable(0), "timing")
-    table = kernel_table(lib)
+    elapsed = timed(torch, step, args.steps, args.warmup, "qr")
+    table = instrumented_pass(torch, g, step, 1)
     # column-major buffers: the factored matrix is A0^T, Q and R likewise
     M = A0.double().T
     Qm, Rm = Q.double().T, torch.triu(A.double().T)
@@ -574,13 +675,14 @@ def run_qr(torch, libmat_amd, args):
                       "flops": "8/3 n^3 (R 4/3 n^3 + Q 4/3 n^3)", "parallelism": "single GPU"},
            "reconstruction_error": recon, "orthogonality_error": orth,
            "qr_r_ms": round(r_ms, 3), "qr_r_gflops": round(4.0 * n ** 3 / 3.0 / r_ms / 1e6, 2),
-           "kernels": breakdown(table, args.steps),
+           "kernels": breakdown(table, 1),
            # the panel is the critical path (the caller's stream); the GEMMs,
            # about as long in sum, run mostly on the side stream beside it
-           "roofline": roofline(table, "qr_panel", "fp32", args.steps, args.profiled),
-           "roofline_updates": roofline(table, "gemm", "fp32", args.steps, args.profiled)}
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_qr(min(args.cpu_n, 512), "f32")
+           "roofline": roofline(table, "qr_panel", "fp32", 1, args.profiled),
+           "roofline_updates": roofline(table, "gemm", "fp32", 1, args.profiled)}
+    # one workgroup per column step: latency-bound, reported against HBM
+    out["roofline"]["bound"] = "latency (reported against hbm)"
+    out["roofline"]["measured_in"] = INSTR_NOTE.format(1)
     return out
 
 
@@ -609,6 +711,14 @@ def make_batch(count, seed=5, lo=3, hi=64):
             for j, i in enumerate(idx):
                 A[aoff[i]:aoff[i] + n * n] = Mf[j]
     return ns, kinds, A, b, aoff, voff
+
+
+def batch_flops(ns, kinds):
+    """Sum of (2/3 n^3 + 2 n^2) over the general systems and (n^3/3 + 2 n^2)
+    over the SPD ones (bench_solve.cpp:79, bench_solve_spd.cpp:89)."""
+    n64 = np.asarray(ns, np.float64)
+    return float(np.sum(np.where(np.asarray(kinds) == 0, 2.0 / 3.0 * n64 ** 3 + 2 * n64 ** 2,
+                                 n64 ** 3 / 3.0 + 2 * n64 ** 2)))
 
 
 def run_batched(torch, libmat_amd, args):
@@ -701,9 +811,7 @@ def run_batched(torch, libmat_amd, args):
     lib.mat_gpu_batch_destroy(plan)
 
     n64 = ns.astype(np.float64)
-    gen_mask = kinds == 0
-    flops = float(np.sum(np.where(gen_mask, 2.0 / 3.0 * n64 ** 3 + 2 * n64 ** 2,
-                                  n64 ** 3 / 3.0 + 2 * n64 ** 2)))
+    flops = batch_flops(ns, kinds)
     bytes_min = float(np.sum(n64 ** 2 + 2 * n64) * 4)
     value = flops / (batch_ms * 1e-3) / 1e9
     achieved = bytes_min / (batch_ms * 1e-3) / 1e9
@@ -732,37 +840,57 @@ def run_batched(torch, libmat_amd, args):
                         "unit": "GB/s", "frac": round(achieved / PEAK["hbm"], 5),
                         "traffic": hbm_traffic("batched", args.profiled),
                         "avg_launch_us": round(batch_ms * 1e3, 3)}}
-    if not args.no_cpu:
-        o, kind = _oracle("f32")
-        csample = min(args.cpu_count, count)
-        mats = [A[aoff[i]:aoff[i] + ns[i] ** 2].reshape(ns[i], ns[i], order="F")
-                for i in range(csample)]
-        vecs = [b[voff[i]:voff[i] + ns[i]] for i in range(csample)]
-        t0 = time.perf_counter()
-        for i in range(csample):
-            if kinds[i] == 0:
-                o.solve(mats[i], vecs[i])
-            else:
-                o.solve_spd(mats[i], vecs[i])
-        t = time.perf_counter() - t0
-        sflops = float(np.sum(np.where(gen_mask[:csample], 2.0 / 3.0 * n64[:csample] ** 3
-                                       + 2 * n64[:csample] ** 2,
-                                       n64[:csample] ** 3 / 3.0 + 2 * n64[:csample] ** 2)))
-        out["cpu_baseline"] = {"value": round(sflops / t / 1e9, 4), "unit": "GFLOP/s", "cores": 1,
-                               "kind": kind,
-                               "sample": f"first {csample} of the {count} systems, mat_solve / "
-                                         f"mat_solve_spd per matrix through ctypes, "
-                                         f"{t:.2f} s, single thread"}
     return out
+
+
+DEFAULTS = {  # n, nrhs, kappas, steps, warmup
+    "mixed-lu": (16384, 16, "1e2,1e4,1e6", 2, 1), "mixed-spd": (32768, 4, "1e2,1e3", 2, 1),
+    "gemm": (8192, 0, None, 20, 3), "chol": (8192, 8, None, 10, 2), "batched": (0, 1, None, 20, 3),
+    "qr": (4096, 0, None, 3, 1)}
+# workload name of the committed HBM-traffic passes, when run at their sizes
+TAGS = {"mixed-lu": "mixed_lu_n16384", "mixed-spd": "mixed_spd_n32768", "gemm": "gemm_8192",
+        "chol": "chol_8192", "batched": "batched_200k", "qr": "qr_4096"}
+# the other configurations of the default run, under these keys
+ALL_CONFIGS = [("gemm", "gemm_fp32_8192"), ("chol", "chol_fp32_8192_8rhs"),
+               ("mixed-spd", "mixed_spd_32768"), ("batched", "batched_200k")]
+
+
+def workload_args(args, workload, steps=None, warmup=None):
+    a = argparse.Namespace(**vars(args))
+    dn, dr, dk, ds, dw = DEFAULTS[workload]
+    a.workload = workload
+    a.n = args.n or dn
+    a.nrhs = args.nrhs or dr
+    a.kappas = args.kappas or dk
+    a.steps = steps or ds
+    a.warmup = dw if warmup is None else warmup
+    same = (a.n == dn and a.nrhs == dr and (workload != "batched" or a.count == 200000))
+    a.profiled = TAGS[workload] if same else None
+    return a
+
+
+def run_one(torch, libmat_amd, a):
+    if a.workload == "mixed-lu":
+        return run_mixed(torch, libmat_amd, a, spd=False)
+    if a.workload == "mixed-spd":
+        return run_mixed(torch, libmat_amd, a, spd=True)
+    if a.workload == "gemm":
+        return run_gemm(torch, libmat_amd, a)
+    if a.workload == "chol":
+        return run_chol(torch, libmat_amd, a)
+    if a.workload == "qr":
+        return run_qr(torch, libmat_amd, a)
+    return run_batched(torch, libmat_amd, a)
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=None)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="mixed-lu",
-                    choices=["mixed-lu", "mixed-spd", "gemm", "chol", "batched", "qr"])
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps of the headline (or of --workload)")
+    ap.add_argument("--warmup", type=int, default=None)
+    ap.add_argument("--workload", default="all",
+                    choices=["all", "mixed-lu", "mixed-spd", "gemm", "chol", "batched", "qr"])
     ap.add_argument("--n", type=int, default=None)
     ap.add_argument("--nrhs", type=int, default=None)
     ap.add_argument("--kappas", default=None)
@@ -771,49 +899,49 @@ def main():
                     help="mixed LU: 0 partial, 1 butterfly, 2 tournament (library default if unset)")
     ap.add_argument("--count", type=int, default=200000)
     ap.add_argument("--loop-sample", type=int, default=2000)
-    ap.add_argument("--cpu-count", type=int, default=200000)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--data", default="conditioned", choices=["conditioned", "uniform"],
                     help="mixed-lu: U[-1,1] systems instead of the conditioned family (same "
                          "shapes; for counter profilers that cannot run the vendor QR)")
-    ap.add_argument("--cpu-n", type=int, default=1024)
+    ap.add_argument("--cpu-worker", default=None, help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-core", type=int, default=None, help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.cpu_worker is not None:
+        cpu_worker(args.cpu_worker.split(","), args.cpu_core)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); the "
               "multi-GPU layout is a design note, not a measured path", file=sys.stderr)
         sys.exit(2)
-    defaults = {"mixed-lu": (16384, 16, "1e2,1e4,1e6", 2), "mixed-spd": (32768, 4, "1e2,1e3", 1),
-                "gemm": (8192, 0, None, 10), "chol": (8192, 8, None, 5), "batched": (0, 1, None, 5),
-                "qr": (4096, 0, None, 3)}
-    dn, dr, dk, ds = defaults[args.workload]
-    args.n = args.n or dn
-    args.nrhs = args.nrhs or dr
-    args.kappas = args.kappas or dk
-    args.steps = args.steps or ds
-    # workload name of the committed HBM-traffic passes, when run at their sizes
-    tags = {"mixed-lu": "mixed_lu_n16384", "mixed-spd": "mixed_spd_n32768", "gemm": "gemm_8192",
-            "chol": "chol_8192", "batched": "batched_200k", "qr": "qr_4096"}
-    same = (args.n == dn and args.nrhs == dr and
-            (args.workload != "batched" or args.count == 200000))
-    args.profiled = tags[args.workload] if same else None
+    head = "mixed-lu" if args.workload == "all" else args.workload
+    subs = ALL_CONFIGS if args.workload == "all" else []
+    if args.workload == "all":
+        args.n = args.nrhs = args.kappas = None
+    # the CPU baselines start first, in their own pinned process, before
+    # this process touches the GPU
+    cpu = CpuBaselines([] if args.no_cpu else [head] + [w for w, _ in subs])
 
     import torch
     import libmat_amd
 
     torch.cuda.set_device(0)
-    if args.workload == "mixed-lu":
-        out = run_mixed(torch, libmat_amd, args, spd=False)
-    elif args.workload == "mixed-spd":
-        out = run_mixed(torch, libmat_amd, args, spd=True)
-    elif args.workload == "gemm":
-        out = run_gemm(torch, libmat_amd, args)
-    elif args.workload == "chol":
-        out = run_chol(torch, libmat_amd, args)
-    elif args.workload == "qr":
-        out = run_qr(torch, libmat_amd, args)
-    else:
-        out = run_batched(torch, libmat_amd, args)
+    out = run_one(torch, libmat_amd, workload_args(args, head, args.steps, args.warmup))
+    if subs:
+        out["configs"] = {}
+        for w, key in subs:
+            log(f"config {key}")
+            try:
+                out["configs"][key] = run_one(torch, libmat_amd, workload_args(args, w))
+            except Exception as e:  # the headline and the other configs still stand
+                out["configs"][key] = {"error": repr(e)}
+            torch.cuda.empty_cache()
+    base = cpu.results()
+    if base:
+        out["cpu_baseline"] = base.get(head)
+        for w, key in subs:
+            if "error" not in out["configs"][key]:
+                out["configs"][key]["cpu_baseline"] = base.get(w)
     print(json.dumps(out), flush=True)
 
 
