@@ -29,6 +29,7 @@
 //    class order by timing graph replays on the actual batch.
 #include <algorithm>
 #include <chrono>
+#include <memory>
 #include <vector>
 
 #include "mat_gpu.h"
@@ -471,6 +472,28 @@ double time_replays(hipStream_t s, Plan &pl, int reps) {
   return ms / reps;
 }
 
+// Frees whatever a plan holds (also a half-built one); errors are ignored,
+// this runs on error paths too.
+void free_plan(Plan *pl) {
+  if (!pl) return;
+  (void)hipDeviceSynchronize();
+  if (pl->exec) (void)hipGraphExecDestroy(pl->exec);
+  if (pl->graph) (void)hipGraphDestroy(pl->graph);
+  if (pl->d_order) (void)hipFree(pl->d_order);
+  if (pl->d_tasks) (void)hipFree(pl->d_tasks);
+  if (pl->d_counter) (void)hipFree(pl->d_counter);
+  if (pl->d_n) (void)hipFree(pl->d_n);
+  delete pl;
+}
+struct PlanFree {
+  void operator()(Plan *pl) const { free_plan(pl); }
+};
+
+void check_orders(size_t count, const int32_t *n) {
+  for (size_t i = 0; i < count; ++i)
+    MG_CHECK(n[i] >= 1 && n[i] <= 64, "batch: matrix order must be in [1, 64]");
+}
+
 template <typename F>
 int guarded(F &&f) {
   try {
@@ -492,8 +515,10 @@ int mat_gpu_batch_create_f32(void *stream, size_t count, const int32_t *n, const
                              mat_gpu_batch_plan **out) {
   return guarded([&] {
     MG_CHECK(out != nullptr && n != nullptr && kind != nullptr, "batch: null argument");
+    check_orders(count, n);
     hipStream_t s = (hipStream_t)stream;
-    Plan *pl = new Plan();
+    // freed on every error path below, handed to the caller on success
+    std::unique_ptr<Plan, PlanFree> pl(new Plan());
     std::vector<int> hn(n, n + count);
     MG_HIP(hipMalloc(&pl->d_n, std::max<size_t>(1, count) * sizeof(int)));
     MG_HIP(hipMemcpyAsync(pl->d_n, hn.data(), count * sizeof(int), hipMemcpyHostToDevice, s));
@@ -541,7 +566,7 @@ int mat_gpu_batch_create_f32(void *stream, size_t count, const int32_t *n, const
       capture(s, *pl);
     }
     MG_HIP(hipStreamSynchronize(s));
-    *out = reinterpret_cast<mat_gpu_batch_plan *>(pl);
+    *out = reinterpret_cast<mat_gpu_batch_plan *>(pl.release());
     return 0;
   });
 }
@@ -571,15 +596,7 @@ int mat_gpu_batch_tuning(const mat_gpu_batch_plan *plan, int *wg_per_cu, int *or
 int mat_gpu_batch_destroy(mat_gpu_batch_plan *plan) {
   return guarded([&] {
     if (!plan) return 0;
-    Plan *pl = reinterpret_cast<Plan *>(plan);
-    MG_HIP(hipDeviceSynchronize());
-    if (pl->exec) MG_HIP(hipGraphExecDestroy(pl->exec));
-    if (pl->graph) MG_HIP(hipGraphDestroy(pl->graph));
-    if (pl->d_order) MG_HIP(hipFree(pl->d_order));
-    if (pl->d_tasks) MG_HIP(hipFree(pl->d_tasks));
-    if (pl->d_counter) MG_HIP(hipFree(pl->d_counter));
-    if (pl->d_n) MG_HIP(hipFree(pl->d_n));
-    delete pl;
+    free_plan(reinterpret_cast<Plan *>(plan));
     return 0;
   });
 }
@@ -589,6 +606,8 @@ int mat_gpu_batch_solve_f32(void *stream, size_t count, const int32_t *n, const 
                             const float *a, const int64_t *a_off, const float *b, float *x,
                             const int64_t *v_off, int32_t *info) {
   return guarded([&] {
+    MG_CHECK(n != nullptr && kind != nullptr, "batch: null argument");
+    check_orders(count, n);
     hipStream_t s = (hipStream_t)stream;
     mg::Workspace &ws = mg::workspace(s);
     std::vector<int> order;

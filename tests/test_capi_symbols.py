@@ -135,3 +135,25 @@ def test_geqrf_rejects_bad_shapes(prec):
     for m, n, lda, what in ((2, 3, 2, b"m >= n"), (3, 0, 3, b"m >= n >= 1"), (4, 2, 3, b"lda")):
         assert fn(None, m, n, None, lda, None) < 0, (m, n, lda)
         assert what in g.lib.mat_gpu_last_error(), (m, n, lda)
+
+
+def test_batch_rejects_bad_orders():
+    """The batched API checks every matrix order (1..64) on the host before
+    it allocates or touches the device: a plan with an order outside the
+    range returns a negative code with a message and hands out no plan
+    (no GPU needed)."""
+    g = libmat_amd.load("f32")
+    g.lib.mat_gpu_last_error.restype = C.c_char_p
+    for bad in (0, 65):
+        n = np.array([3, bad, 8], np.int32)
+        kind = np.zeros(3, np.int8)
+        plan = C.c_void_p(0)
+        rc = g.lib.mat_gpu_batch_create_f32(None, C.c_size_t(3), n.ctypes.data_as(C.c_void_p),
+                                            kind.ctypes.data_as(C.c_void_p), None, None, None,
+                                            None, None, None, 0, C.byref(plan))
+        assert rc < 0 and plan.value is None, bad
+        assert b"matrix order" in g.lib.mat_gpu_last_error()
+        rc = g.lib.mat_gpu_batch_solve_f32(None, C.c_size_t(3), n.ctypes.data_as(C.c_void_p),
+                                           kind.ctypes.data_as(C.c_void_p), None, None, None,
+                                           None, None, None)
+        assert rc < 0 and b"matrix order" in g.lib.mat_gpu_last_error()
