@@ -285,6 +285,20 @@ int mat_gpu_batch_destroy(mat_gpu_batch_plan *plan);
 int mat_gpu_batch_solve_f32(void *stream, size_t count, const int32_t *n, const int8_t *kind,
                             const float *a, const int64_t *a_off, const float *b, float *x,
                             const int64_t *v_off, int32_t *info);
+/* mat_gpu_batch_solve_f32 that allocates nothing (no counterpart in libmat):
+ * for callers that cannot allocate memory, such as serving loops that
+ * forbid it.  Its scratch (the orders, the task queue and the per-class
+ * counters) lives in one fixed array of about 6 MiB per GPU, held until the
+ * process ends, so a call takes at most 262144 matrices (larger batches:
+ * a negative code and a message).  Same kernels and results as
+ * mat_gpu_batch_solve_f32.  Every call on one GPU shares that array: the
+ * library serializes calls within a process (a lock held for the whole,
+ * synchronous call); nothing stops two processes from sharing a GPU, each
+ * with its own array.  Synchronous. */
+int mat_gpu_batch_solve_noalloc_f32(void *stream, size_t count, const int32_t *n,
+                                    const int8_t *kind, const float *a, const int64_t *a_off,
+                                    const float *b, float *x, const int64_t *v_off,
+                                    int32_t *info);
 
 #ifdef __cplusplus
 }

@@ -44,6 +44,7 @@ EXT_SYMBOLS = [
     "mat_gpu_posv_f32", "mat_gpu_posv_f64", "mat_gpu_ir_defaults", "mat_gpu_gesv_mixed",
     "mat_gpu_posv_mixed", "mat_gpu_batch_create_f32", "mat_gpu_batch_run",
     "mat_gpu_batch_tuning", "mat_gpu_batch_destroy", "mat_gpu_batch_solve_f32",
+    "mat_gpu_batch_solve_noalloc_f32",
 ]
 
 
@@ -139,6 +140,7 @@ class GpuLib:
         s("mat_gpu_batch_tuning", i32, [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(dbl)])
         s("mat_gpu_batch_destroy", i32, [vp])
         s("mat_gpu_batch_solve_f32", i32, [vp, sz, vp, vp, vp, vp, vp, vp, vp, vp])
+        s("mat_gpu_batch_solve_noalloc_f32", i32, [vp, sz, vp, vp, vp, vp, vp, vp, vp, vp])
 
     def last_error(self) -> str:
         return self.lib.mat_gpu_last_error().decode()

@@ -30,6 +30,8 @@
 #include <algorithm>
 #include <chrono>
 #include <memory>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "mat_gpu.h"
@@ -340,6 +342,23 @@ __attribute__((amdgpu_waves_per_eu(ClassShape<NB, KIND>::MINW))) void batched_ke
 
 }  // namespace mg
 
+// ------------------------------------------------ no-allocation scratch --
+// Device scratch of mat_gpu_batch_solve_noalloc_f32: the batch's orders,
+// its matrices in task order, the task queue and the per-class counters,
+// for at most NOALLOC_MAX matrices.  Each GPU holds its own copy for the
+// life of the process; host code reaches it only through the address
+// hipGetSymbolAddress returns for the current device.
+namespace {
+constexpr size_t NOALLOC_MAX = size_t(1) << 18;
+constexpr size_t round256(size_t x) { return (x + 255) / 256 * 256; }
+constexpr size_t NA_N = 0;
+constexpr size_t NA_ORDER = round256(NA_N + NOALLOC_MAX * sizeof(int));
+constexpr size_t NA_TASKS = round256(NA_ORDER + NOALLOC_MAX * sizeof(int));
+constexpr size_t NA_COUNTER = round256(NA_TASKS + NOALLOC_MAX * sizeof(mg::BatchTask));
+constexpr size_t NA_BYTES = round256(NA_COUNTER + mg::NCLASS * sizeof(unsigned));
+}  // namespace
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
 // ------------------------------------------------------------- plan API --
 namespace {
 
@@ -353,7 +372,8 @@ struct Plan {
   int *d_n = nullptr;
   int wg_per_cu = 4;
   int cls_t0[mg::NCLASS] = {}, cls_t1[mg::NCLASS] = {};
-  std::vector<int> cls_order;  // classes with tasks, in launch order
+  int cls_order[mg::NCLASS] = {};  // classes with tasks, in launch order
+  int ncls = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   double tuned_ms = 0;
@@ -364,37 +384,54 @@ struct Plan {
 int bin_of(int n) { return n <= 8 ? 0 : (n <= 16 ? 1 : (n <= 32 ? 2 : 3)); }
 
 // Build tasks: bins (NB, kind) -> chunks of 64/NB matrices.  order_mode 0:
-// largest bins first; 1: smallest first.
-void make_tasks(size_t count, const int32_t *n, const int8_t *kind, int order_mode,
-                std::vector<int> &order, std::vector<BatchTask> &tasks, Plan &pl) {
-  std::vector<int> idx[4][2];
+// largest bins first; 1: smallest first.  Within a class the matrices keep
+// their batch order.  Writes order[count] and tasks[<= count] (a counting
+// sort: no allocation) and returns the number of tasks.
+int make_tasks_into(size_t count, const int32_t *n, const int8_t *kind, int order_mode,
+                    int *order, BatchTask *tasks, Plan &pl) {
+  size_t cnt[mg::NCLASS] = {}, off[mg::NCLASS] = {}, pos[mg::NCLASS] = {};
   for (size_t i = 0; i < count; ++i) {
     MG_CHECK(n[i] >= 1 && n[i] <= 64, "batch: matrix order must be in [1, 64]");
-    idx[bin_of(n[i])][kind[i] ? 1 : 0].push_back((int)i);
+    ++cnt[mg::class_of(bin_of(n[i]), kind[i] ? 1 : 0)];
   }
-  order.clear();
-  tasks.clear();
-  pl.cls_order.clear();
+  size_t o = 0;
   for (int bi = 0; bi < 4; ++bi) {
     const int b = order_mode == 0 ? 3 - bi : bi;
-    const int per = 64 / (8 << b);
     for (int k = 0; k < 2; ++k) {
-      auto &v = idx[b][k];
       const int c = mg::class_of(b, k);
-      pl.cls_t0[c] = (int)tasks.size();
-      if (!v.empty()) pl.cls_order.push_back(c);
-      for (size_t s = 0; s < v.size(); s += per) {
+      off[c] = pos[c] = o;
+      o += cnt[c];
+    }
+  }
+  for (size_t i = 0; i < count; ++i) order[pos[mg::class_of(bin_of(n[i]), kind[i] ? 1 : 0)]++] = (int)i;
+  int nt = 0;
+  pl.ncls = 0;
+  for (int bi = 0; bi < 4; ++bi) {
+    const int b = order_mode == 0 ? 3 - bi : bi;
+    const size_t per = 64 / (8 << b);
+    for (int k = 0; k < 2; ++k) {
+      const int c = mg::class_of(b, k);
+      pl.cls_t0[c] = nt;
+      if (cnt[c]) pl.cls_order[pl.ncls++] = c;
+      for (size_t s = 0; s < cnt[c]; s += per) {
         BatchTask t;
         t.bin = b;
         t.kind = k;
-        t.first = (int)order.size();
-        t.count = (int)std::min<size_t>(per, v.size() - s);
-        for (int q = 0; q < t.count; ++q) order.push_back(v[s + q]);
-        tasks.push_back(t);
+        t.first = (int)(off[c] + s);
+        t.count = (int)std::min(per, cnt[c] - s);
+        tasks[nt++] = t;
       }
-      pl.cls_t1[c] = (int)tasks.size();
+      pl.cls_t1[c] = nt;
     }
   }
+  return nt;
+}
+
+void make_tasks(size_t count, const int32_t *n, const int8_t *kind, int order_mode,
+                std::vector<int> &order, std::vector<BatchTask> &tasks, Plan &pl) {
+  order.resize(count);
+  tasks.resize(count);
+  tasks.resize(make_tasks_into(count, n, kind, order_mode, order.data(), tasks.data(), pl));
 }
 
 void upload_plan(hipStream_t s, Plan &pl, const std::vector<int> &order,
@@ -439,7 +476,7 @@ void launch_one(hipStream_t s, const Plan &pl, int c) {
 // The class kernels in launch order, after the counters' reset.
 void launch(hipStream_t s, Plan &pl) {
   mg::fill_u32(s, pl.d_counter, mg::NCLASS, 0u);
-  for (int c : pl.cls_order) launch_one(s, pl, c);
+  for (int i = 0; i < pl.ncls; ++i) launch_one(s, pl, pl.cls_order[i]);
 }
 
 void capture(hipStream_t s, Plan &pl) {
@@ -601,6 +638,31 @@ int mat_gpu_batch_destroy(mat_gpu_batch_plan *plan) {
   });
 }
 
+// The no-allocation entry point's state: the scratch array's device
+// address per device (resolved once per device, with it current), and host
+// staging of the order and the task queue.  One lock covers all of it and
+// is held for a whole call, so calls from several threads take turns.
+std::mutex g_na_mu;
+constexpr int NA_DEVICES = 64;
+unsigned char *g_na_ptr[NA_DEVICES] = {};
+int g_na_order[NOALLOC_MAX];
+mg::BatchTask g_na_tasks[NOALLOC_MAX];
+
+// caller holds g_na_mu
+unsigned char *noalloc_scratch() {
+  int dev = 0;
+  MG_HIP(hipGetDevice(&dev));
+  MG_CHECK(dev >= 0 && dev < NA_DEVICES, "batch noalloc: device index out of range");
+  if (!g_na_ptr[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    MG_CHECK(e == hipSuccess && p != nullptr,
+             std::string("batch noalloc: hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_na_ptr[dev] = (unsigned char *)p;
+  }
+  return g_na_ptr[dev];
+}
+
 // One launch without a graph (used for per-matrix launch loops too).
 int mat_gpu_batch_solve_f32(void *stream, size_t count, const int32_t *n, const int8_t *kind,
                             const float *a, const int64_t *a_off, const float *b, float *x,
@@ -639,6 +701,55 @@ int mat_gpu_batch_solve_f32(void *stream, size_t count, const int32_t *n, const 
     pl.wg_per_cu = 4;
     launch(s, pl);
     // host vectors must outlive the async copies
+    MG_HIP(hipStreamSynchronize(s));
+    return 0;
+  });
+}
+
+// mat_gpu_batch_solve_f32 with its scratch (orders, task queue, counters)
+// in the fixed per-GPU array instead of library workspace: nothing is
+// allocated, on the device or the host.  Same kernels and launch shape,
+// so the results are the same bits.
+int mat_gpu_batch_solve_noalloc_f32(void *stream, size_t count, const int32_t *n,
+                                    const int8_t *kind, const float *a, const int64_t *a_off,
+                                    const float *b, float *x, const int64_t *v_off,
+                                    int32_t *info) {
+  return guarded([&] {
+    MG_CHECK(n != nullptr && kind != nullptr, "batch: null argument");
+    MG_CHECK(count <= NOALLOC_MAX,
+             "batch noalloc: at most 262144 matrices per call (the size of the fixed scratch "
+             "array); use mat_gpu_batch_solve_f32 or a plan for larger batches");
+    check_orders(count, n);
+    hipStream_t s = (hipStream_t)stream;
+    std::lock_guard<std::mutex> lk(g_na_mu);
+    unsigned char *base = noalloc_scratch();
+    Plan pl;
+    const int nt = make_tasks_into(count, n, kind, 0, g_na_order, g_na_tasks, pl);
+    int *d_n = (int *)(base + NA_N);
+    int *d_order = (int *)(base + NA_ORDER);
+    BatchTask *d_tasks = (BatchTask *)(base + NA_TASKS);
+    unsigned *d_counter = (unsigned *)(base + NA_COUNTER);
+    if (count) {
+      MG_HIP(hipMemcpyAsync(d_n, n, count * sizeof(int), hipMemcpyHostToDevice, s));
+      MG_HIP(hipMemcpyAsync(d_order, g_na_order, count * sizeof(int), hipMemcpyHostToDevice, s));
+      MG_HIP(hipMemcpyAsync(d_tasks, g_na_tasks, (size_t)nt * sizeof(BatchTask),
+                            hipMemcpyHostToDevice, s));
+    }
+    pl.p.A = a;
+    pl.p.aoff = (const long long *)a_off;
+    pl.p.b = b;
+    pl.p.x = x;
+    pl.p.voff = (const long long *)v_off;
+    pl.p.n = d_n;
+    pl.p.info = info;
+    pl.p.order = d_order;
+    pl.p.tasks = d_tasks;
+    pl.p.counter = d_counter;
+    pl.p.eps = 1e-6f;
+    pl.d_counter = d_counter;
+    pl.wg_per_cu = 4;
+    launch(s, pl);  // the counters are cleared by a fill kernel first
+    // the staging arrays and n are read by the copies until here
     MG_HIP(hipStreamSynchronize(s));
     return 0;
   });
