@@ -424,6 +424,8 @@ def run_mixed(torch, libmat_amd, args, spd):
         opts.passes = args.passes
     if args.pivoting is not None and not spd:
         opts.pivoting = args.pivoting
+    if args.leaf is not None:
+        opts.leaf = args.leaf
     status = [libmat_amd.IrStatus() for _ in kappas]
     solve = lib.mat_gpu_posv_mixed if spd else lib.mat_gpu_gesv_mixed
     plain = lib.mat_gpu_posv_f64 if spd else lib.mat_gpu_gesv_f64
@@ -897,6 +899,8 @@ def main():
     ap.add_argument("--passes", type=int, default=None)
     ap.add_argument("--pivoting", type=int, default=None, choices=[0, 1, 2],
                     help="mixed LU: 0 partial, 1 butterfly, 2 tournament (library default if unset)")
+    ap.add_argument("--leaf", type=int, default=None,
+                    help="mixed solvers: LU pivoting block width (library default if unset)")
     ap.add_argument("--count", type=int, default=200000)
     ap.add_argument("--loop-sample", type=int, default=2000)
     ap.add_argument("--no-plain", action="store_true")

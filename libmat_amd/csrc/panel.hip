@@ -30,6 +30,14 @@
 // At the end, workgroup 0 turns the leaf's interchanges into row pairs that
 // every later row move of this leaf reuses.  Every poll is bounded and
 // reports a timeout instead of hanging.
+//
+// Two variants of the exchange (panel_exchange() picks per type):
+//   * LOCAL: every workgroup of the launch on one XCD (8x the workgroups,
+//     every eighth takes part, dispatch being round-robin over the XCDs);
+//     the words are stored at workgroup scope and stay in that XCD's L2,
+//     where the other workgroups' agent-scope polls find them;
+//   * panel_gepp_la_kernel: the rank-1 update of a step runs while the next
+//     step's keys travel; only the next column is updated first.
 #include <algorithm>
 #include <atomic>
 #include <cstdint>
@@ -230,7 +238,32 @@ __device__ __forceinline__ void lds_load_row(const T *src, T (&v)[N]) {
   }
 }
 
-template <typename T, int WMAX, int RPT>
+// Exchange stores.  Across the whole device the words are written at agent
+// scope (through to memory, where every XCD's loads find them).  With all
+// of a launch's workgroups on one XCD (LOCAL), they are written at
+// workgroup scope and stay in that XCD's L2, where the agent-scope loads of
+// the other workgroups on the same XCD find them (measured on MI355X: a
+// store/poll round trip of ~0.57 us instead of ~1.07 us).
+template <bool LOCAL>
+__device__ __forceinline__ void xstore(uint64_t *p, uint64_t v) {
+  if constexpr (LOCAL) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  else __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup i on
+// XCD i mod 8).  A LOCAL launch has 8x the workgroups and only every
+// eighth takes part, so all of them share XCD 0 and its L2.
+constexpr int NXCD = 8;
+template <bool LOCAL>
+__device__ __forceinline__ int panel_wg() {
+  return LOCAL ? (int)(blockIdx.x / NXCD) : (int)blockIdx.x;
+}
+template <bool LOCAL>
+__device__ __forceinline__ bool panel_idle() {
+  return LOCAL && (blockIdx.x % NXCD) != 0;
+}
+
+template <typename T, int WMAX, int RPT, bool LOCAL>
 __global__ __launch_bounds__(PanelNT<T>::value) void panel_gepp_kernel(PanelP<T> p) {
   constexpr int PANEL_NT = PanelNT<T>::value;
   constexpr int PANEL_NW = PANEL_NT / 64;
@@ -241,8 +274,9 @@ __global__ __launch_bounds__(PanelNT<T>::value) void panel_gepp_kernel(PanelP<T>
   constexpr int ROW0 = 4;
   constexpr int REC = ROW0 + WMAX * NWD;
   constexpr int KREC = WMAX * NWD;
+  if (panel_idle<LOCAL>()) return;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const int b = blockIdx.x;
+  const int b = panel_wg<LOCAL>();
   const int P = p.nwg;
   const int rend = p.r0 + p.m;
   const int w = p.w;
@@ -353,19 +387,16 @@ __global__ __launch_bounds__(PanelNT<T>::value) void panel_gepp_kernel(PanelP<T>
           const int gw = gm ? __builtin_ctzll(gm) : 0;
           uint64_t *mine = p.rec + ((long)par * P + b) * REC;
           if (lane < KW)
-            __hip_atomic_store(mine + lane, tagged(tag, gkey.word(lane)), __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
+            xstore<LOCAL>(mine + lane, tagged(tag, gkey.word(lane)));
           for (int x = lane; x < WMAX * NWD; x += 64) {
             const T v = wrow[par][gw][x / NWD];
-            __hip_atomic_store(mine + ROW0 + x, tagged(tag, word_of<T>(v, x % NWD)),
-                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            xstore<LOCAL>(mine + ROW0 + x, tagged(tag, word_of<T>(v, x % NWD)));
           }
           const bool own_k = (k - p.r0) / RPW == b;
           if (own_k)
             for (int x = lane; x < KREC; x += 64) {
               const T v = krow[par][x / NWD];
-              __hip_atomic_store(p.krec + (long)par * KREC + x, tagged(tag, word_of<T>(v, x % NWD)),
-                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              xstore<LOCAL>(p.krec + (long)par * KREC + x, tagged(tag, word_of<T>(v, x % NWD)));
             }
           bool timeout = false;
           // phase 1: lane x < P reads workgroup x's key words
@@ -553,6 +584,349 @@ __global__ __launch_bounds__(PanelNT<T>::value) void panel_gepp_kernel(PanelP<T>
   }
 }
 
+constexpr int WREC_MAX = 256;  // sizing of the exchange record buffer (64 workgroups x 4)
+
+// ---------------------------------------------------------------------------
+// panel_gepp_kernel with the rank-1 update taken off the column step's
+// critical path (P > 1 workgroups).  In panel_gepp_kernel every thread
+// updates all its unfinished columns before the next pivot search, and the
+// exchange then waits on memory with the update already paid.  Here, once
+// the pivot row is in, only the next column is updated (the eager column),
+// the next search runs on it, and the full fused update and rotation of
+// the step (the pending update) run while the next step's keys travel:
+//   1. every wavefront finds its candidate on the eager column and parks
+//      it, and row k, in LDS as they are in registers (before the pending
+//      update); barrier;
+//   2. wavefront 0 merges, publishes the key and the winning row and (from
+//      the workgroup holding it) row k, each word brought up to date on the
+//      way out: the same fma with the same operands as the register update
+//      (one per word instead of one per element of every row);
+//   3. every wavefront applies the pending update to its registers;
+//   4. wavefront 0 polls the keys, fetches the winning row and row k;
+//      barrier; swap, scale, eager column.
+// The results are the same bits as panel_gepp_kernel's.
+template <typename T, int WMAX, int RPT, bool LOCAL>
+__global__ __launch_bounds__(PanelNT<T>::value) void panel_gepp_la_kernel(PanelP<T> p) {
+  constexpr int PANEL_NT = PanelNT<T>::value;
+  constexpr int PANEL_NW = PANEL_NT / 64;
+  constexpr int RPW = PANEL_NT * RPT;
+  constexpr int NWD = Words<T>::N;
+  constexpr int KW = PivKey<T>::NW;
+  constexpr int ROW0 = 4;
+  constexpr int REC = ROW0 + WMAX * NWD;
+  constexpr int KREC = WMAX * NWD;
+  if (panel_idle<LOCAL>()) return;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int b = panel_wg<LOCAL>();
+  const int P = p.nwg;
+  const int rend = p.r0 + p.m;
+  const int w = p.w;
+
+  __shared__ __align__(16) T wrow[2][PANEL_NW][WMAX];  // candidates, not yet updated
+  __shared__ __align__(16) PivKey<T> wkey[2][PANEL_NW];
+  __shared__ __align__(16) T krow[2][WMAX];            // row k, not yet updated
+  __shared__ __align__(16) uint32_t stage[2][2 * WMAX * NWD];  // pivot row, row k
+  __shared__ int s_piv[2];
+  __shared__ int s_hist[WMAX];
+  __shared__ int s_abort;
+
+  T a[RPT][WMAX];
+  int row[RPT];
+#pragma unroll
+  for (int q = 0; q < RPT; ++q) {
+    row[q] = p.r0 + b * RPW + q * PANEL_NT + t;
+    const T *src = p.W + (long)row[q] * p.ld + p.c0;
+#pragma unroll
+    for (int c = 0; c < WMAX; ++c) a[q][c] = (row[q] < rend && c < w) ? src[c] : T(0);
+  }
+  if (t == 0) s_abort = 0;
+
+  // the pending update: 0 none, 1 rotation only (pivot below eps), 2 scale
+  // and update; its pivot row is stage[ppar] (row k's half if pk)
+  int pend = 0, ppar = 0, prem = 0, pk = 0, kprev = 0;
+  T pinv = T(0);
+  T nxt[RPT];
+#pragma unroll
+  for (int q = 0; q < RPT; ++q) nxt[q] = a[q][0];
+
+  auto pend_row = [&]() -> const T * {
+    return reinterpret_cast<const T *>(&stage[ppar][pk ? WMAX * NWD : 0]);
+  };
+  // the pending update of the registers, fused with the rotation
+  auto apply_pending = [&]() {
+    if (pend == 2) {
+      T pra[WMAX];
+      lds_load_row<T, WMAX>(pend_row(), pra);
+      bool below[RPT], all_below = true;
+      T lq[RPT], a0[RPT];
+#pragma unroll
+      for (int q = 0; q < RPT; ++q) {
+        below[q] = row[q] > kprev && row[q] < rend;
+        all_below = all_below && below[q];
+        const T l = a[q][0] * pinv;
+        a0[q] = below[q] ? l : a[q][0];
+        lq[q] = l;
+      }
+      if (__all(all_below)) {
+#pragma unroll
+        for (int c = 1; c < WMAX; ++c) {
+          const bool live = c < prem;
+          const T prc = pra[c];
+#pragma unroll
+          for (int q = 0; q < RPT; ++q) {
+            const T u = fma(-prc, lq[q], a[q][c]);
+            a[q][c - 1] = live ? u : a[q][c];
+          }
+        }
+      } else {
+#pragma unroll
+        for (int c = 1; c < WMAX; ++c) {
+          const bool live = c < prem;
+          const T prc = pra[c];
+#pragma unroll
+          for (int q = 0; q < RPT; ++q) {
+            const T u = fma(-prc, lq[q], a[q][c]);
+            a[q][c - 1] = (live && below[q]) ? u : a[q][c];
+          }
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < RPT; ++q) a[q][WMAX - 1] = a0[q];
+    } else if (pend == 1) {
+#pragma unroll
+      for (int q = 0; q < RPT; ++q) {
+        const T f = a[q][0];
+#pragma unroll
+        for (int c = 0; c + 1 < WMAX; ++c) a[q][c] = a[q][c + 1];
+        a[q][WMAX - 1] = f;
+      }
+    }
+    pend = 0;
+  };
+  // element v of a parked row (row index r) after the pending update: the
+  // same operations as apply_pending on that row
+  auto fixed = [&](const T *S, int v, int r) -> T {
+    if (pend == 0) return S[v];
+    if (pend == 1) return v < WMAX - 1 ? S[v + 1] : S[0];
+    const bool below = r > kprev && r < rend;
+    const T l = S[0] * pinv;
+    if (v == WMAX - 1) return below ? l : S[0];
+    const int c = v + 1;
+    return (c < prem && below) ? fma(-pend_row()[c], l, S[c]) : S[c];
+  };
+
+  int step = 0;
+  for (int j = 0; j < w; ++j) {
+    const int k = p.c0 + j;
+    if (k < p.kmax) {
+      const int par = step & 1;
+      // 1. wavefront candidate on the eager column; park it and row k as
+      //    they are (the pending update is applied on the way out)
+      PivKey<T> key;
+      key.clear();
+      int wbest = -1;
+      if constexpr (sizeof(T) == 4) {
+        float tv = -1.0f;
+        int tq = -1;
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+          const float av = fabsf(nxt[q]);
+          if (row[q] < rend && row[q] >= k && av > tv) {  // NaN never wins
+            tv = av;
+            tq = q;
+          }
+        }
+        const float mx = wave_max_f32(tv);
+        if (mx >= 0.0f) {
+#pragma unroll
+          for (int q = RPT - 1; q >= 0; --q) {
+            const unsigned long long msk = __ballot(tq == q && tv == mx);
+            if (msk) wbest = p.r0 + b * RPW + q * PANEL_NT + wave * 64 + __builtin_ctzll(msk);
+          }
+          key.offer(mx, wbest);
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < RPT; ++q)
+          if (row[q] < rend && row[q] >= k) key.offer(nxt[q], row[q]);
+        key.wave_reduce();
+        wbest = key.valid() ? key.row() : -1;
+      }
+      if (lane == 0) wkey[par][wave] = key;
+#pragma unroll
+      for (int q = 0; q < RPT; ++q) {
+        if (row[q] == wbest) lds_store_row<T, WMAX>(wrow[par][wave], a[q]);
+        if (row[q] == k) lds_store_row<T, WMAX>(krow[par], a[q]);
+      }
+      __syncthreads();
+      const unsigned tag = p.epoch * 64u + (unsigned)step + 1u;
+      bool timeout = false;
+      int gpiv = k, cw = 0;
+      if (wave == 0) {
+        // 2. workgroup candidate; publish key, winning row, row k
+        PivKey<T> mk;
+        mk.clear();
+        if (lane < PANEL_NW) mk = wkey[par][lane];
+        PivKey<T> gkey = mk;
+        gkey.wave_reduce();
+        const unsigned long long gm =
+            __ballot(lane < PANEL_NW && mk.valid() && gkey.valid() && mk.row() == gkey.row());
+        const int gw = gm ? __builtin_ctzll(gm) : 0;
+        uint64_t *mine = p.rec + ((long)par * P + b) * REC;
+        if (lane < KW) xstore<LOCAL>(mine + lane, tagged(tag, gkey.word(lane)));
+        if (gkey.valid()) {
+          const int cr = gkey.row();
+          for (int x = lane; x < WMAX * NWD; x += 64)
+            xstore<LOCAL>(mine + ROW0 + x, tagged(tag, word_of<T>(fixed(wrow[par][gw], x / NWD, cr),
+                                                                 x % NWD)));
+        }
+        if ((k - p.r0) / RPW == b)
+          for (int x = lane; x < KREC; x += 64)
+            xstore<LOCAL>(p.krec + (long)par * KREC + x,
+                          tagged(tag, word_of<T>(fixed(krow[par], x / NWD, k), x % NWD)));
+      }
+      // 3. the pending update, while the keys travel
+      apply_pending();
+      if (wave == 0) {
+        // 4. poll the P keys, then fetch the winning row and row k
+        PivKey<T> ck;
+        ck.clear();
+        if (lane < P) {
+          const uint64_t *rx = p.rec + ((long)par * P + lane) * REC;
+          uint64_t wd[KW];
+#pragma unroll
+          for (int h = 0; h < KW; ++h)
+            wd[h] = __hip_atomic_load(rx + h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+          for (int h = 0; h < KW; ++h) {
+            long spins = 0;
+            while ((wd[h] >> 32) != tag) {
+              if (++spins > SPIN_LIMIT) {
+                timeout = true;
+                break;
+              }
+              __builtin_amdgcn_s_sleep(1);
+              wd[h] = __hip_atomic_load(rx + h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            ck.set_word(h, (uint32_t)wd[h]);
+          }
+        }
+        PivKey<T> best = ck;
+        best.wave_reduce();
+        gpiv = best.valid() ? best.row() : k;
+        const bool is_win = lane < P && ck.valid() && best.valid() && ck.row() == gpiv;
+        const unsigned long long wm = __ballot(is_win);
+        cw = wm ? __builtin_ctzll(wm) : 0;
+        const uint64_t *wrec = p.rec + ((long)par * P + cw) * REC + ROW0;
+#pragma unroll
+        for (int u = 0; u < (2 * WMAX * NWD + 63) / 64; ++u) {
+          const int x = lane + 64 * u;
+          if (x < 2 * WMAX * NWD) {
+            const bool is_k = x >= WMAX * NWD;
+            const int xx = is_k ? x - WMAX * NWD : x;
+            if (is_k || gpiv != k) {
+              const uint64_t *addr = is_k ? p.krec + (long)par * KREC + xx : wrec + xx;
+              uint64_t wd = __hip_atomic_load(addr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              long spins = 0;
+              while ((wd >> 32) != tag) {
+                if (++spins > SPIN_LIMIT) {
+                  timeout = true;
+                  break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+                wd = __hip_atomic_load(addr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              }
+              stage[par][x] = (uint32_t)wd;
+            }
+          }
+        }
+        if (__any(timeout) && lane == 0) {
+          s_abort = 1;
+          __hip_atomic_store(p.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (lane == 0) s_piv[par] = gpiv;
+      }
+      __syncthreads();
+      if (s_abort) return;
+      const int piv = s_piv[par];
+      if (b == 0 && t == 0) p.ipiv[k] = piv;
+      if (t == 0) s_hist[j] = piv;
+      // 5. swap; scale and the eager column; the rest becomes pending
+      const T *dr = reinterpret_cast<const T *>(&stage[par][WMAX * NWD]);
+      const T *pr = piv == k ? dr : reinterpret_cast<const T *>(&stage[par][0]);
+      constexpr bool PRA_FIRST = !(sizeof(T) == 8 && WMAX == 64);
+      T pra[WMAX];
+      if constexpr (PRA_FIRST) lds_load_row<T, WMAX>(pr, pra);
+      if (piv != k) {
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+          if (row[q] == k) {
+            if constexpr (PRA_FIRST) {
+#pragma unroll
+              for (int c = 0; c < WMAX; ++c) a[q][c] = pra[c];
+            } else {
+              lds_load_row<T, WMAX>(pr, a[q]);
+            }
+          } else if (row[q] == piv) {
+            lds_load_row<T, WMAX>(dr, a[q]);
+          }
+        }
+      }
+      const T pv = PRA_FIRST ? pra[0] : pr[0];
+      const T pr1 = PRA_FIRST ? pra[1] : pr[1];
+      ppar = par;
+      pk = piv == k ? 1 : 0;
+      kprev = k;
+      if (!((T)fabs(pv) < p.eps)) {
+        const T inv = T(1) / pv;
+        pinv = inv;
+        prem = w - j;
+        pend = 2;
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+          const bool below = row[q] > k && row[q] < rend;
+          const T l = a[q][0] * inv;
+          nxt[q] = (1 < prem && below) ? fma(-pr1, l, a[q][1]) : a[q][1];
+        }
+      } else {
+        pend = 1;
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) nxt[q] = a[q][1];
+      }
+      ++step;
+    } else {
+      // column not pivoted (k >= n-1): the pending update, then rotate only
+      apply_pending();
+      if (t == 0) s_hist[j] = k;
+#pragma unroll
+      for (int q = 0; q < RPT; ++q) {
+        const T f = a[q][0];
+#pragma unroll
+        for (int c = 0; c + 1 < WMAX; ++c) a[q][c] = a[q][c + 1];
+        a[q][WMAX - 1] = f;
+        nxt[q] = a[q][0];
+      }
+    }
+  }
+  apply_pending();
+#pragma unroll
+  for (int q = 0; q < RPT; ++q) {
+    if (row[q] < rend) {
+      T *dst = p.W + (long)row[q] * p.ld + p.c0;
+#pragma unroll
+      for (int sl = 0; sl < WMAX; ++sl) {
+        int c = sl + w;
+        if (c >= WMAX) c -= WMAX;
+        if (c < w) dst[c] = a[q][sl];
+      }
+    }
+  }
+  if (b == 0) {
+    __syncthreads();
+    if (wave == 0) swap_pairs(lane < w ? s_hist[lane] : 0, p.c0, w, p.pairs);
+  }
+}
+
 // Apply a leaf's interchanges, given as row pairs by the panel kernel, to
 // the columns [cl, ch) outside the leaf's own columns: each thread moves
 // one column's touched entries, all reads first, then all writes, so the
@@ -600,6 +974,36 @@ __global__ __launch_bounds__(256) void rowswap_kernel(T *W, long ld, int cl, int
 }
 
 
+// Exchange of the multi-workgroup panel, bits: 4 = the update off the
+// critical path (panel_gepp_la_kernel) instead of panel_gepp_kernel; 2 = on
+// one XCD through its L2 (launches of at most panel_local_max() workgroups
+// that fit there at once).  Measured at n = 16384 (mixed solver's fp32 LU,
+// plain fp64 LU): fp32 per 32-column leaf 133.7 us (0), 125.8 (2), 128.9
+// (4), 128.3 (6); plain fp64 solve 519 ms (0), 512 (2), 499 (4, 6).
+// Defaults: 2 for float, 4 for double; MATGPU_PANEL_EXCHANGE overrides.
+template <typename T>
+static int panel_exchange() {
+  static const int v = [] {
+    const char *e = getenv("MATGPU_PANEL_EXCHANGE");
+    return e ? atoi(e) : (sizeof(T) == 4 ? 2 : 4);
+  }();
+  return v;
+}
+static int occupancy(const void *kernel, int threads) {
+  int n = 0;
+  MG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, threads, 0));
+  return n;
+}
+static int panel_local_max() {
+  static const int v = [] {
+    // one workgroup per CU of the XCD: two per CU slowed the fp64 solve
+    // from 512 to 650 ms
+    const char *e = getenv("MATGPU_PANEL_LOCAL_MAX");
+    return e ? atoi(e) : 32;
+  }();
+  return v;
+}
+
 template <typename T, int WMAX, int RPT>
 static void launch_panel(hipStream_t s, PanelP<T> &p) {
   constexpr int PANEL_NT = PanelNT<T>::value;
@@ -609,7 +1013,24 @@ static void launch_panel(hipStream_t s, PanelP<T> &p) {
   // device (one per CU), which the exchange needs; a launch that could not
   // be resident would end in the bounded poll's timeout, not in a hang.
   KScope ks(s, "panel", 2.0 * p.m * p.w * sizeof(T));
-  hipLaunchKernelGGL((panel_gepp_kernel<T, WMAX, RPT>), dim3(p.nwg), dim3(PANEL_NT), 0, s, p);
+  const bool la = p.nwg > 1 && (panel_exchange<T>() & 4);
+  // the exchange needs every workgroup resident at once: on one XCD that
+  // is its CUs times the workgroups one CU holds at this kernel's register use
+  static const int per_cu[2] = {
+      occupancy((const void *)panel_gepp_kernel<T, WMAX, RPT, true>, PANEL_NT),
+      occupancy((const void *)panel_gepp_la_kernel<T, WMAX, RPT, true>, PANEL_NT)};
+  const int fit = per_cu[la ? 1 : 0] * (num_cus() / NXCD);
+  const bool local =
+      p.nwg > 1 && (panel_exchange<T>() & 2) && p.nwg <= std::min(fit, panel_local_max());
+  const dim3 grid(local ? p.nwg * NXCD : p.nwg);
+  if (la && local)
+    hipLaunchKernelGGL((panel_gepp_la_kernel<T, WMAX, RPT, true>), grid, dim3(PANEL_NT), 0, s, p);
+  else if (la)
+    hipLaunchKernelGGL((panel_gepp_la_kernel<T, WMAX, RPT, false>), grid, dim3(PANEL_NT), 0, s, p);
+  else if (local)
+    hipLaunchKernelGGL((panel_gepp_kernel<T, WMAX, RPT, true>), grid, dim3(PANEL_NT), 0, s, p);
+  else
+    hipLaunchKernelGGL((panel_gepp_kernel<T, WMAX, RPT, false>), grid, dim3(PANEL_NT), 0, s, p);
   MG_HIP(hipGetLastError());
 }
 
@@ -719,7 +1140,7 @@ static void lu_leaf_w(hipStream_t s, int n, T *W, long ld, int c0, int w, int *i
   p.pairs = pairs + (long)c0 * PAIR_STRIDE;
   // the exchange polls one workgroup per lane of a wavefront
   const int maxwg = 64;
-  p.rec = (uint64_t *)ws.get("panel.rec", 2L * maxwg * (4 + 64 * Words<T>::N) * 8);
+  p.rec = (uint64_t *)ws.get("panel.rec", 2L * WREC_MAX * (4 + 64 * Words<T>::N) * 8);
   p.krec = (uint64_t *)ws.get("panel.krec", 2L * 64 * Words<T>::N * 8);
   (void)REC;
   p.err = sync_error_word(s);
