@@ -741,6 +741,10 @@ def run_batched(torch, libmat_amd, args):
                                          ptr(info), 1, C.byref(plan)), "batch plan")
     wg, order, tuned = C.c_int(), C.c_int(), C.c_double()
     g.check(lib.mat_gpu_batch_tuning(plan, C.byref(wg), C.byref(order), C.byref(tuned)), "tuning")
+    bin_mask, untuned = C.c_int(), C.c_double()
+    waves = (C.c_int32 * 8)()
+    g.check(lib.mat_gpu_batch_tuning_shape(plan, C.byref(bin_mask), waves, C.byref(untuned)),
+            "tuning shape")
 
     def step():
         g.check(lib.mat_gpu_batch_run(s, plan), "batch run")
@@ -826,9 +830,12 @@ def run_batched(torch, libmat_amd, args):
            "config": {"workload": "batched uneven small solves", "count": count,
                       "parallelism": "single GPU"},
            "matrices_per_s": round(count / (batch_ms * 1e-3), 1),
-           "batch_ms": round(batch_ms, 4), "tuning": {"wg_per_cu": wg.value,
-                                                      "order_mode": order.value,
-                                                      "tuned_ms": round(tuned.value, 4)},
+           "batch_ms": round(batch_ms, 4),
+           "tuning": {"wg_per_cu": wg.value, "order_mode": order.value,
+                      "bins_in_use": [8 << b for b in range(4) if bin_mask.value >> b & 1],
+                      "waves_per_workgroup": {f"{'spd' if c % 2 else 'general'}_{8 << (c // 2)}":
+                                              waves[c] for c in range(8)},
+                      "tuned_ms": round(tuned.value, 4), "untuned_ms": round(untuned.value, 4)},
            "max_backward_error_sampled": worst,
            "per_matrix_launch_loop": {"ms_for_all": round(loop_ms, 2),
                                       "sample": f"{sample} one-matrix plans, one graph launch each",

@@ -271,7 +271,9 @@ int mat_gpu_posv_mixed(void *stream, size_t n, size_t nrhs, const double *a, siz
  * counter reset and one persistent kernel per size bin and kind) in a
  * hipGraph; mat_gpu_batch_run replays it (the data
  * behind the pointers may change between runs, the pointers and sizes may
- * not).  autotune != 0 times the candidate launch shapes on this batch. */
+ * not).  autotune != 0 times candidate plans on this batch and keeps the
+ * fastest: the bins in use, the wavefronts per workgroup of every class,
+ * the resident workgroups per CU and the class order. */
 typedef struct mat_gpu_batch_plan mat_gpu_batch_plan;
 int mat_gpu_batch_create_f32(void *stream, size_t count, const int32_t *n, const int8_t *kind,
                              const float *a, const int64_t *a_off, const float *b, float *x,
@@ -280,6 +282,13 @@ int mat_gpu_batch_create_f32(void *stream, size_t count, const int32_t *n, const
 int mat_gpu_batch_run(void *stream, mat_gpu_batch_plan *plan);
 int mat_gpu_batch_tuning(const mat_gpu_batch_plan *plan, int *wg_per_cu, int *order_mode,
                          double *tuned_ms);
+/* The autotuner's shape choices: bin_mask bit b set = the bin of groups of
+ * 8 << b lanes is in use (an order goes to the smallest bin in use that
+ * holds it; bit 3 is always set), waves[8] = wavefronts per workgroup of
+ * the class (bin b, kind k) at index 2b + k, and the time of the untuned
+ * plan (0 for a plan built without autotuning). */
+int mat_gpu_batch_tuning_shape(const mat_gpu_batch_plan *plan, int *bin_mask, int32_t *waves,
+                               double *untuned_ms);
 int mat_gpu_batch_destroy(mat_gpu_batch_plan *plan);
 /* Plan + one ordinary launch, no graph; synchronous. */
 int mat_gpu_batch_solve_f32(void *stream, size_t count, const int32_t *n, const int8_t *kind,

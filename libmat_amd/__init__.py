@@ -43,7 +43,8 @@ EXT_SYMBOLS = [
     "mat_gpu_potrs_f32", "mat_gpu_potrs_f64", "mat_gpu_gesv_f32", "mat_gpu_gesv_f64",
     "mat_gpu_posv_f32", "mat_gpu_posv_f64", "mat_gpu_ir_defaults", "mat_gpu_gesv_mixed",
     "mat_gpu_posv_mixed", "mat_gpu_batch_create_f32", "mat_gpu_batch_run",
-    "mat_gpu_batch_tuning", "mat_gpu_batch_destroy", "mat_gpu_batch_solve_f32",
+    "mat_gpu_batch_tuning", "mat_gpu_batch_tuning_shape", "mat_gpu_batch_destroy",
+    "mat_gpu_batch_solve_f32",
     "mat_gpu_batch_solve_noalloc_f32",
 ]
 
@@ -138,6 +139,8 @@ class GpuLib:
           [vp, sz, vp, vp, vp, vp, vp, vp, vp, vp, i32, C.POINTER(vp)])
         s("mat_gpu_batch_run", i32, [vp, vp])
         s("mat_gpu_batch_tuning", i32, [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(dbl)])
+        s("mat_gpu_batch_tuning_shape", i32, [vp, C.POINTER(i32), C.POINTER(C.c_int32),
+                                              C.POINTER(dbl)])
         s("mat_gpu_batch_destroy", i32, [vp])
         s("mat_gpu_batch_solve_f32", i32, [vp, sz, vp, vp, vp, vp, vp, vp, vp, vp])
         s("mat_gpu_batch_solve_noalloc_f32", i32, [vp, sz, vp, vp, vp, vp, vp, vp, vp, vp])

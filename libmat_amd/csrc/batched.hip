@@ -25,8 +25,10 @@
 //    (one task = the matrices of one wavefront) from the class's atomic
 //    counter.  The launches (one counter reset, then the class kernels in
 //    order) are captured once in a hipGraph and replayed.
-//  * Autotuning picks the number of resident workgroups per CU and the
-//    class order by timing graph replays on the actual batch.
+//  * Autotuning (coordinate descent, graph replays timed on the actual
+//    batch) picks the bins in use (the order thresholds: a small bin can be
+//    merged into the next), the wavefronts per workgroup of every class
+//    (2, 4 or 8), the resident workgroups per CU and the class order.
 #include <algorithm>
 #include <chrono>
 #include <memory>
@@ -314,6 +316,7 @@ constexpr int class_of(int bin, int kind) { return bin * 2 + kind; }
 
 template <int NB, int KIND>
 struct ClassShape {
+  // wavefronts per workgroup when not tuned (the autotuner tries 2, 4, 8)
   static constexpr int WAVES = (KIND == 1 && NB == 64) ? 2 : 4;
   // floats per wavefront: hand-off buffers, plus the staged L for Cholesky
   static constexpr int LDSW = 2 * HB + (KIND == 1 ? 64 * NB : 0);
@@ -322,12 +325,12 @@ struct ClassShape {
   static constexpr int MINW = (KIND == 1 && NB == 64) ? 2 : 1;
 };
 
-template <int NB, int KIND>
-__global__ __launch_bounds__((64 * ClassShape<NB, KIND>::WAVES))
+template <int NB, int KIND, int W>
+__global__ __launch_bounds__(64 * W)
 __attribute__((amdgpu_waves_per_eu(ClassShape<NB, KIND>::MINW))) void batched_kernel(BatchP p, int t0,
                                                                                   int t1, int cls) {
   using S = ClassShape<NB, KIND>;
-  __shared__ __align__(16) float lds[S::WAVES][S::LDSW];
+  __shared__ __align__(16) float lds[W][S::LDSW];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   while (true) {
     int task = 0;
@@ -371,17 +374,29 @@ struct Plan {
   unsigned *d_counter = nullptr;
   int *d_n = nullptr;
   int wg_per_cu = 4;
+  // bins in use: bit b = the group of 8 << b lanes (orders up to 8 << b);
+  // an order goes to the smallest bin in use that holds it (bin 3 always)
+  int bin_mask = 0xF;
+  int waves[mg::NCLASS] = {mg::ClassShape<8, 0>::WAVES,  mg::ClassShape<8, 1>::WAVES,
+                           mg::ClassShape<16, 0>::WAVES, mg::ClassShape<16, 1>::WAVES,
+                           mg::ClassShape<32, 0>::WAVES, mg::ClassShape<32, 1>::WAVES,
+                           mg::ClassShape<64, 0>::WAVES, mg::ClassShape<64, 1>::WAVES};
   int cls_t0[mg::NCLASS] = {}, cls_t1[mg::NCLASS] = {};
   int cls_order[mg::NCLASS] = {};  // classes with tasks, in launch order
   int ncls = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   double tuned_ms = 0;
+  double untuned_ms = 0;
   int tuned_wg_per_cu = 0;
   int tuned_order = 0;
 };
 
-int bin_of(int n) { return n <= 8 ? 0 : (n <= 16 ? 1 : (n <= 32 ? 2 : 3)); }
+int bin_of(int n, int mask = 0xF) {
+  for (int b = 0; b < 3; ++b)
+    if (((mask >> b) & 1) && n <= (8 << b)) return b;
+  return 3;
+}
 
 // Build tasks: bins (NB, kind) -> chunks of 64/NB matrices.  order_mode 0:
 // largest bins first; 1: smallest first.  Within a class the matrices keep
@@ -392,7 +407,7 @@ int make_tasks_into(size_t count, const int32_t *n, const int8_t *kind, int orde
   size_t cnt[mg::NCLASS] = {}, off[mg::NCLASS] = {}, pos[mg::NCLASS] = {};
   for (size_t i = 0; i < count; ++i) {
     MG_CHECK(n[i] >= 1 && n[i] <= 64, "batch: matrix order must be in [1, 64]");
-    ++cnt[mg::class_of(bin_of(n[i]), kind[i] ? 1 : 0)];
+    ++cnt[mg::class_of(bin_of(n[i], pl.bin_mask), kind[i] ? 1 : 0)];
   }
   size_t o = 0;
   for (int bi = 0; bi < 4; ++bi) {
@@ -403,7 +418,8 @@ int make_tasks_into(size_t count, const int32_t *n, const int8_t *kind, int orde
       o += cnt[c];
     }
   }
-  for (size_t i = 0; i < count; ++i) order[pos[mg::class_of(bin_of(n[i]), kind[i] ? 1 : 0)]++] = (int)i;
+  for (size_t i = 0; i < count; ++i)
+    order[pos[mg::class_of(bin_of(n[i], pl.bin_mask), kind[i] ? 1 : 0)]++] = (int)i;
   int nt = 0;
   pl.ncls = 0;
   for (int bi = 0; bi < 4; ++bi) {
@@ -447,17 +463,25 @@ void upload_plan(hipStream_t s, Plan &pl, const std::vector<int> &order,
   pl.p.tasks = pl.d_tasks;
 }
 
-template <int NB, int KIND>
-void launch_class(hipStream_t s, const Plan &pl, int c) {
-  constexpr int W = mg::ClassShape<NB, KIND>::WAVES;
+template <int NB, int KIND, int W>
+void launch_class_w(hipStream_t s, const Plan &pl, int c) {
   const int t0 = pl.cls_t0[c], t1 = pl.cls_t1[c];
   // resident wavefronts as for wg_per_cu workgroups of four per CU, and no
   // more workgroups than the class has tasks for
   const int grid = std::max(1, std::min((int)mg::ceil_div(t1 - t0, W),
                                         mg::num_cus() * pl.wg_per_cu * 4 / W));
-  hipLaunchKernelGGL((mg::batched_kernel<NB, KIND>), dim3(grid), dim3(64 * W), 0, s, pl.p, t0, t1,
-                     c);
+  hipLaunchKernelGGL((mg::batched_kernel<NB, KIND, W>), dim3(grid), dim3(64 * W), 0, s, pl.p, t0,
+                     t1, c);
   MG_HIP(hipGetLastError());
+}
+
+template <int NB, int KIND>
+void launch_class(hipStream_t s, const Plan &pl, int c) {
+  switch (pl.waves[c]) {
+    case 2: launch_class_w<NB, KIND, 2>(s, pl, c); break;
+    case 8: launch_class_w<NB, KIND, 8>(s, pl, c); break;
+    default: launch_class_w<NB, KIND, 4>(s, pl, c); break;
+  }
 }
 
 void launch_one(hipStream_t s, const Plan &pl, int c) {
@@ -578,27 +602,54 @@ int mat_gpu_batch_create_f32(void *stream, size_t count, const int32_t *n, const
       pl->tuned_wg_per_cu = 4;
       capture(s, *pl);
     } else {
-      double best = 1e30;
-      int best_wg = 4, best_order = 0;
-      for (int om = 0; om < 2; ++om) {
-        make_tasks(count, n, kind, om, order, tasks, *pl);
+      // coordinate descent over the plan's choices, each timed by graph
+      // replays on this batch: the bins in use (merging a small bin into
+      // the next saves a launch and its tail, at the price of idle lanes),
+      // the wavefronts per workgroup of every class, the resident
+      // workgroups per CU and the class order; a choice is kept when it is
+      // at least 1% faster
+      int order_mode = 0;
+      auto trial = [&]() {
+        make_tasks(count, n, kind, order_mode, order, tasks, *pl);
         upload_plan(s, *pl, order, tasks);
-        for (int wg : {1, 2, 4, 8}) {
-          pl->wg_per_cu = wg;
-          capture(s, *pl);
-          const double ms = time_replays(s, *pl, 3);
-          if (ms < best) {
-            best = ms;
-            best_wg = wg;
-            best_order = om;
-          }
+        capture(s, *pl);
+        return time_replays(s, *pl, 3);
+      };
+      double best = trial();
+      pl->untuned_ms = best;
+      auto consider = [&](auto set, auto undo) {
+        set();
+        const double ms = trial();
+        if (ms < 0.99 * best) best = ms;
+        else undo();
+      };
+      for (int mask = 0; mask < 8; ++mask) {
+        const int prev = pl->bin_mask, cand = mask | 8;
+        if (cand == prev) continue;
+        consider([&] { pl->bin_mask = cand; }, [&] { pl->bin_mask = prev; });
+      }
+      make_tasks(count, n, kind, order_mode, order, tasks, *pl);  // classes in use
+      const int ncls = pl->ncls;
+      int used[mg::NCLASS];
+      for (int i = 0; i < ncls; ++i) used[i] = pl->cls_order[i];
+      for (int i = 0; i < ncls; ++i) {
+        const int c = used[i];
+        for (int wv : {2, 4, 8}) {
+          const int prev = pl->waves[c];
+          if (wv == prev) continue;
+          consider([&] { pl->waves[c] = wv; }, [&] { pl->waves[c] = prev; });
         }
       }
-      make_tasks(count, n, kind, best_order, order, tasks, *pl);
+      for (int wg : {1, 2, 8, 4}) {
+        const int prev = pl->wg_per_cu;
+        if (wg == prev) continue;
+        consider([&] { pl->wg_per_cu = wg; }, [&] { pl->wg_per_cu = prev; });
+      }
+      consider([&] { order_mode = 1; }, [&] { order_mode = 0; });
+      make_tasks(count, n, kind, order_mode, order, tasks, *pl);
       upload_plan(s, *pl, order, tasks);
-      pl->wg_per_cu = best_wg;
-      pl->tuned_wg_per_cu = best_wg;
-      pl->tuned_order = best_order;
+      pl->tuned_wg_per_cu = pl->wg_per_cu;
+      pl->tuned_order = order_mode;
       pl->tuned_ms = best;
       capture(s, *pl);
     }
@@ -626,6 +677,19 @@ int mat_gpu_batch_tuning(const mat_gpu_batch_plan *plan, int *wg_per_cu, int *or
     if (wg_per_cu) *wg_per_cu = pl->tuned_wg_per_cu;
     if (order_mode) *order_mode = pl->tuned_order;
     if (tuned_ms) *tuned_ms = pl->tuned_ms;
+    return 0;
+  });
+}
+
+int mat_gpu_batch_tuning_shape(const mat_gpu_batch_plan *plan, int *bin_mask, int32_t *waves,
+                               double *untuned_ms) {
+  return guarded([&] {
+    MG_CHECK(plan != nullptr, "batch: null plan");
+    const Plan *pl = reinterpret_cast<const Plan *>(plan);
+    if (bin_mask) *bin_mask = pl->bin_mask;
+    if (waves)
+      for (int c = 0; c < mg::NCLASS; ++c) waves[c] = pl->waves[c];
+    if (untuned_ms) *untuned_ms = pl->untuned_ms;
     return 0;
   });
 }

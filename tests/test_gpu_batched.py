@@ -109,3 +109,39 @@ def test_batch_graph_replay_and_not_spd():
         assert wg.value in (1, 2, 4, 8) and ms.value > 0
     finally:
         g.lib.mat_gpu_batch_destroy(plan)
+
+
+@pytest.mark.parametrize("lo,hi", [(3, 64), (3, 9), (33, 64)])
+def test_tuned_plan_matches_untuned(lo, hi):
+    """The autotuner's choices (bins in use, wavefronts per workgroup per
+    class, workgroups per CU, class order) change where a matrix is solved,
+    not how: a tuned plan gives the untuned plan's bits, and both the
+    oracle's answers."""
+    g = libmat_amd.load("f32")
+    ns, kinds, mats, bs, A, b, aoff, voff = make_batch(2000, seed=7 + lo, lo=lo, hi=hi)
+    out = []
+    for tune in (0, 1):
+        d = upload(ns, A, b, aoff, voff)
+        plan = C.c_void_p()
+        g.check(g.lib.mat_gpu_batch_create_f32(stream(), ns.size, ns.ctypes.data_as(C.c_void_p),
+                                               kinds.ctypes.data_as(C.c_void_p), ptr(d["A"]),
+                                               ptr(d["aoff"]), ptr(d["b"]), ptr(d["x"]),
+                                               ptr(d["voff"]), ptr(d["info"]), tune,
+                                               C.byref(plan)), "batch_create")
+        try:
+            g.check(g.lib.mat_gpu_batch_run(stream(), plan), "batch_run")
+            torch.cuda.synchronize()
+            mask, untuned = C.c_int(), C.c_double()
+            waves = (C.c_int32 * 8)()
+            g.check(g.lib.mat_gpu_batch_tuning_shape(plan, C.byref(mask), waves,
+                                                     C.byref(untuned)), "shape")
+            assert mask.value & 8 and all(w in (2, 4, 8) for w in waves)
+            if tune:
+                assert untuned.value > 0
+                print(f"orders {lo}..{hi}: bins {mask.value:04b} waves {list(waves)} "
+                      f"untuned {untuned.value:.3f} ms", flush=True)
+        finally:
+            g.lib.mat_gpu_batch_destroy(plan)
+        out.append((d["x"].cpu().numpy(), d["info"].cpu().numpy()))
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+    check(ns, kinds, mats, bs, out[1][0], out[1][1], voff)
