@@ -11,9 +11,13 @@ _GOLD = {}
 
 
 def golden(prec):
+    """The reference's recorded inputs and outputs (tests/golden/golden_*.lmfx,
+    written by tests/golden/gen_golden.py)."""
     if prec not in _GOLD:
-        with np.load(os.path.join(HERE, "golden", f"golden_{prec}.npz")) as z:
-            _GOLD[prec] = {k: z[k] for k in z.files}
+        import sys
+        sys.path.insert(0, os.path.join(HERE, "golden"))
+        import lmfx
+        _GOLD[prec] = lmfx.read(os.path.join(HERE, "golden", f"golden_{prec}.lmfx"))
     return _GOLD[prec]
 
 

@@ -4,12 +4,14 @@ Runs only where the reference sources exist: it builds oracle/_ref (the
 reference's mat.h compiled by oracle/Makefile, x86 scalar path) and records
 inputs and outputs of mat_mul / mat_gemm, mat_plu, mat_lu, mat_chol,
 mat_solve, mat_solve_spd, the four triangular solves and mat_det, in float
-and double builds, as compressed .npz files next to this script.  Inputs are
+and double builds, as LMFX files next to this script (golden_f32.lmfx,
+golden_f64.lmfx; the format is described in lmfx.py and read from C by
+lmfx.h).  Inputs are
 seeded numpy draws (uniform[-1,1]; SPD inputs G G^T / n + I), plus the
 singular and non-positive-definite cases that exercise the reference's
 eps-skip (mat.h:4234) and -1 paths (mat.h:6043).
 
-    python tests/golden/gen_golden.py
+    python tests/golden/gen_golden.py [output directory]
 """
 import os
 import sys
@@ -20,6 +22,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 from oracle import oracle as orc  # noqa: E402
+
+sys.path.insert(0, HERE)
+import lmfx  # noqa: E402
 
 PLU_SIZES = [1, 2, 3, 23, 24, 25, 47, 48, 49, 64, 65, 100]
 LU_SIZES = [3, 4, 10, 24, 25, 50]
@@ -33,7 +38,7 @@ def spd(rng, n, dt):
     return (g @ g.T / n + np.eye(n)).astype(dt)
 
 
-def main():
+def main(outdir=HERE):
     orc.build(ref=True)
     for prec in ("f32", "f64"):
         r = orc.Oracle(prec, kind="ref")
@@ -106,10 +111,10 @@ def main():
                         f"tri_{n}_tril_unit": r.solve_tril_unit(np.tril(T), b),
                         f"tri_{n}_triu": r.solve_triu(np.triu(T), b),
                         f"tri_{n}_trilt": r.solve_trilt(np.tril(T), b)})
-        path = os.path.join(HERE, f"golden_{prec}.npz")
-        np.savez_compressed(path, **out)
+        path = os.path.join(outdir, f"golden_{prec}.lmfx")
+        lmfx.write(path, out)
         print(path, os.path.getsize(path))
 
 
 if __name__ == "__main__":
-    main()
+    main(*sys.argv[1:])

@@ -77,3 +77,16 @@ def test_concurrent_plu_and_solve(gpu, prec):
         (L0, U0, p0, _), x0 = got[i][0]
         for (L, U, p, _), x in got[i][1:]:
             assert np.array_equal(p, p0) and np.array_equal(U, U0) and np.array_equal(x, x0)
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_c_fixture_check_on_gpu(gpu, tmp_path, prec):
+    """The C fixture checker, linked to the GPU library, reproduces the
+    reference's recorded outputs within the stated tolerances."""
+    from test_fixture_format import GOLDEN, build_check
+    exe = build_check(str(tmp_path / "fx"), prec, reference=False)
+    print(f"fixture_check {prec}", flush=True)
+    r = subprocess.run([exe, os.path.join(GOLDEN, f"golden_{prec}.lmfx"), "tol"],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert " 0 failures" in r.stdout
