@@ -356,15 +356,34 @@ class CpuBaselines:
 
 
 # ------------------------------------------------------------ mixed-lu (default)
+OWN_QR = False  # --qr own: orthogonal factors from this library's QR
+
+
+def q_of_gaussian(torch, n, gen):
+    """Q factor of a seeded n x n Gaussian (fp64): the vendor QR, or with
+    --qr own this library's mat_gpu_geqrf_f64 (same family of matrices;
+    used where the vendor QR does not run, such as under counter profiling)."""
+    G = torch.randn(n, n, dtype=torch.float64, device="cuda", generator=gen)
+    if not OWN_QR:
+        Q, _ = torch.linalg.qr(G)
+        return Q
+    import libmat_amd
+    g = libmat_amd.load("f64")
+    Qb = torch.empty_like(G)
+    # G's row-major buffer is a column-major matrix too (its transpose):
+    # another Gaussian; the column-major buffer of its Q is Q^T row-major
+    g.check(g.lib.mat_gpu_geqrf_f64(torch.cuda.current_stream().cuda_stream, n, n, ptr(G), n,
+                                    ptr(Qb)), "geqrf")
+    torch.cuda.synchronize()
+    log("own QR done")
+    return Qb.T
+
+
 def make_general(torch, n, kappas, nrhs, seed=2026):
     """A_kappa = U diag(sigma_kappa) V^T (fp64, column-major buffers), one B."""
     gen = torch.Generator(device="cuda").manual_seed(seed)
-    G1 = torch.randn(n, n, dtype=torch.float64, device="cuda", generator=gen)
-    U, _ = torch.linalg.qr(G1)
-    del G1
-    G2 = torch.randn(n, n, dtype=torch.float64, device="cuda", generator=gen)
-    V, _ = torch.linalg.qr(G2)
-    del G2
+    U = q_of_gaussian(torch, n, gen)
+    V = q_of_gaussian(torch, n, gen)
     mats = []
     for kappa in kappas:
         sig = torch.logspace(0, -np.log10(kappa), n, dtype=torch.float64, device="cuda")
@@ -496,7 +515,8 @@ def run_mixed(torch, libmat_amd, args, spd):
            "data": ("synthetic (seeded; U[-1,1] entries, kappa not controlled)"
                     if args.data == "uniform" else
                     "synthetic (seeded; " + ("Q diag(lambda) Q^T, Q = QR of Gaussian" if spd else
-                                             "U diag(sigma) V^T, U,V = QR of Gaussian") + ")"),
+                                             "U diag(sigma) V^T, U,V = QR of Gaussian")
+                    + (" by this library's QR" if OWN_QR else "") + ")"),
            "config": {"workload": name, "n": n, "nrhs": nrhs, "kappas": kappas,
                       "factorization": ("fp32 Cholesky, FP8 e4m3 per-128x128-tile-scaled MFMA "
                                         "trailing updates" if spd else
@@ -542,7 +562,7 @@ def orthonormal(torch, n, gen):
 
 def make_spd(torch, n, kappas, nrhs, seed=2027):
     gen = torch.Generator(device="cuda").manual_seed(seed)
-    Q = orthonormal(torch, n, gen)
+    Q = q_of_gaussian(torch, n, gen) if OWN_QR else orthonormal(torch, n, gen)
     mats = []
     for kappa in kappas:
         u = torch.rand(n, dtype=torch.float64, device="cuda", generator=gen)
@@ -915,12 +935,17 @@ def main():
     ap.add_argument("--data", default="conditioned", choices=["conditioned", "uniform"],
                     help="mixed-lu: U[-1,1] systems instead of the conditioned family (same "
                          "shapes; for counter profilers that cannot run the vendor QR)")
+    ap.add_argument("--qr", default="vendor", choices=["vendor", "own"],
+                    help="orthogonal factors of the conditioned systems from the vendor QR or "
+                         "from this library's (for counter profilers)")
     ap.add_argument("--cpu-worker", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--cpu-core", type=int, default=None, help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.cpu_worker is not None:
         cpu_worker(args.cpu_worker.split(","), args.cpu_core)
         return
+    global OWN_QR
+    OWN_QR = args.qr == "own"
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); the "
               "multi-GPU layout is a design note, not a measured path", file=sys.stderr)
