@@ -872,16 +872,67 @@ def run_batched(torch, libmat_amd, args):
     return out
 
 
+# ------------------------------------------------------------ drop-in, host buffers
+def run_dropin(torch, libmat_amd, args):
+    """What a libmat caller gets from the drop-in symbols: mat_mul,
+    mat_solve and mat_solve_spd on host Mats (fp32), timed through the C ABI
+    from the call to its return -- uploads, the GPU work and the downloads
+    over PCIe included (the Mats are built before the timed calls, so no
+    host-side copies are timed).  Median of `reps` calls after one warm-up
+    call, per order n."""
+    g = libmat_amd.load("f32")
+    rng = np.random.default_rng(3)
+    out = {}
+    for n in (256, 2048, 8192):
+        # each Mat points into the numpy buffer returned beside it: keep both
+        A, ab = g.to_mat(rng.uniform(-1, 1, (n, n)))
+        Bm, bb = g.to_mat(rng.uniform(-1, 1, (n, n)))
+        Cm, cb = g.empty_mat(n, n)
+        G = rng.standard_normal((n, n))
+        S, sb = g.to_mat(G @ G.T / n + np.eye(n))
+        b, vb = g.to_mat(rng.uniform(-1, 1, (n, 1)))
+        x, xb = g.empty_mat(n, 1)
+        keep = (ab, bb, cb, sb, vb, xb)
+        reps = 5 if n <= 2048 else 3
+
+        def med(fn):
+            fn()
+            ts = []
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                fn()
+                ts.append(time.perf_counter() - t0)
+            return float(np.median(ts))
+
+        t_mul = med(lambda: g.lib.mat_mul(C.byref(Cm), C.byref(A), C.byref(Bm)))
+        t_sol = med(lambda: g.lib.mat_solve(C.byref(x), C.byref(A), C.byref(b)))
+        t_spd = med(lambda: g.lib.mat_solve_spd(C.byref(x), C.byref(S), C.byref(b)))
+        del keep
+        out[str(n)] = {
+            "mat_mul": {"ms": round(t_mul * 1e3, 3), "GFLOP/s": round(2.0 * n ** 3 / t_mul / 1e9, 2)},
+            "mat_solve": {"ms": round(t_sol * 1e3, 3),
+                          "GFLOP/s": round((2.0 / 3.0 * n ** 3 + 2.0 * n ** 2) / t_sol / 1e9, 2)},
+            "mat_solve_spd": {"ms": round(t_spd * 1e3, 3),
+                              "GFLOP/s": round((n ** 3 / 3.0 + 2.0 * n ** 2) / t_spd / 1e9, 2)}}
+        log(f"drop-in n={n}: {out[str(n)]}")
+    return {"metric": "drop-in end-to-end time (host Mats, PCIe transfers included)",
+            "unit": "ms", "dtype": "f32", "higher_is_better": False,
+            "data": "synthetic U[-1,1]; SPD G G^T / n + I",
+            "config": {"workload": "libmat drop-in calls on host buffers", "orders": [256, 2048, 8192]},
+            "per_order": out}
+
+
 DEFAULTS = {  # n, nrhs, kappas, steps, warmup
     "mixed-lu": (16384, 16, "1e2,1e4,1e6", 2, 1), "mixed-spd": (32768, 4, "1e2,1e3", 2, 1),
     "gemm": (8192, 0, None, 20, 3), "chol": (8192, 8, None, 10, 2), "batched": (0, 1, None, 20, 3),
-    "qr": (4096, 0, None, 3, 1)}
+    "qr": (4096, 0, None, 3, 1), "dropin": (0, 0, None, 1, 0)}
 # workload name of the committed HBM-traffic passes, when run at their sizes
 TAGS = {"mixed-lu": "mixed_lu_n16384", "mixed-spd": "mixed_spd_n32768", "gemm": "gemm_8192",
-        "chol": "chol_8192", "batched": "batched_200k", "qr": "qr_4096"}
+        "chol": "chol_8192", "batched": "batched_200k", "qr": "qr_4096", "dropin": "dropin"}
 # the other configurations of the default run, under these keys
 ALL_CONFIGS = [("gemm", "gemm_fp32_8192"), ("chol", "chol_fp32_8192_8rhs"),
-               ("mixed-spd", "mixed_spd_32768"), ("batched", "batched_200k")]
+               ("mixed-spd", "mixed_spd_32768"), ("batched", "batched_200k"),
+               ("dropin", "dropin_end_to_end")]
 
 
 def workload_args(args, workload, steps=None, warmup=None):
@@ -909,6 +960,8 @@ def run_one(torch, libmat_amd, a):
         return run_chol(torch, libmat_amd, a)
     if a.workload == "qr":
         return run_qr(torch, libmat_amd, a)
+    if a.workload == "dropin":
+        return run_dropin(torch, libmat_amd, a)
     return run_batched(torch, libmat_amd, a)
 
 
@@ -919,7 +972,8 @@ def main():
                     help="timed steps of the headline (or of --workload)")
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--workload", default="all",
-                    choices=["all", "mixed-lu", "mixed-spd", "gemm", "chol", "batched", "qr"])
+                    choices=["all", "mixed-lu", "mixed-spd", "gemm", "chol", "batched", "qr",
+                             "dropin"])
     ap.add_argument("--n", type=int, default=None)
     ap.add_argument("--nrhs", type=int, default=None)
     ap.add_argument("--kappas", default=None)
@@ -956,7 +1010,7 @@ def main():
         args.n = args.nrhs = args.kappas = None
     # the CPU baselines start first, in their own pinned process, before
     # this process touches the GPU
-    cpu = CpuBaselines([] if args.no_cpu else [head] + [w for w, _ in subs])
+    cpu = CpuBaselines([] if args.no_cpu else [head] + [w for w, _ in subs if w != "dropin"])
 
     import torch
     import libmat_amd
@@ -976,7 +1030,7 @@ def main():
     if base:
         out["cpu_baseline"] = base.get(head)
         for w, key in subs:
-            if "error" not in out["configs"][key]:
+            if "error" not in out["configs"][key] and w in base:
                 out["configs"][key]["cpu_baseline"] = base.get(w)
     print(json.dumps(out), flush=True)
 
