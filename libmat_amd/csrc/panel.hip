@@ -1140,8 +1140,16 @@ static void lu_leaf_w(hipStream_t s, int n, T *W, long ld, int c0, int w, int *i
   p.pairs = pairs + (long)c0 * PAIR_STRIDE;
   // the exchange polls one workgroup per lane of a wavefront
   const int maxwg = 64;
-  p.rec = (uint64_t *)ws.get("panel.rec", 2L * WREC_MAX * (4 + 64 * Words<T>::N) * 8);
-  p.krec = (uint64_t *)ws.get("panel.krec", 2L * 64 * Words<T>::N * 8);
+  // a fresh allocation holds stale bytes, and a stale word whose high half
+  // equals a launch's tag would pass as published: clear it (tag 0 is never
+  // used), in stream order before the first kernel that polls it
+  const size_t rec_bytes = 2L * WREC_MAX * (4 + 64 * Words<T>::N) * 8;
+  const size_t krec_bytes = 2L * 64 * Words<T>::N * 8;
+  bool fresh = false;
+  p.rec = (uint64_t *)ws.get("panel.rec", rec_bytes, &fresh);
+  if (fresh) fill_u32(s, (unsigned *)p.rec, (long)(rec_bytes / 4), 0u);
+  p.krec = (uint64_t *)ws.get("panel.krec", krec_bytes, &fresh);
+  if (fresh) fill_u32(s, (unsigned *)p.krec, (long)(krec_bytes / 4), 0u);
   (void)REC;
   p.err = sync_error_word(s);
   p.epoch = epoch.fetch_add(1, std::memory_order_relaxed) % ((1u << 25) - 1) + 1;

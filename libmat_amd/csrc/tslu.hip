@@ -642,7 +642,13 @@ void tslu_leaf(hipStream_t s, int n, float *W, long ld, int c0, int w, int *ipiv
   p.ipiv = ipiv;
   p.pairs = pairs + (long)c0 * PAIR_STRIDE;
   p.err = sync_error_word(s);
-  p.rec = (uint64_t *)ws.get("tslu.rec", (size_t)32 * TS_W * TS_REC * 8);
+  // a fresh allocation holds stale bytes, and a stale word whose high half
+  // equals this launch's tag (small integers here) would pass as published:
+  // clear it (tag 0 is never used) before the first kernel polls it
+  const size_t rec_bytes = (size_t)32 * TS_W * TS_REC * 8;
+  bool fresh = false;
+  p.rec = (uint64_t *)ws.get("tslu.rec", rec_bytes, &fresh);
+  if (fresh) fill_u32(s, (unsigned *)p.rec, (long)(rec_bytes / 4), 0u);
   unsigned tag = epoch.fetch_add(1, std::memory_order_relaxed) + 1;
   if (tag == 0) tag = epoch.fetch_add(1, std::memory_order_relaxed) + 1;
   p.tag = tag;
