@@ -1,7 +1,7 @@
 """Builds the gfx950 shared libraries libmatgpu_f32.so / libmatgpu_f64.so.
 
 Every HIP source is compiled once to an object under libmat_amd/_build/;
-the drop-in layer (capi_dropin.cpp) is compiled once per element type
+the drop-in layer (capi_dropin.cpp, mat_host.cpp) is compiled once per element type
 (MAT_DOUBLE_PRECISION for the double flavour, as libmat selects
 mat_elem_t); both libraries link the same kernel objects.  Usage:
     python -m libmat_amd.build   (or libmat_amd.build.build())
@@ -23,6 +23,8 @@ ARCH = "gfx950"
 HIP_SOURCES = ["gemm.hip", "lowp.hip", "panel.hip", "trsm.hip", "chol.hip", "lu.hip",
                "solve.hip", "trsv.hip", "rbt.hip", "batched.hip", "tslu.hip", "qr.hip"]
 CPP_SOURCES = ["runtime.cpp", "capi_ext.cpp"]
+# compiled once per element type (MAT_DOUBLE_PRECISION for the double flavour)
+PER_TYPE = ["capi_dropin.cpp", "mat_host.cpp"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", f"-I{CSRC}", f"-I{os.path.join(ROOT, 'include')}"]
 # Packed fp32 VALU ops stay disabled in all device code.
@@ -51,8 +53,9 @@ def build(verbose: bool = False, jobs: int | None = None) -> list[str]:
     for s in CPP_SOURCES:
         tasks.append([HIPCC, *COMMON, "-c", os.path.join(CSRC, s), "-o", _obj(s)])
     for tag, defs in (("_f32", []), ("_f64", ["-DMAT_DOUBLE_PRECISION"])):
-        tasks.append([HIPCC, *COMMON, *defs, "-c", os.path.join(CSRC, "capi_dropin.cpp"), "-o",
-                      _obj("capi_dropin.cpp", tag)])
+        for src in PER_TYPE:
+            tasks.append([HIPCC, *COMMON, *defs, "-c", os.path.join(CSRC, src), "-o",
+                          _obj(src, tag)])
     with ThreadPoolExecutor(jobs) as ex:
         for msg in ex.map(_run, tasks):
             if verbose and msg:
@@ -63,7 +66,7 @@ def build(verbose: bool = False, jobs: int | None = None) -> list[str]:
         out = os.path.join(HERE, f"libmatgpu{tag}.so")
         tmp = out + ".tmp"
         _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *common_objs,
-              _obj("capi_dropin.cpp", tag)])
+              *[_obj(src, tag) for src in PER_TYPE]])
         os.replace(tmp, out)
         outs.append(out)
     return outs
