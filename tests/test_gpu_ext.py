@@ -261,6 +261,32 @@ def test_geqrf_device_matches_dropin(gpu, prec):
     assert np.array_equal(from_dev(dA, m, n), Rd)
 
 
+@pytest.mark.parametrize("prec", PRECS)
+def test_geqrf_padded_leading_dimension(gpu, prec):
+    """A padded leading dimension (lda = m + 7, which the header allows):
+    R (the top m rows) and Q bit for bit the lda = m result, with and
+    without q, and the padding rows untouched."""
+    g = gpu[prec]
+    m, n, pad = 700, 333, 7
+    A = np.random.default_rng(5).uniform(-1, 1, (m, n))
+    fn = getattr(g.lib, f"mat_gpu_geqrf_{prec}")
+    dA = to_dev(A, DT[prec])
+    dQ = to_dev(np.zeros((m, m)), DT[prec])
+    assert fn(stream(), m, n, ptr(dA), m, ptr(dQ)) == 0
+    R0, Q0 = from_dev(dA, m, n), from_dev(dQ, m, m)
+    for with_q in (True, False):
+        Ap = np.full((m + pad, n), 12345.0)
+        Ap[:m] = A
+        dAp = to_dev(Ap, DT[prec])
+        dQp = to_dev(np.zeros((m, m)), DT[prec])
+        assert fn(stream(), m, n, ptr(dAp), m + pad, ptr(dQp) if with_q else None) == 0
+        got = from_dev(dAp, m + pad, n)
+        assert np.array_equal(got[:m], R0), with_q
+        assert np.all(got[m:] == 12345.0), with_q
+        if with_q:
+            assert np.array_equal(from_dev(dQp, m, m), Q0)
+
+
 def test_geqrf_full_size_properties(gpu):
     """QR at the benchmark's size (fp32 4096 x 4096, through the device
     entry point): Q^T Q = I and A = Q R checked in fp64 on the GPU, the
