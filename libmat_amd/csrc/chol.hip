@@ -103,12 +103,217 @@ __global__ __launch_bounds__(CHOL_NB * (CHOL_NB / CS)) void chol_split_kernel(in
   }
 }
 
+// The same diagonal block, blocked: the 64 x 64 block in LDS (row-major,
+// stride 68), factored in sub-panels of SB columns; per sub-panel
+//   1. wavefront 0 reads the SB x SB diagonal tile into every lane's
+//      registers and factors it there (each lane the whole tile, no
+//      cross-lane traffic: bound by FMA issue, not by a chain of exchanges),
+//      lane 0 stores it, and lanes 0 .. m-1 solve their row below the tile,
+//      x L11^T = a, from the same registers (x_J *= 1/L(J, J), then
+//      x_c -= x_J L(c, J), the elimination's order);
+//   2. the trailing lower triangle (m x m) takes the rank-SB update, 2 x 2
+//      elements per thread;
+// two workgroup barriers per SB columns.  -1 at the first d <= 0 (the
+// reference's test; NaN passes it) as the other kernels: nothing is written
+// back.
+constexpr int CB_LD = CHOL_NB + 4;
+
+// v of the first active lane, as a wavefront-uniform value
+template <typename T>
+__device__ __forceinline__ T first_lane(T v) {
+  if constexpr (sizeof(T) == 4) {
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+  } else {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffffll));
+    const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+  }
+}
+
+// SB consecutive values from 16-byte aligned LDS into registers
+template <typename T, int SB>
+__device__ __forceinline__ void lds_row(const T *p, T *x) {
+#pragma unroll
+  for (int c = 0; c < SB; c += 16 / (int)sizeof(T)) {
+    if constexpr (sizeof(T) == 4) {
+      const float4 v = *reinterpret_cast<const float4 *>(p + c);
+      x[c] = v.x, x[c + 1] = v.y, x[c + 2] = v.z, x[c + 3] = v.w;
+    } else {
+      const double2 v = *reinterpret_cast<const double2 *>(p + c);
+      x[c] = v.x, x[c + 1] = v.y;
+    }
+  }
+}
+
+template <typename T, int SB>
+__global__ __launch_bounds__(256) void chol_blk_kernel(int n, T *A, long lda, int j0, int *info) {
+  constexpr int NB = CHOL_NB, LD = CB_LD;
+  __shared__ __align__(16) T S[NB * LD];
+  __shared__ int s_fail;
+  const int t = threadIdx.x;
+  const int failed = *(volatile int *)info;  // an earlier column already failed
+  {
+    // thread t reads row t % 64 of columns t / 64 + 4k (coalesced along the
+    // column); elements outside the lower triangle read A(0, 0) instead and
+    // are replaced by zero, so the loads need no branch
+    constexpr int PER = NB * NB / 256;
+    const int r = t % NB, c0 = t / NB;
+    const T *p = A + r + (long)c0 * lda;
+    const long step = 4 * lda;
+    T v[PER];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int c = c0 + 4 * k;
+      const bool in = r < n && c < n && c <= r;
+      const T x = *(in ? p + k * step : A);
+      v[k] = in ? x : T(0);
+    }
+    if (failed != 0) return;
+    if (t == 0) s_fail = 0;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) S[r * LD + c0 + 4 * k] = v[k];
+  }
+  __syncthreads();
+  for (int jb = 0; jb < n; jb += SB) {
+    const int w = min(SB, n - jb), m = n - jb - SB;
+    if (t < 64) {
+      // 1. the tile in every lane of wavefront 0 (lower triangle used)
+      T a[SB][SB], inv[SB];
+#pragma unroll
+      for (int i = 0; i < SB; ++i) lds_row<T, SB>(S + (jb + i) * LD + jb, a[i]);
+      int fail = 0;
+#pragma unroll
+      for (int J = 0; J < SB; ++J) {
+        inv[J] = T(0);  // columns past n: no scaling, no update
+        if (J < w && fail == 0) {
+          const T d = first_lane(a[J][J]);
+          if (d <= T(0)) {
+            fail = jb + J + 1;
+          } else {
+            T l, iv;
+            if constexpr (sizeof(T) == 4) {
+              if (d >= 1e-30f) {
+                iv = __builtin_amdgcn_rsqf(d);
+                l = d * iv;
+              } else {
+                l = sqrtf(d);
+                iv = 1.0f / l;
+              }
+            } else {
+              l = sqrt(d);
+              iv = T(1) / l;
+            }
+            a[J][J] = l;
+            inv[J] = iv;
+#pragma unroll
+            for (int i = J + 1; i < SB; ++i) a[i][J] *= iv;
+#pragma unroll
+            for (int c = J + 1; c < SB; ++c)
+#pragma unroll
+              for (int i = c; i < SB; ++i) a[i][c] = fma(-a[i][J], a[c][J], a[i][c]);
+          }
+        }
+      }
+      if (fail) {
+        if (t == 0) s_fail = fail;
+      } else {
+        // the wavefront has read the whole tile: its LDS ops run in order,
+        // so lane 0 may overwrite it now
+        if (t == 0) {
+#pragma unroll
+          for (int i = 0; i < SB; ++i)
+#pragma unroll
+            for (int c = 0; c < SB; ++c) S[(jb + i) * LD + jb + c] = c <= i ? a[i][c] : T(0);
+        }
+        if (t < m) {
+          T *row = S + (jb + SB + t) * LD + jb;
+          T x[SB];
+          lds_row<T, SB>(row, x);
+#pragma unroll
+          for (int J = 0; J < SB; ++J) {
+            x[J] *= inv[J];
+#pragma unroll
+            for (int c = J + 1; c < SB; ++c) x[c] = fma(-x[J], a[c][J], x[c]);
+          }
+#pragma unroll
+          for (int c = 0; c < SB; ++c) row[c] = x[c];
+        }
+      }
+    }
+    __syncthreads();
+    if (s_fail) {
+      if (t == 0) *info = j0 + s_fail;
+      return;
+    }
+    if (m > 0) {
+      // 2. rank-SB update of the trailing m x m lower triangle in 2 x 2
+      // blocks; block (bi, bc), bc <= bi, rows 2bi, 2bi + 1, columns
+      // 2bc, 2bc + 1 (a row or column at m lies inside the 64-row block, as
+      // m is odd only when n < 64, and is never written)
+      const int nb = (m + 1) / 2, nblk = nb * (nb + 1) / 2;
+      for (int e = t; e < nblk; e += 256) {
+        int bi = (int)((sqrtf(8.0f * (float)e + 1.0f) - 1.0f) * 0.5f);
+        while ((bi + 1) * (bi + 2) / 2 <= e) ++bi;
+        while (bi * (bi + 1) / 2 > e) --bi;
+        const int bc = e - bi * (bi + 1) / 2;
+        const int i0 = 2 * bi, c0 = 2 * bc;
+        const int R0 = jb + SB + i0, C0 = jb + SB + c0;
+        T xi0[SB], xi1[SB], xc0[SB], xc1[SB];
+        lds_row<T, SB>(S + R0 * LD + jb, xi0);
+        lds_row<T, SB>(S + (R0 + 1) * LD + jb, xi1);
+        lds_row<T, SB>(S + C0 * LD + jb, xc0);
+        lds_row<T, SB>(S + (C0 + 1) * LD + jb, xc1);
+        T a00 = S[R0 * LD + C0], a01 = S[R0 * LD + C0 + 1];
+        T a10 = S[(R0 + 1) * LD + C0], a11 = S[(R0 + 1) * LD + C0 + 1];
+#pragma unroll
+        for (int k = 0; k < SB; ++k) {
+          a00 = fma(-xi0[k], xc0[k], a00);
+          a01 = fma(-xi0[k], xc1[k], a01);
+          a10 = fma(-xi1[k], xc0[k], a10);
+          a11 = fma(-xi1[k], xc1[k], a11);
+        }
+        S[R0 * LD + C0] = a00;
+        if (c0 + 1 <= i0) S[R0 * LD + C0 + 1] = a01;
+        if (i0 + 1 < m) {
+          S[(R0 + 1) * LD + C0] = a10;
+          if (c0 + 1 < m) S[(R0 + 1) * LD + C0 + 1] = a11;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  for (int e = t; e < NB * NB; e += 256) {
+    const int r = e % NB, c = e / NB;
+    if (r < n && c < n && c <= r) A[r + (long)c * lda] = S[r * LD + c];
+  }
+}
+
+// Diagonal-block kernel: 1 = blocked with 8-column sub-panels (default),
+// 2 = blocked with 16-column sub-panels, 0 = eight lanes per row
+// (chol_split_kernel); MATGPU_CHOL_LEAF overrides.  Measured on MI355X for
+// one 64 x 64 block alone: fp32 17.9 / 21.8 / 19.4 us, fp64 24.8 / 39.8 /
+// 28.4 us (a 16 x 16 tile held whole in every lane is ~1100 instructions
+// of one wavefront, and spills in fp64).
+static int chol_leaf_kind() {
+  static const int v = [] {
+    const char *e = getenv("MATGPU_CHOL_LEAF");
+    return e ? atoi(e) : 1;
+  }();
+  return v;
+}
+
 template <typename T>
 static void potrf_rec(hipStream_t s, int n, T *A, long lda, int j0, int *info, Mul mul) {
   if (n <= CHOL_NB) {
     KScope ks(s, "chol_leaf", (double)n * n * sizeof(T));
-    hipLaunchKernelGGL((chol_split_kernel<T, CHOL_CS>), dim3(1), dim3(CHOL_NB * (CHOL_NB / CHOL_CS)),
-                       0, s, n, A, lda, j0, info);
+    if (chol_leaf_kind() == 1)
+      hipLaunchKernelGGL((chol_blk_kernel<T, 8>), dim3(1), dim3(256), 0, s, n, A, lda, j0, info);
+    else if (chol_leaf_kind() == 2)
+      hipLaunchKernelGGL((chol_blk_kernel<T, 16>), dim3(1), dim3(256), 0, s, n, A, lda, j0, info);
+    else
+      hipLaunchKernelGGL((chol_split_kernel<T, CHOL_CS>), dim3(1),
+                         dim3(CHOL_NB * (CHOL_NB / CHOL_CS)), 0, s, n, A, lda, j0, info);
     MG_HIP(hipGetLastError());
     return;
   }
