@@ -334,9 +334,10 @@ __global__ __launch_bounds__(ChainNT<T>::NT) void tri_chain_kernel(int n, int nr
   constexpr int LDS = TB + 1;
   constexpr int NW = ChainWords<T>::N;
   constexpr int BW = TB * NC * NW;  // published words per block, [c][j][h]
-  // fp32 folded solves run their tile products on MFMA (8 wavefronts x 16
-  // rows = the 128-row block, all 16 right-hand sides)
-  constexpr bool MF = std::is_same<T, float>::value && FOLD && NT == 512 && TB == 128 && NC == 16;
+  // fp32 solves run their tile products on MFMA (8 wavefronts x 16 rows =
+  // the 128-row block, all 16 right-hand sides); unfolded ones also the
+  // final product with the inverted diagonal block
+  constexpr bool MF = std::is_same<T, float>::value && NT == 512 && TB == 128 && NC == 16;
   __shared__ T S[2][TB * LDS];
   __shared__ T Xs[TB * NC];  // [row j][rhs c]
   __shared__ int s_rb, s_fail;
@@ -399,10 +400,21 @@ __global__ __launch_bounds__(ChainNT<T>::NT) void tri_chain_kernel(int n, int nr
       } else {
         chain_load<T, TB, NT>(Dv, 0, TB, 0, TB, v);
       }
+      if constexpr (MF) {
+        // b_r in the accumulator layout: register q of lane l holds row
+        // 16 * wave + 4 * (l / 16) + q, column l % 16
+        const int l = t & 63, wv = t >> 6, c = l & 15;
 #pragma unroll
-      for (int q = 0; q < CPT; ++q) {
-        const int c = g * CPT + q;
-        acc[q] = (i < rrows && c < nrhs) ? X[(ro + i) + (long)c * ldx] : T(0);
+        for (int q = 0; q < 4; ++q) {
+          const int row = 16 * wv + 4 * (l >> 4) + q;
+          macc[q] = (row < rrows && c < nrhs) ? (float)X[(ro + row) + (long)c * ldx] : 0.0f;
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < CPT; ++q) {
+          const int c = g * CPT + q;
+          acc[q] = (i < rrows && c < nrhs) ? X[(ro + i) + (long)c * ldx] : T(0);
+        }
       }
     }
     for (int idx = 0; idx < cnt; ++idx) {
@@ -469,6 +481,23 @@ __global__ __launch_bounds__(ChainNT<T>::NT) void tri_chain_kernel(int n, int nr
       // after the last tile the folded path publishes without touching LDS
       // (and ends with its own barrier): no barrier in between
       if (!FOLD || idx + 1 < cnt) __syncthreads();
+    }
+    if constexpr (MF && !FOLD) {
+      // X_r = D_r^-1 acc: D_r (in v since the last tile) and acc to LDS;
+      // the loop's last barrier has retired every reader of Xs and of this
+      // buffer
+      T *Sd = S[cnt & 1];
+      chain_store<T, TB, NT>(true, v, Sd);
+      {
+        const int l = t & 63, wv = t >> 6, c = l & 15;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) Xs[(16 * wv + 4 * (l >> 4) + q) * NC + c] = macc[q];
+      }
+      __syncthreads();
+      chain_f32x4 dacc = {0.0f, 0.0f, 0.0f, 0.0f};
+      chain_mfma<false>(reinterpret_cast<const float *>(Sd), LDS, reinterpret_cast<const float *>(Xs),
+                        dacc);
+      macc = dacc;
     }
     T out[CPT];
     if constexpr (FOLD) {
