@@ -68,10 +68,11 @@ sys.path.insert(0, ROOT)
 # MI355X published dense peaks
 PEAK = {"bf16": 2500.0, "fp8": 5000.0, "fp32": 157.3, "fp64": 78.6, "hbm": 8000.0}
 FAMILIES = ["panel", "rowswap", "gemm", "gemm_bf16x", "gemm_fp8", "pack", "trsm", "tri_chain",
-            "chol_leaf", "qr_panel"]
+            "chol_leaf", "chol_panel", "qr_panel"]
 # family -> (bound, unit, peak key for the given dtype)
 BOUND = {"panel": ("hbm", "GB/s"), "rowswap": ("hbm", "GB/s"), "pack": ("hbm", "GB/s"),
          "tri_chain": ("hbm", "GB/s"), "chol_leaf": ("hbm", "GB/s"), "trsm": ("hbm", "GB/s"),
+         "chol_panel": ("hbm", "GB/s"),
          "qr_panel": ("hbm", "GB/s"),
          "gemm": ("mfma", "TFLOP/s"), "gemm_bf16x": ("mfma", "TFLOP/s"),
          "gemm_fp8": ("mfma", "TFLOP/s")}

@@ -412,18 +412,47 @@ void split_lu_colmajor(hipStream_t s, int n, const T *A, long ld, T *L, T *U) {
   MG_HIP(hipGetLastError());
 }
 
-template <typename T>
-__global__ void copy_lower_kernel(int n, const T *A, long lda, T *L, long ldl) {
-  const long idx = (long)blockIdx.x * 256 + threadIdx.x;
-  if (idx >= (long)n * n) return;
-  const int i = (int)(idx % n), j = (int)(idx / n);
-  L[i + (long)j * ldl] = (i >= j) ? A[i + (long)j * lda] : T(0);
+// One workgroup per column j: L(:, j) = A(:, j) on and below the diagonal,
+// zero above it; with VEC, 16-byte accesses (both columns 16-byte aligned,
+// n a multiple of the vector width), and the part above the diagonal is
+// written without being read.
+template <typename T, bool VEC>
+__global__ __launch_bounds__(256) void copy_lower_kernel(int n, const T *A, long lda, T *L,
+                                                         long ldl) {
+  const int j = blockIdx.x;
+  const T *a = A + (long)j * lda;
+  T *l = L + (long)j * ldl;
+  if constexpr (VEC) {
+    constexpr int V = 16 / (int)sizeof(T);
+    using Vec = typename std::conditional<sizeof(T) == 4, float4, double2>::type;
+    for (int i0 = threadIdx.x * V; i0 < n; i0 += 256 * V) {
+      Vec v;
+      if (i0 + V - 1 < j) {
+        v = Vec{};
+      } else {
+        v = *reinterpret_cast<const Vec *>(a + i0);
+        T *e = reinterpret_cast<T *>(&v);
+#pragma unroll
+        for (int k = 0; k < V; ++k)
+          if (i0 + k < j) e[k] = T(0);
+      }
+      *reinterpret_cast<Vec *>(l + i0) = v;
+    }
+  } else {
+    for (int i = threadIdx.x; i < n; i += 256) l[i] = i >= j ? a[i] : T(0);
+  }
 }
 
 template <typename T>
 void copy_lower(hipStream_t s, int n, const T *A, long lda, T *L, long ldl) {
-  hipLaunchKernelGGL((copy_lower_kernel<T>), dim3(ceil_div((size_t)n * n, 256)), dim3(256), 0, s, n,
-                     A, lda, L, ldl);
+  if (n <= 0) return;
+  constexpr int V = 16 / (int)sizeof(T);
+  const bool vec = n % V == 0 && lda % V == 0 && ldl % V == 0 && (uintptr_t)A % 16 == 0 &&
+                   (uintptr_t)L % 16 == 0;
+  if (vec)
+    hipLaunchKernelGGL((copy_lower_kernel<T, true>), dim3(n), dim3(256), 0, s, n, A, lda, L, ldl);
+  else
+    hipLaunchKernelGGL((copy_lower_kernel<T, false>), dim3(n), dim3(256), 0, s, n, A, lda, L, ldl);
   MG_HIP(hipGetLastError());
 }
 
