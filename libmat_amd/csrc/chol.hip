@@ -415,7 +415,8 @@ static void potrf_rec(hipStream_t s, int n, T *A, long lda, int j0, int *info, M
 // A -= a a^T on the lower triangle of the m x m matrix C (a: m x k), in
 // the precision the recursion uses for its trailing updates.
 template <typename T>
-static void syrk_lower(hipStream_t s, long m, long k, View<const T> a, T *C, long lda, Mul mul) {
+static void syrk_lower(hipStream_t s, long m, long k, View<const T> a, T *C, long lda, Mul mul,
+                       int grid_cap = 0) {
   if (m <= 0 || k <= 0) return;
   if constexpr (std::is_same<T, float>::value) {
     if ((double)m * m * k >= kLowpMinWork) {
@@ -429,7 +430,7 @@ static void syrk_lower(hipStream_t s, long m, long k, View<const T> a, T *C, lon
       }
     }
   }
-  gemm<T>(s, m, m, k, T(-1), a, a.t(), T(1), View<T>{C, 1, lda}, true);
+  gemm<T>(s, m, m, k, T(-1), a, a.t(), T(1), View<T>{C, 1, lda}, true, grid_cap);
 }
 
 // Right-looking blocked Cholesky with look-ahead over NB-wide column
@@ -510,6 +511,15 @@ static void chol_panel(hipStream_t s, int n, int mrows, T *A, long lda, int j0, 
   MG_HIP(hipGetLastError());
 }
 
+// Workgroups of the side-stream SYRK (0 = one per tile; MATGPU_SYRK_CAP).
+static int syrk_cap() {
+  static const int v = [] {
+    const char *e = getenv("MATGPU_SYRK_CAP");
+    return e ? atoi(e) : 0;
+  }();
+  return v;
+}
+
 // Windowed Cholesky: W-column windows of 64-column blocks.
 //   caller's stream, per block k of window [w0, w1):
 //     block k's columns (rows bk..n-1) -= the window's earlier panels
@@ -553,7 +563,7 @@ static void potrf_window(hipStream_t s, int n, T *A, long lda, int *info, int W)
       MG_HIP(hipEventRecord(evP, s));
       MG_HIP(hipStreamWaitEvent(u, evP, 0));
       syrk_lower<T>(u, n - w2, w1 - w0, View<const T>{at(w2, w0), 1, lda}, at(w2, w2), lda,
-                    Mul::Native);
+                    Mul::Native, syrk_cap());
       MG_HIP(hipEventRecord(evR, u));
       side = true;
     }

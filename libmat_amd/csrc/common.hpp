@@ -130,10 +130,12 @@ inline unsigned ceil_div(size_t a, size_t b) { return (unsigned)((a + b - 1) / b
 // C = alpha * A * B + beta * C on arbitrary strided views (M x K, K x N,
 // M x N).  beta == 0 never reads C.  tri = 1 (or true): only elements with
 // row >= col of C are written; tri = 2: only row <= col (tiles wholly
-// outside the triangle are skipped); tri = 0: all of C.
+// outside the triangle are skipped); tri = 0: all of C.  grid_cap > 0
+// limits the launch to that many workgroups, each walking several tiles
+// (work meant to run beside a critical path on another stream).
 template <typename T>
 void gemm(hipStream_t s, long M, long N, long K, T alpha, View<const T> A,
-          View<const T> B, T beta, View<T> C, int tri = 0);
+          View<const T> B, T beta, View<T> C, int tri = 0, int grid_cap = 0);
 
 // Below this many multiply-adds (M*N*K) the factorizations run their
 // bf16-split updates as native fp32 MFMA GEMMs instead.

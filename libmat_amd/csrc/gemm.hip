@@ -62,6 +62,12 @@ struct GemmP {
   T alpha, beta;
   int tiles_m, tiles_n, tiles;
   int lower;
+  // compact: only the tiles on or below the diagonal are enumerated (lower
+  // products with square tiles), so the ids dealt to the XCDs are all work
+  int compact;
+  // persistent: the grid is smaller than the tile count and each
+  // workgroup walks its XCD's share of tiles
+  int persist;
 };
 
 // Loads one BK x MNT operand slab from global memory into registers.
@@ -146,8 +152,81 @@ __device__ inline bool map_tile(int bid, int grid, int tiles, int tiles_m, int t
   return true;
 }
 
+// Compact lower enumeration: id -> (tm, tn), tn <= tm, row by row (the
+// first tiles_n rows form a triangle, later rows hold tiles_n tiles each).
+__device__ inline void lower_tile(int id, int tiles_n, int &tm, int &tn) {
+  const int tri = tiles_n * (tiles_n + 1) / 2;
+  if (id < tri) {
+    int r = (int)((sqrtf(8.0f * (float)id + 1.0f) - 1.0f) * 0.5f);
+    while ((r + 1) * (r + 2) / 2 <= id) ++r;
+    while (r * (r + 1) / 2 > id) --r;
+    tm = r;
+    tn = id - r * (r + 1) / 2;
+  } else {
+    const int q = id - tri;
+    tm = tiles_n + q / tiles_n;
+    tn = q % tiles_n;
+  }
+}
+
+// Tile id -> (tm, tn) for the compact or the banded order.
+__device__ inline void tile_of(const int id, int compact, int tiles_m, int tiles_n, int &tm,
+                               int &tn) {
+  if (compact) {
+    lower_tile(id, tiles_n, tm, tn);
+    return;
+  }
+  constexpr int GROUP_M = 8;
+  const int per_group = GROUP_M * tiles_n;
+  const int g = id / per_group;
+  const int first_m = g * GROUP_M;
+  const int gsz = min(tiles_m - first_m, GROUP_M);
+  const int in = id % per_group;
+  tm = first_m + in % gsz;
+  tn = in / gsz;
+}
+
+template <typename T, int BM, int BN, int BK, bool A_MN, bool B_MN, bool VEC>
+__device__ __forceinline__ void gemm_tile(const GemmP<T> &p, int tm, int tn, T (*As)[BK * (BM + 16 / sizeof(T))],
+                                          T (*Bs)[BK * (BN + 16 / sizeof(T))]);
+
 template <typename T, int BM, int BN, int BK, bool A_MN, bool B_MN, bool VEC>
 __global__ __launch_bounds__(256) void gemm_kernel(GemmP<T> p) {
+  constexpr int PAD = 16 / (int)sizeof(T);
+  __shared__ T As[2][BK * (BM + PAD)];
+  __shared__ T Bs[2][BK * (BN + PAD)];
+  if (!p.persist) {
+    int tm, tn;
+    if (p.compact) {
+      constexpr int XCD = 8;
+      const int grid = gridDim.x, bid = blockIdx.x;
+      const int id = grid % XCD == 0 ? (bid % XCD) * (grid / XCD) + bid / XCD : bid;
+      if (id >= p.tiles) return;
+      lower_tile(id, p.tiles_n, tm, tn);
+    } else if (!map_tile(blockIdx.x, gridDim.x, p.tiles, p.tiles_m, p.tiles_n, tm, tn)) {
+      return;
+    }
+    gemm_tile<T, BM, BN, BK, A_MN, B_MN, VEC>(p, tm, tn, As, Bs);
+    return;
+  }
+  // persistent (grid a multiple of 8): XCD x = blockIdx % 8 owns the ids
+  // [x * per, (x + 1) * per), per = tiles / 8 rounded up, which its
+  // workgroups take in turn
+  constexpr int XCD = 8;
+  const int x = blockIdx.x % XCD, j = blockIdx.x / XCD, wgs = gridDim.x / XCD;
+  const int per = (p.tiles + XCD - 1) / XCD;
+  for (int k = j; k < per; k += wgs) {
+    const int id = x * per + k;
+    if (id >= p.tiles) break;
+    int tm, tn;
+    tile_of(id, p.compact, p.tiles_m, p.tiles_n, tm, tn);
+    gemm_tile<T, BM, BN, BK, A_MN, B_MN, VEC>(p, tm, tn, As, Bs);
+  }
+}
+
+template <typename T, int BM, int BN, int BK, bool A_MN, bool B_MN, bool VEC>
+__device__ __forceinline__ void gemm_tile(const GemmP<T> &p, int tm, int tn, T (*As)[BK * (BM + 16 / sizeof(T))],
+                                          T (*Bs)[BK * (BN + 16 / sizeof(T))]) {
   using F = Mf<T>;
   constexpr int FM = F::FM;
   constexpr int G = 64 / FM;
@@ -158,11 +237,6 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmP<T> p) {
   constexpr int PAD = 16 / (int)sizeof(T);
   constexpr int LDA_S = BM + PAD, LDB_S = BN + PAD;
 
-  __shared__ T As[2][BK * LDA_S];
-  __shared__ T Bs[2][BK * LDB_S];
-
-  int tm, tn;
-  if (!map_tile(blockIdx.x, gridDim.x, p.tiles, p.tiles_m, p.tiles_n, tm, tn)) return;
   const int bm0 = tm * BM, bn0 = tn * BN;
   // triangular C: skip tiles wholly outside the requested triangle
   if (p.lower == 1 && bm0 + BM - 1 < bn0) return;  // strictly above the diagonal
@@ -241,12 +315,23 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmP<T> p) {
 }
 
 template <typename T, bool A_MN, bool B_MN, int BM, int BN, int BK>
-static void launch_tiles(hipStream_t s, GemmP<T> &p, bool vec) {
+static void launch_tiles(hipStream_t s, GemmP<T> &p, bool vec, int grid_cap) {
   p.tiles_m = ceil_div(p.M, BM);
   p.tiles_n = ceil_div(p.N, BN);
-  p.tiles = p.tiles_m * p.tiles_n;
+  p.compact = p.lower == 1 && BM == BN;
+  if (p.compact) {
+    const long tm = p.tiles_m, tn = p.tiles_n, d = std::min(tm, tn);
+    p.tiles = (int)(d * (d + 1) / 2 + std::max(0L, tm - tn) * tn);
+  } else {
+    p.tiles = p.tiles_m * p.tiles_n;
+  }
   int grid = p.tiles;
   if (grid >= 8) grid = (grid + 7) / 8 * 8;
+  p.persist = 0;
+  if (grid_cap >= 8 && grid > grid_cap) {
+    grid = grid_cap / 8 * 8;
+    p.persist = 1;
+  }
   KScope ks(s, "gemm", (p.lower ? 1.0 : 2.0) * p.M * p.N * p.K);
   if (vec)
     hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, A_MN, B_MN, true>), dim3(grid), dim3(256), 0, s, p);
@@ -263,7 +348,7 @@ static void launch_tiles(hipStream_t s, GemmP<T> &p, bool vec) {
 // two per CU) with 64-deep slabs (32 for f64, for LDS) take a quarter of the
 // round trips.
 template <typename T, bool A_MN, bool B_MN>
-static void launch_mode(hipStream_t s, GemmP<T> &p, bool vec, int in_place) {
+static void launch_mode(hipStream_t s, GemmP<T> &p, bool vec, int in_place, int grid_cap) {
   const long tiles128 = (long)ceil_div(p.M, 128) * ceil_div(p.N, 128);
   if (in_place) {
     // C is an operand (B := D B, or its transpose B^T := B^T D^T): safe only
@@ -273,24 +358,24 @@ static void launch_mode(hipStream_t s, GemmP<T> &p, bool vec, int in_place) {
     // the other dimension in 64-wide tiles and K in deep slabs: these
     // products are small and latency-bound (one round trip per slab)
     constexpr int BKI = sizeof(T) == 4 ? 64 : 32;
-    if (in_place == 2) launch_tiles<T, A_MN, B_MN, 128, 64, BKI>(s, p, vec);
-    else launch_tiles<T, A_MN, B_MN, 64, 128, BKI>(s, p, vec);
+    if (in_place == 2) launch_tiles<T, A_MN, B_MN, 128, 64, BKI>(s, p, vec, 0);
+    else launch_tiles<T, A_MN, B_MN, 64, 128, BKI>(s, p, vec, 0);
   } else if (tiles128 < 256 && p.K >= 32) {
     constexpr int BKS = sizeof(T) == 4 ? 64 : 32;
-    launch_tiles<T, A_MN, B_MN, 64, 64, BKS>(s, p, vec);
+    launch_tiles<T, A_MN, B_MN, 64, 64, BKS>(s, p, vec, grid_cap);
   } else {
-    launch_tiles<T, A_MN, B_MN, 128, 128, 16>(s, p, vec);
+    launch_tiles<T, A_MN, B_MN, 128, 128, 16>(s, p, vec, grid_cap);
   }
 }
 
 template <typename T>
 void gemm(hipStream_t s, long M, long N, long K, T alpha, View<const T> A, View<const T> B,
-          T beta, View<T> C, int tri) {
+          T beta, View<T> C, int tri, int grid_cap) {
   if (M <= 0 || N <= 0) return;
   if (C.rs != 1) {
     // Row-major C: compute C^T = B^T A^T into a column-major view.
     MG_CHECK(C.cs == 1 && tri == 0, "gemm: unsupported C layout");
-    gemm<T>(s, N, M, K, alpha, B.t(), A.t(), beta, View<T>{C.p, 1, C.rs}, false);
+    gemm<T>(s, N, M, K, alpha, B.t(), A.t(), beta, View<T>{C.p, 1, C.rs}, false, grid_cap);
     return;
   }
   MG_CHECK(M < (1L << 31) && N < (1L << 31) && K < (1L << 31), "gemm: dimension too large");
@@ -323,15 +408,15 @@ void gemm(hipStream_t s, long M, long N, long K, T alpha, View<const T> A, View<
   // of an inverted diagonal block); disjoint sub-views of one matrix, as in
   // the factorizations' updates, are not in place
   const int in_place = (const void *)C.p == (const void *)B.p ? 2 : ((const void *)C.p == (const void *)A.p ? 1 : 0);
-  if (a_mn && !b_mn) launch_mode<T, true, false>(s, p, vec, in_place);
-  else if (a_mn && b_mn) launch_mode<T, true, true>(s, p, vec, in_place);
-  else if (!a_mn && !b_mn) launch_mode<T, false, false>(s, p, vec, in_place);
-  else launch_mode<T, false, true>(s, p, vec, in_place);
+  if (a_mn && !b_mn) launch_mode<T, true, false>(s, p, vec, in_place, grid_cap);
+  else if (a_mn && b_mn) launch_mode<T, true, true>(s, p, vec, in_place, grid_cap);
+  else if (!a_mn && !b_mn) launch_mode<T, false, false>(s, p, vec, in_place, grid_cap);
+  else launch_mode<T, false, true>(s, p, vec, in_place, grid_cap);
 }
 
 template void gemm<float>(hipStream_t, long, long, long, float, View<const float>,
-                          View<const float>, float, View<float>, int);
+                          View<const float>, float, View<float>, int, int);
 template void gemm<double>(hipStream_t, long, long, long, double, View<const double>,
-                           View<const double>, double, View<double>, int);
+                           View<const double>, double, View<double>, int, int);
 
 }  // namespace mg
