@@ -180,12 +180,32 @@ struct LpP {
   int lower;
 };
 
+// Workgroup id -> (tm, tn).  The ids are regrouped by XCD; lower products
+// (square tiles) enumerate only the tiles on or below the diagonal, row by
+// row (the first tiles_n rows form a triangle, later rows hold tiles_n
+// tiles each), so every XCD gets an equal share of real work; other
+// products go in bands of GROUP_M tile rows.
 __device__ inline bool lp_map_tile(int bid, int grid, int tiles, int tiles_m, int tiles_n, int &tm,
-                                   int &tn) {
+                                   int &tn, int lower) {
   constexpr int XCD = 8, GROUP_M = 8;
   int id = bid;
   if (grid % XCD == 0) id = (bid % XCD) * (grid / XCD) + bid / XCD;
   if (id >= tiles) return false;
+  if (lower) {
+    const int tri = tiles_n * (tiles_n + 1) / 2;
+    if (id < tri) {
+      int r = (int)((sqrtf(8.0f * (float)id + 1.0f) - 1.0f) * 0.5f);
+      while ((r + 1) * (r + 2) / 2 <= id) ++r;
+      while (r * (r + 1) / 2 > id) --r;
+      tm = r;
+      tn = id - r * (r + 1) / 2;
+    } else {
+      const int q = id - tri;
+      tm = tiles_n + q / tiles_n;
+      tn = q % tiles_n;
+    }
+    return true;
+  }
   const int per_group = GROUP_M * tiles_n;
   const int g = id / per_group;
   const int first_m = g * GROUP_M;
@@ -204,7 +224,7 @@ __global__ __launch_bounds__(256) void gemm_bf16x_kernel(LpP p) {
   __shared__ unsigned short As[2][NP][BM * LDK];
   __shared__ unsigned short Bs[2][NP][BN * LDK];
   int tm, tn;
-  if (!lp_map_tile(blockIdx.x, gridDim.x, p.tiles, p.tiles_m, p.tiles_n, tm, tn)) return;
+  if (!lp_map_tile(blockIdx.x, gridDim.x, p.tiles, p.tiles_m, p.tiles_n, tm, tn, p.lower)) return;
   const int bm0 = tm * BM, bn0 = tn * BN;
   if (p.lower && bm0 + BM - 1 < bn0) return;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
@@ -325,7 +345,7 @@ __global__ __launch_bounds__(256) void gemm_fp8_kernel(F8P p) {
   __shared__ unsigned char As[BM * LDK];
   __shared__ unsigned char Bs[BN * LDK];
   int tm, tn;
-  if (!lp_map_tile(blockIdx.x, gridDim.x, p.tiles, p.tiles_m, p.tiles_n, tm, tn)) return;
+  if (!lp_map_tile(blockIdx.x, gridDim.x, p.tiles, p.tiles_m, p.tiles_n, tm, tn, p.lower)) return;
   const int bm0 = tm * BM, bn0 = tn * BN;
   if (p.lower && bm0 + BM - 1 < bn0) return;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
@@ -400,12 +420,19 @@ static long roundup(long x, long m) { return (x + m - 1) / m * m; }
 
 // Normalizes the problem so that C is column-major; returns false if C is
 // row-major and the caller must transpose the problem.
+// Tiles launched: all, or (lower) those on or below the diagonal.
+static int lp_tile_count(int tiles_m, int tiles_n, int lower) {
+  if (!lower) return tiles_m * tiles_n;
+  const long tm = tiles_m, tn = tiles_n, d = std::min(tm, tn);
+  return (int)(d * (d + 1) / 2 + std::max(0L, tm - tn) * tn);
+}
+
 void gemm_bf16x(hipStream_t s, long M, long N, long K, float alpha, View<const float> A,
-                View<const float> B, View<float> C, int passes) {
+                View<const float> B, View<float> C, int passes, bool lower_only) {
   if (M <= 0 || N <= 0 || K <= 0) return;
   if (C.rs != 1) {
-    MG_CHECK(C.cs == 1, "gemm_bf16x: unsupported C layout");
-    gemm_bf16x(s, N, M, K, alpha, B.t(), A.t(), View<float>{C.p, 1, C.rs}, passes);
+    MG_CHECK(C.cs == 1 && !lower_only, "gemm_bf16x: unsupported C layout");
+    gemm_bf16x(s, N, M, K, alpha, B.t(), A.t(), View<float>{C.p, 1, C.rs}, passes, false);
     return;
   }
   MG_CHECK(passes == 1 || passes == 3 || passes == 6, "gemm_bf16x: passes must be 1, 3 or 6");
@@ -436,8 +463,8 @@ void gemm_bf16x(hipStream_t s, long M, long N, long K, float alpha, View<const f
   p.alpha = alpha;
   p.tiles_m = (int)(Mp / LP_BM);
   p.tiles_n = (int)(Np / LP_BN);
-  p.tiles = p.tiles_m * p.tiles_n;
-  p.lower = 0;
+  p.lower = lower_only ? 1 : 0;
+  p.tiles = lp_tile_count(p.tiles_m, p.tiles_n, p.lower);
   int grid = p.tiles;
   if (grid >= 8) grid = (grid + 7) / 8 * 8;
   KScope ks(s, "gemm_bf16x", 2.0 * passes * M * N * K);
@@ -488,8 +515,8 @@ void gemm_fp8(hipStream_t s, long M, long N, long K, float alpha, View<const flo
   p.alpha = alpha;
   p.tiles_m = (int)(Mp / 128);
   p.tiles_n = (int)(Np / 128);
-  p.tiles = p.tiles_m * p.tiles_n;
   p.lower = lower_only ? 1 : 0;
+  p.tiles = lp_tile_count(p.tiles_m, p.tiles_n, p.lower);
   int grid = p.tiles;
   if (grid >= 8) grid = (grid + 7) / 8 * 8;
   KScope ks(s, "gemm_fp8", 2.0 * M * N * K);
