@@ -62,12 +62,8 @@ struct GemmP {
   T alpha, beta;
   int tiles_m, tiles_n, tiles;
   int lower;
-  // compact: only the tiles on or below the diagonal are enumerated (lower
-  // products with square tiles), so the ids dealt to the XCDs are all work
-  int compact;
-  // persistent: the grid is smaller than the tile count and each
-  // workgroup walks its XCD's share of tiles
-  int persist;
+  // tile count: all tiles, or for lower products with square tiles only the
+  // tiles on or below the diagonal (the ids dealt to the XCDs are all work)
 };
 
 // Loads one BK x MNT operand slab from global memory into registers.
@@ -169,58 +165,43 @@ __device__ inline void lower_tile(int id, int tiles_n, int &tm, int &tn) {
   }
 }
 
-// Tile id -> (tm, tn) for the compact or the banded order.
-__device__ inline void tile_of(const int id, int compact, int tiles_m, int tiles_n, int &tm,
-                               int &tn) {
-  if (compact) {
-    lower_tile(id, tiles_n, tm, tn);
-    return;
-  }
-  constexpr int GROUP_M = 8;
-  const int per_group = GROUP_M * tiles_n;
-  const int g = id / per_group;
-  const int first_m = g * GROUP_M;
-  const int gsz = min(tiles_m - first_m, GROUP_M);
-  const int in = id % per_group;
-  tm = first_m + in % gsz;
-  tn = in / gsz;
-}
-
 template <typename T, int BM, int BN, int BK, bool A_MN, bool B_MN, bool VEC>
 __device__ __forceinline__ void gemm_tile(const GemmP<T> &p, int tm, int tn, T (*As)[BK * (BM + 16 / sizeof(T))],
                                           T (*Bs)[BK * (BN + 16 / sizeof(T))]);
 
-template <typename T, int BM, int BN, int BK, bool A_MN, bool B_MN, bool VEC>
+// MAP 0: one tile per workgroup in the banded order; 1: one tile per
+// workgroup, compact lower order; 2: compact lower order, persistent (grid a
+// multiple of 8: XCD x = blockIdx % 8 owns the ids [x * per, (x + 1) * per),
+// per = tiles / 8 rounded up, which its workgroups take in turn; the last
+// slab's barrier retires every LDS reader before the next tile).  Separate
+// instantiations, so that the one-tile kernels keep their registers.
+template <typename T, int BM, int BN, int BK, bool A_MN, bool B_MN, bool VEC, int MAP>
 __global__ __launch_bounds__(256) void gemm_kernel(GemmP<T> p) {
   constexpr int PAD = 16 / (int)sizeof(T);
+  constexpr int XCD = 8;
   __shared__ T As[2][BK * (BM + PAD)];
   __shared__ T Bs[2][BK * (BN + PAD)];
-  if (!p.persist) {
+  if constexpr (MAP == 0) {
     int tm, tn;
-    if (p.compact) {
-      constexpr int XCD = 8;
-      const int grid = gridDim.x, bid = blockIdx.x;
-      const int id = grid % XCD == 0 ? (bid % XCD) * (grid / XCD) + bid / XCD : bid;
-      if (id >= p.tiles) return;
+    if (!map_tile(blockIdx.x, gridDim.x, p.tiles, p.tiles_m, p.tiles_n, tm, tn)) return;
+    gemm_tile<T, BM, BN, BK, A_MN, B_MN, VEC>(p, tm, tn, As, Bs);
+  } else if constexpr (MAP == 1) {
+    const int grid = gridDim.x, bid = blockIdx.x;
+    const int id = grid % XCD == 0 ? (bid % XCD) * (grid / XCD) + bid / XCD : bid;
+    if (id >= p.tiles) return;
+    int tm, tn;
+    lower_tile(id, p.tiles_n, tm, tn);
+    gemm_tile<T, BM, BN, BK, A_MN, B_MN, VEC>(p, tm, tn, As, Bs);
+  } else {
+    const int x = blockIdx.x % XCD, j = blockIdx.x / XCD, wgs = gridDim.x / XCD;
+    const int per = (p.tiles + XCD - 1) / XCD;
+    for (int k = j; k < per; k += wgs) {
+      const int id = x * per + k;
+      if (id >= p.tiles) break;
+      int tm, tn;
       lower_tile(id, p.tiles_n, tm, tn);
-    } else if (!map_tile(blockIdx.x, gridDim.x, p.tiles, p.tiles_m, p.tiles_n, tm, tn)) {
-      return;
+      gemm_tile<T, BM, BN, BK, A_MN, B_MN, VEC>(p, tm, tn, As, Bs);
     }
-    gemm_tile<T, BM, BN, BK, A_MN, B_MN, VEC>(p, tm, tn, As, Bs);
-    return;
-  }
-  // persistent (grid a multiple of 8): XCD x = blockIdx % 8 owns the ids
-  // [x * per, (x + 1) * per), per = tiles / 8 rounded up, which its
-  // workgroups take in turn
-  constexpr int XCD = 8;
-  const int x = blockIdx.x % XCD, j = blockIdx.x / XCD, wgs = gridDim.x / XCD;
-  const int per = (p.tiles + XCD - 1) / XCD;
-  for (int k = j; k < per; k += wgs) {
-    const int id = x * per + k;
-    if (id >= p.tiles) break;
-    int tm, tn;
-    tile_of(id, p.compact, p.tiles_m, p.tiles_n, tm, tn);
-    gemm_tile<T, BM, BN, BK, A_MN, B_MN, VEC>(p, tm, tn, As, Bs);
   }
 }
 
@@ -318,8 +299,8 @@ template <typename T, bool A_MN, bool B_MN, int BM, int BN, int BK>
 static void launch_tiles(hipStream_t s, GemmP<T> &p, bool vec, int grid_cap) {
   p.tiles_m = ceil_div(p.M, BM);
   p.tiles_n = ceil_div(p.N, BN);
-  p.compact = p.lower == 1 && BM == BN;
-  if (p.compact) {
+  const bool compact = p.lower == 1 && BM == BN;
+  if (compact) {
     const long tm = p.tiles_m, tn = p.tiles_n, d = std::min(tm, tn);
     p.tiles = (int)(d * (d + 1) / 2 + std::max(0L, tm - tn) * tn);
   } else {
@@ -327,16 +308,26 @@ static void launch_tiles(hipStream_t s, GemmP<T> &p, bool vec, int grid_cap) {
   }
   int grid = p.tiles;
   if (grid >= 8) grid = (grid + 7) / 8 * 8;
-  p.persist = 0;
-  if (grid_cap >= 8 && grid > grid_cap) {
-    grid = grid_cap / 8 * 8;
-    p.persist = 1;
-  }
+  // the persistent form exists for the large-tile lower products (SYRK)
+  constexpr bool can_persist = BM == 128 && BN == 128;
+  const bool persist = can_persist && compact && grid_cap >= 8 && grid > grid_cap;
+  if (persist) grid = grid_cap / 8 * 8;
   KScope ks(s, "gemm", (p.lower ? 1.0 : 2.0) * p.M * p.N * p.K);
-  if (vec)
-    hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, A_MN, B_MN, true>), dim3(grid), dim3(256), 0, s, p);
-  else
-    hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, A_MN, B_MN, false>), dim3(grid), dim3(256), 0, s, p);
+#define MG_GEMM_LAUNCH(MAP_)                                                                        \
+  {                                                                                                 \
+    if (vec)                                                                                        \
+      hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, A_MN, B_MN, true, MAP_>), dim3(grid), dim3(256), \
+                         0, s, p);                                                                  \
+    else                                                                                            \
+      hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, A_MN, B_MN, false, MAP_>), dim3(grid),          \
+                         dim3(256), 0, s, p);                                                       \
+  }
+  if constexpr (can_persist) {
+    if (persist) MG_GEMM_LAUNCH(2) else if (compact) MG_GEMM_LAUNCH(1) else MG_GEMM_LAUNCH(0)
+  } else {
+    if (compact) MG_GEMM_LAUNCH(1) else MG_GEMM_LAUNCH(0)
+  }
+#undef MG_GEMM_LAUNCH
   MG_HIP(hipGetLastError());
 }
 

@@ -185,12 +185,18 @@ struct LpP {
 // row (the first tiles_n rows form a triangle, later rows hold tiles_n
 // tiles each), so every XCD gets an equal share of real work; other
 // products go in bands of GROUP_M tile rows.
+__device__ inline void lp_tile_of(int id, int tiles_m, int tiles_n, int &tm, int &tn, int lower);
 __device__ inline bool lp_map_tile(int bid, int grid, int tiles, int tiles_m, int tiles_n, int &tm,
                                    int &tn, int lower) {
-  constexpr int XCD = 8, GROUP_M = 8;
+  constexpr int XCD = 8;
   int id = bid;
   if (grid % XCD == 0) id = (bid % XCD) * (grid / XCD) + bid / XCD;
   if (id >= tiles) return false;
+  lp_tile_of(id, tiles_m, tiles_n, tm, tn, lower);
+  return true;
+}
+__device__ inline void lp_tile_of(int id, int tiles_m, int tiles_n, int &tm, int &tn, int lower) {
+  constexpr int GROUP_M = 8;
   if (lower) {
     const int tri = tiles_n * (tiles_n + 1) / 2;
     if (id < tri) {
@@ -204,7 +210,7 @@ __device__ inline bool lp_map_tile(int bid, int grid, int tiles, int tiles_m, in
       tm = tiles_n + q / tiles_n;
       tn = q % tiles_n;
     }
-    return true;
+    return;
   }
   const int per_group = GROUP_M * tiles_n;
   const int g = id / per_group;
@@ -213,7 +219,6 @@ __device__ inline bool lp_map_tile(int bid, int grid, int tiles, int tiles_m, in
   const int in = id % per_group;
   tm = first_m + in % gsz;
   tn = in / gsz;
-  return true;
 }
 
 // 256 threads, 2x2 wavefronts, 64x64 per wavefront (2x2 MFMA 32x32x16).
