@@ -179,10 +179,11 @@ int mat_gpu_getrf_f32(void *stream, size_t n, float *a, size_t lda, int32_t *per
 int mat_gpu_getrf_f64(void *stream, size_t n, double *a, size_t lda, int32_t *perm, int leaf,
                       int *swaps);
 /* Same packed result with tournament pivoting (CALU) in the leaves of at
- * most 16384 rows: PA = LU with other pivots than libmat's partial
- * pivoting, and one exchange between workgroups per 32-column leaf instead
- * of one per column.  No counterpart in libmat (its LU is mat_plu,
- * mat.h:4436-4474); the mixed solver's factorization with pivoting = 2. */
+ * most 65536 rows: PA = LU with other pivots than libmat's partial
+ * pivoting, and the workgroups meeting once per level of a reduction tree
+ * per 32-column leaf instead of once per column.  No counterpart in libmat
+ * (its LU is mat_plu, mat.h:4436-4474); the mixed solver's factorization
+ * (pivoting = 2, the default). */
 int mat_gpu_getrf_tournament_f32(void *stream, size_t n, float *a, size_t lda, int32_t *perm,
                                  int *swaps);
 /* X = A^-1 B from mat_gpu_getrf's packed factor and perm; nrhs columns. */
@@ -233,14 +234,15 @@ typedef struct {
   int max_iter; /* default 30 */
   double tol;   /* default 1e-14 */
   int leaf;     /* LU pivoting block width in [1, 64], default 32 */
-  int pivoting; /* gesv only: 0 = partial pivoting, libmat's rule
-                   (default); 1 = recursive random butterfly transform +
-                   no pivoting (faster factorization, but its growth can
-                   stall refinement on ill-conditioned matrices);
-                   2 = tournament pivoting (CALU) for leaves of at most 32
-                   columns and 16384 rows (partial pivoting elsewhere):
-                   one exchange between workgroups per leaf instead of
-                   one per column */
+  int pivoting; /* gesv only: 0 = partial pivoting, libmat's rule;
+                   1 = recursive random butterfly transform + no pivoting
+                   (faster factorization, but its growth can stall
+                   refinement on ill-conditioned matrices); 2 = tournament
+                   pivoting (CALU, default) for leaves of at most 32
+                   columns and 65536 rows (partial pivoting elsewhere):
+                   the workgroups meet once per level of a reduction tree
+                   (three levels at 16384 rows) instead of once per
+                   column */
 } mat_gpu_ir_opts;
 typedef struct {
   int iterations;        /* refinement steps taken */

@@ -309,7 +309,7 @@ void mat_gpu_ir_defaults(mat_gpu_ir_opts *o, int spd) {
   o->max_iter = 30;
   o->tol = 1e-14;
   o->leaf = 32;
-  o->pivoting = 0;
+  o->pivoting = spd ? 0 : 2;
 }
 
 int mat_gpu_gesv_mixed(void *s, size_t n, size_t nrhs, const double *a, size_t lda, const double *b,

@@ -223,16 +223,38 @@ __device__ inline void lp_tile_of(int id, int tiles_m, int tiles_n, int &tm, int
 
 // 256 threads, 2x2 wavefronts, 64x64 per wavefront (2x2 MFMA 32x32x16).
 // NP planes per operand; PASSES products per K step (see file comment).
+//
+// Operands move global -> LDS with the direct load (global_load_lds, 16 bytes
+// a lane, no registers in between) into one 32-deep slab buffer per
+// workgroup (48 KB for three planes), so three workgroups share a compute
+// unit and one's products cover the others' loads.  A buffer holds 128 rows
+// of 32 bf16 (64 bytes), row r's 16-byte chunk c in slot c ^ h(r),
+// h(r) = ((r >> 2) ^ (r >> 1)) & 3, so that the fragment reads (32 rows, one
+// chunk) spread over all banks; a lane of the direct load writes slot
+// lane % 4 of row lane / 4 of its wavefront's 16-row block and so loads
+// chunk slot ^ h(row).
+// The MFMA takes B's fragment as its row operand and A's as its column
+// operand (it computes C^T), so in the accumulator a lane's 32 lanes run
+// along C's rows: each epilogue access of column-major C is one 128-byte
+// run per half-wavefront, and all of a lane's C values are read before any
+// is written.
+typedef __attribute__((address_space(3))) void lp_lds_void;
+typedef const __attribute__((address_space(1))) void lp_glob_void;
+
+__device__ __forceinline__ int lp_swz(int r) { return ((r >> 2) ^ (r >> 1)) & 3; }
+
 template <int NP, int PASSES>
-__global__ __launch_bounds__(256) void gemm_bf16x_kernel(LpP p) {
-  constexpr int BM = LP_BM, BN = LP_BN, BK = 32, LDK = BK + 8;
-  __shared__ unsigned short As[2][NP][BM * LDK];
-  __shared__ unsigned short Bs[2][NP][BN * LDK];
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+void gemm_bf16x_kernel(LpP p) {
+  constexpr int BM = LP_BM, BN = LP_BN, BK = 32;
+  constexpr int TILE = BM * BK;  // bf16 per plane and operand (8 KB)
+  __shared__ __align__(16) unsigned short sm[2][NP][TILE];  // [A, B][plane]
   int tm, tn;
   if (!lp_map_tile(blockIdx.x, gridDim.x, p.tiles, p.tiles_m, p.tiles_n, tm, tn, p.lower)) return;
   const int bm0 = tm * BM, bn0 = tn * BN;
   if (p.lower && bm0 + BM - 1 < bn0) return;
-  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int t = threadIdx.x, lane = t & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
   const int wm = wave & 1, wn = wave >> 1;
   const int g = lane >> 5, li = lane & 31;
 
@@ -244,93 +266,91 @@ __global__ __launch_bounds__(256) void gemm_bf16x_kernel(LpP p) {
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.f;
 
-  // global -> register staging: each thread moves two 16-byte row chunks
-  // per plane per operand
-  uint4 ra[NP][2], rb[NP][2];
-  const int srow = t >> 2, sch = (t & 3) * 8;
+  // direct loads: instruction q of wavefront w fills rows (4q + w) * 16 ..
+  // + 15 of a buffer; this lane's row and the global chunk it fetches
+  long goffA[2], goffB[2];
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const int r = (4 * q + wave) * 16 + (lane >> 2);
+    const int c = (lane & 3) ^ lp_swz(r);
+    goffA[q] = (long)(bm0 + r) * p.Kp + c * 8;
+    goffB[q] = (long)(bn0 + r) * p.Kp + c * 8;
+  }
+  // fragment offsets: row (x * 32 + li) of the wavefront's 64-row block,
+  // chunk 2 * ks + g
+  const int hl = lp_swz(li);
+  const int off0 = li * BK + ((0 + g) ^ hl) * 8, off1 = li * BK + ((2 + g) ^ hl) * 8;
+
   const int nk = p.Kp / BK;
-  const unsigned short *Ag = p.A + (long)(bm0 + srow) * p.Kp + sch;
-  const unsigned short *Bg = p.B + (long)(bn0 + srow) * p.Kp + sch;
-  const long rstride = 64L * p.Kp;
-#pragma unroll
-  for (int pl = 0; pl < NP; ++pl) {
-    ra[pl][0] = *reinterpret_cast<const uint4 *>(Ag + pl * p.planeA);
-    ra[pl][1] = *reinterpret_cast<const uint4 *>(Ag + pl * p.planeA + rstride);
-    rb[pl][0] = *reinterpret_cast<const uint4 *>(Bg + pl * p.planeB);
-    rb[pl][1] = *reinterpret_cast<const uint4 *>(Bg + pl * p.planeB + rstride);
-  }
-#pragma unroll
-  for (int pl = 0; pl < NP; ++pl) {
-    *reinterpret_cast<uint4 *>(&As[0][pl][srow * LDK + sch]) = ra[pl][0];
-    *reinterpret_cast<uint4 *>(&As[0][pl][(srow + 64) * LDK + sch]) = ra[pl][1];
-    *reinterpret_cast<uint4 *>(&Bs[0][pl][srow * LDK + sch]) = rb[pl][0];
-    *reinterpret_cast<uint4 *>(&Bs[0][pl][(srow + 64) * LDK + sch]) = rb[pl][1];
-  }
-  __syncthreads();
   for (int kt = 0; kt < nk; ++kt) {
-    const int cur = kt & 1;
-    const bool more = kt + 1 < nk;
-    const int k1 = more ? (kt + 1) * BK : 0;
-    // prefetch of the next slab (the last iteration re-reads slab 0, unused)
+    const int k0 = kt * BK;
 #pragma unroll
-    for (int pl = 0; pl < NP; ++pl) {
-      ra[pl][0] = *reinterpret_cast<const uint4 *>(Ag + pl * p.planeA + k1);
-      ra[pl][1] = *reinterpret_cast<const uint4 *>(Ag + pl * p.planeA + rstride + k1);
-      rb[pl][0] = *reinterpret_cast<const uint4 *>(Bg + pl * p.planeB + k1);
-      rb[pl][1] = *reinterpret_cast<const uint4 *>(Bg + pl * p.planeB + rstride + k1);
-    }
+    for (int pl = 0; pl < NP; ++pl)
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const int dst = (4 * q + wave) * 512;  // bf16 offset of the 1 KB block
+        __builtin_amdgcn_global_load_lds((lp_glob_void *)(p.A + pl * p.planeA + goffA[q] + k0),
+                                         (lp_lds_void *)&sm[0][pl][dst], 16, 0, 0);
+        __builtin_amdgcn_global_load_lds((lp_glob_void *)(p.B + pl * p.planeB + goffB[q] + k0),
+                                         (lp_lds_void *)&sm[1][pl][dst], 16, 0, 0);
+      }
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this slab is in LDS
+    __syncthreads();
 #pragma unroll
     for (int ks = 0; ks < BK / 16; ++ks) {
+      const int off = ks ? off1 : off0;
       bf16x8 fa[NP][2], fb[NP][2];
 #pragma unroll
       for (int pl = 0; pl < NP; ++pl)
 #pragma unroll
         for (int x = 0; x < 2; ++x) {
-          fa[pl][x] = *reinterpret_cast<const bf16x8 *>(&As[cur][pl][(wm * 64 + x * 32 + li) * LDK + ks * 16 + g * 8]);
-          fb[pl][x] = *reinterpret_cast<const bf16x8 *>(&Bs[cur][pl][(wn * 64 + x * 32 + li) * LDK + ks * 16 + g * 8]);
+          fa[pl][x] = *reinterpret_cast<const bf16x8 *>(&sm[0][pl][(wm * 64 + x * 32) * BK + off]);
+          fb[pl][x] = *reinterpret_cast<const bf16x8 *>(&sm[1][pl][(wn * 64 + x * 32) * BK + off]);
         }
-#pragma unroll
-      for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b) {
-          // small terms first, leading term last
-          if constexpr (PASSES >= 6) {
-            acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[0][a], fb[2 % NP][b], acc[a][b], 0, 0, 0);
-            acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[1 % NP][a], fb[1 % NP][b], acc[a][b], 0, 0, 0);
-            acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[2 % NP][a], fb[0][b], acc[a][b], 0, 0, 0);
-          }
-          if constexpr (PASSES >= 3) {
-            acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[0][a], fb[1 % NP][b], acc[a][b], 0, 0, 0);
-            acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[1 % NP][a], fb[0][b], acc[a][b], 0, 0, 0);
-          }
-          acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[0][a], fb[0][b], acc[a][b], 0, 0, 0);
-        }
-    }
-    if (more) {
-#pragma unroll
-      for (int pl = 0; pl < NP; ++pl) {
-        *reinterpret_cast<uint4 *>(&As[cur ^ 1][pl][srow * LDK + sch]) = ra[pl][0];
-        *reinterpret_cast<uint4 *>(&As[cur ^ 1][pl][(srow + 64) * LDK + sch]) = ra[pl][1];
-        *reinterpret_cast<uint4 *>(&Bs[cur ^ 1][pl][srow * LDK + sch]) = rb[pl][0];
-        *reinterpret_cast<uint4 *>(&Bs[cur ^ 1][pl][(srow + 64) * LDK + sch]) = rb[pl][1];
+      // small terms first, leading term last; the four accumulators take
+      // turns so that consecutive MFMAs are independent
+#define LP_PROD(PA, PB)                                                                   \
+  _Pragma("unroll") for (int a = 0; a < 2; ++a)                                           \
+  _Pragma("unroll") for (int b = 0; b < 2; ++b)                                           \
+      acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fb[PB][b], fa[PA][a], acc[a][b], 0, 0, 0);
+      if constexpr (PASSES >= 6) {
+        LP_PROD(0, 2 % NP)
+        LP_PROD(1 % NP, 1 % NP)
+        LP_PROD(2 % NP, 0)
       }
+      if constexpr (PASSES >= 3) {
+        LP_PROD(0, 1 % NP)
+        LP_PROD(1 % NP, 0)
+      }
+      LP_PROD(0, 0)
+#undef LP_PROD
     }
-    __syncthreads();
+    __syncthreads();  // every wavefront is done with the slab before the next one lands
   }
+  // acc[a][b][r] = C(i, j) with i = bm0 + wm*64 + a*32 + li (a lane's own
+  // row) and j = bn0 + wn*64 + b*32 + 8*(r/4) + 4*g + r%4
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < 2; ++a) {
+    const int i = bm0 + wm * 64 + a * 32 + li;
 #pragma unroll
     for (int b = 0; b < 2; ++b) {
-      const int j = bn0 + wn * 64 + b * 32 + li;
-      if (j >= p.N) continue;
+      float *cb = p.C + (long)i + (long)(bn0 + wn * 64 + b * 32 + 4 * g) * p.ldc;
+      const int jb = bn0 + wn * 64 + b * 32 + 4 * g;
+      float cv[16];
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
-        const int i = bm0 + wm * 64 + a * 32 + 8 * (r >> 2) + 4 * g + (r & 3);
-        if (i >= p.M || (p.lower && i < j)) continue;
-        float *c = p.C + (long)i + (long)j * p.ldc;
-        *c += p.alpha * acc[a][b][r];
+        const int jo = 8 * (r >> 2) + (r & 3);
+        const bool in = i < p.M && jb + jo < p.N && !(p.lower && i < jb + jo);
+        cv[r] = in ? cb[(long)jo * p.ldc] : 0.0f;
+      }
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int jo = 8 * (r >> 2) + (r & 3);
+        const bool in = i < p.M && jb + jo < p.N && !(p.lower && i < jb + jo);
+        if (in) cb[(long)jo * p.ldc] = cv[r] + p.alpha * acc[a][b][r];
       }
     }
+  }
 }
 
 // FP8: A, B packed e4m3 [mn][Kp]; scales per (128-tile, 128-K block).

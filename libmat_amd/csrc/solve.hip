@@ -394,7 +394,7 @@ int gesv_mixed(hipStream_t s, int n, int nrhs, const double *A, long lda, const 
   o.max_iter = 30;
   o.tol = 1e-14;
   o.leaf = 32;
-  o.pivoting = 0;
+  o.pivoting = 2;
   if (opt) o = *opt;
   MG_CHECK(o.passes == 0 || o.passes == 1 || o.passes == 3 || o.passes == 6,
            "gesv_mixed: passes must be 0 (fp32 MFMA), 1, 3 or 6");

@@ -9,30 +9,39 @@
 //
 // The panel is W[r0 : r0+m, c0 : c0+w] (w <= 32).  Partial pivoting needs a
 // reduction over all m rows for every column; here the rows are reduced in a
-// tree of "playoffs" instead, and the workgroups meet once per leaf:
-//   1. each wavefront holds up to 64*RPT0 rows in registers and picks w of
-//      them by Gaussian elimination with partial pivoting on its own rows
-//      (no interchanges: chosen rows just leave the competition);
-//   2. wavefront 0 plays the 4*w winners of its workgroup the same way;
-//   3. with more than one workgroup, every workgroup publishes its w
-//      winners (row index and original values, as tagged words) and every
-//      workgroup plays all G*w of them the same way (in two rounds when
-//      G > 4), so all of them know the w pivot rows, in order, and the
-//      factored w x w block [L11 \ U11] (the pivot rows as captured when
-//      chosen in the last round);
-//   4. each workgroup computes its own rows of L21 = A21 U11^-1 from their
+// tree of "playoffs" instead, and the workgroups meet once per tree level:
+//   1. workgroup b holds rows r0 + 256b .. +255, one per lane of its four
+//      wavefronts, and picks w of them by Gaussian elimination with partial
+//      pivoting on its own rows (no interchanges: a chosen row leaves the
+//      competition and is no longer updated, so it ends as its row of the
+//      factored block);
+//   2. the workgroups publish their winners' row indices; the last of every
+//      group of eight to arrive (an atomic ticket) plays the group's 256
+//      winners again on their original values, and so on up the tree (two
+//      levels above the leaves for up to 16384 rows), until one workgroup
+//      plays the final round, writes the factored w x w block [L11 \ U11]
+//      and the pivot rows, and raises a flag;
+//   3. every workgroup computes its rows of L21 = A21 U11^-1 from their
 //      original values (row-wise substitution with the operation order of
 //      the elimination: multiply by 1/pivot, then the rank-1 updates, and
-//      no scaling where |pivot| < eps), workgroup 0 writes [L11 \ U11] into
-//      the top w rows and the rows those displaced into the winners' old
+//      no scaling where |pivot| < eps); workgroup 0 writes [L11 \ U11] into
+//      the top w rows and the rows those displaced into the pivots' old
 //      places, and turns the pivot order into libmat's sequential
 //      interchanges (ipiv) and the leaf's row pairs.
-// Every playoff plays on original row values, so no workgroup writes the
-// matrix before all have read it: step 3 reads the published copies, and
-// the rows a workgroup overwrites in step 4 are its own or the window's.
+// A playoff runs on all four SIMDs of a compute unit: every step, each
+// wavefront finds its candidate (DPP max + ballot) and parks the
+// candidate's row in LDS, one barrier, every wavefront takes the best of the
+// four and the winner's row from LDS, and updates its own rows (about 1200
+// cycles a step, measured; a playoff in one wavefront issues one instruction
+// per four or more cycles, so 64 rows there cost about half as much per step
+// as 256 rows here).  Rows per lane are a parameter (TS_RPT): four rows per
+// lane took 2200 cycles a step, more than the level they save.
+// Every playoff reads original row values (the matrix is written only after
+// the final round), so no workgroup overwrites a row another one may read.
 #include <algorithm>
 #include <atomic>
 #include <cstdint>
+#include <utility>
 
 #include "ops.hpp"
 #include "pivot.hpp"
@@ -41,10 +50,14 @@ namespace mg {
 
 namespace {
 
-constexpr int TS_NT = 256;  // threads per workgroup (4 wavefronts)
-constexpr int TS_W = 32;    // widest leaf
-constexpr int TS_RS = 36;   // LDS row stride in floats: 16-byte aligned rows
-constexpr int TS_REC = 1 + TS_W;  // tagged words per published row: index, values
+constexpr int TS_NT = 256;   // threads per workgroup: 4 wavefronts
+constexpr int TS_RPT = 1;    // playing rows per lane
+constexpr int TS_PR = TS_NT * TS_RPT;  // rows (candidates) of one playoff
+constexpr int TS_W = 32;     // widest leaf
+constexpr int TS_RS = 36;    // LDS row stride in floats: 16-byte aligned rows
+constexpr int TS_FAN = TS_PR / TS_W;  // tree nodes per node of the next level (8)
+constexpr int TS_LMAX = 3;   // tree levels above the leaves: 256 -> 32 -> 4 -> 1
+constexpr int TS_GMAX = 256;  // workgroups (one per 256 rows), all resident
 
 struct TsluP {
   float *W;
@@ -52,37 +65,36 @@ struct TsluP {
   int r0, m, c0, w;  // panel rows r0 .. r0+m-1 (r0 == c0), columns c0 .. c0+w-1
   float eps;
   int *ipiv;
-  int *pairs;  // this leaf's row pairs (PAIR_STRIDE ints)
-  int *err;    // set to 1 on a poll timeout
-  uint64_t *rec;  // [G][TS_W][TS_REC] tagged words
+  int *pairs;     // this leaf's row pairs (PAIR_STRIDE ints)
+  int *err;       // set to 1 on a poll timeout
+  int *pub;       // [TS_LMAX][TS_GMAX][TS_W] winners' row indices per tree node
+  unsigned *cnt;  // [TS_LMAX][TS_GMAX] arrival tickets (wrap to 0 after the last)
+  float *F;       // [TS_W][TS_W] factored pivot rows of the final round
+  int *fin;       // [TS_W] pivot rows (original indices), in order
+  uint64_t *done; // tagged word: the final round is written
   unsigned tag;   // distinct per launch, never 0
-  int G;          // workgroups
+  int G;          // workgroups (one per 256 rows)
+  int P;          // leaf workgroups of the tree (one per TS_PR rows; = G here)
 };
 
-// Orders the wavefront's LDS traffic around a lane-to-lane hand-off (one
-// wavefront: the LDS serves its instructions in order).
-__device__ __forceinline__ void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-}
-
-// Wavefront max of non-negative int keys (-1: none): DPP within each
-// 16-lane row (one v_max_i32 with a DPP operand per level), then the four
-// row results on the scalar unit.
+// Wavefront max of int keys, in every lane: DPP within each 16-lane row (one
+// v_max_i32 with a DPP operand per level), then the rows combined by the
+// gfx950 half-wave swaps.
 template <int CTRL>
 __device__ __forceinline__ int dpp_maxi(int v) {
   // INT_MIN (the identity of max) as the old value lets the compiler fuse
   // the DPP move into the max; every lane of these patterns has a source
   return max(v, __builtin_amdgcn_update_dpp((int)0x80000000, v, CTRL, 0xf, 0xf, false));
 }
-__device__ __forceinline__ int wave_max_i32(int v) {
+__device__ __forceinline__ int wave_max_all(int v) {
   v = dpp_maxi<0xB1>(v);   // quad_perm(1,0,3,2)
   v = dpp_maxi<0x4E>(v);   // quad_perm(2,3,0,1)
   v = dpp_maxi<0x141>(v);  // row_half_mirror
   v = dpp_maxi<0x140>(v);  // row_mirror
-  const int r0 = __builtin_amdgcn_readlane(v, 0), r1 = __builtin_amdgcn_readlane(v, 16);
-  const int r2 = __builtin_amdgcn_readlane(v, 32), r3 = __builtin_amdgcn_readlane(v, 48);
-  return max(max(r0, r1), max(r2, r3));
+  const auto s32 = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+  v = max((int)s32[0], (int)s32[1]);
+  const auto s16 = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+  return max((int)s16[0], (int)s16[1]);
 }
 
 // 1/x for the multipliers: the hardware reciprocal (1 ulp).  The playoffs
@@ -90,532 +102,452 @@ __device__ __forceinline__ int wave_max_i32(int v) {
 // rows and the other rows of L come from the same 1/pivot.
 __device__ __forceinline__ float ts_recip(float x) { return __builtin_amdgcn_rcpf(x); }
 
-// One chunk of four slots of a step's rank-1 update: the pivot row's values
-// come from the winner lane (readlane) just before the update of the slot,
-// so only four scalars are live at a time.
-template <int RPT, int j, int k>
-__device__ __forceinline__ void ts_chunk(float (&a)[RPT][TS_W], const float (&l)[RPT], int wq, int wl) {
-  // slots j+1 .. of chunk k (none when the chunk ends at or before slot j)
-  constexpr int s0 = 4 * k > j ? 4 * k : j + 1;
+typedef float ts_f4 __attribute__((ext_vector_type(4)));
+
+// Per-step exchange of a workgroup playoff, double-buffered by step parity
+// (one barrier per step: a buffer is rewritten two steps later, after every
+// wavefront has passed the barrier that follows its reads).
+struct __align__(16) TsXchg {
+  float row[2][4][TS_RS];  // per wavefront: its candidate's row (columns J..)
+  unsigned key[2][4];      // per wavefront: (rank << 10) | (1023 - slot)
+};
+
+// The playing rows of one lane: candidate slot q * 256 + thread (q < TS_RPT).
+// a[q][] the row's current values (columns in natural order), live while
+// the row may still win, pick = the step that chose it (-1: none).
+struct TsRows {
+  float a[TS_RPT][TS_W];
+  bool live[TS_RPT];
+  int pick[TS_RPT];
+};
+
+// The candidate's row (columns J..) into its wavefront's exchange slot.
+template <int J, int Q>
+__device__ __forceinline__ void ts_park(const TsRows &r, float *dst) {
 #pragma unroll
-  for (int sl = s0; sl < 4 * k + 4; ++sl) {
-    float src = a[0][sl];
-#pragma unroll
-    for (int q = 1; q < RPT; ++q) src = wq == q ? a[q][sl] : src;
-    const float us = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(src), wl));
-#pragma unroll
-    for (int q = 0; q < RPT; ++q) a[q][sl] = fmaf(-us, l[q], a[q][sl]);
-  }
+  for (int c4 = J / 4; c4 < TS_W / 4; ++c4)
+    *reinterpret_cast<float4 *>(dst + 4 * c4) =
+        make_float4(r.a[Q][4 * c4], r.a[Q][4 * c4 + 1], r.a[Q][4 * c4 + 2], r.a[Q][4 * c4 + 3]);
 }
 
-// One elimination step of a playoff (see ts_play), column J = 4*g + j
-// (slot j of the current group); slots < live_n hold unfinished columns.
-template <int RPT, int j, bool KEEP>
-__device__ __forceinline__ bool ts_step(float (&a)[RPT][TS_W], bool (&live)[RPT], int J, int live_n,
-                                        float eps, int *win, float *rb, float *mult) {
-  const int lane = threadIdx.x & 63;
-  // keys: |value| bits (monotonic for non-negative floats), -1 for rows
-  // out of the competition and for NaN
-  int key[RPT];
+template <int J, int... Q>
+__device__ __forceinline__ void ts_parks(const TsRows &r, float *dst, bool cand, int wq,
+                                         std::integer_sequence<int, Q...>) {
+  ((cand && wq == Q ? ts_park<J, Q>(r, dst) : void()), ...);
+}
+
+// Step J of a workgroup playoff (FULL: w == 32, no step is skipped).  The
+// candidate of a wavefront is its live row of largest |value| in column J
+// (NaN never wins; a column of NaN takes the first live row), lowest slot
+// on ties.  The workgroup's winner is the best of the four by a 22-bit
+// rank, lowest slot on ties: the rank keeps the exponent and the top 14
+// mantissa bits of |value|, so a candidate within a relative 2^-14 of
+// another may win in its place (a pivot that large is as good for the
+// elimination, and every wavefront takes the same decision).  The winner
+// leaves the competition; every live row gets its multiplier (times
+// 1/pivot) and the rank-1 update, both skipped where |pivot| < eps
+// (libmat's rule).  Thread J's mywin gets the winner slot, -1 when no live
+// rows are left.  qlive[q]: some row q of this wavefront was live when the
+// playoff began (rows q of the wavefront that are all dead skip the update).
+template <int J, bool FULL>
+__device__ __forceinline__ void ts_step(TsRows &r, const bool (&qlive)[TS_RPT], int w, float eps,
+                                        TsXchg &x, int &mywin) {
+  if (!FULL && J >= w) return;
+  const int t = threadIdx.x, lane = t & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
+  constexpr int par = J & 1;
+  int key[TS_RPT];
+  int kbest = -1;
 #pragma unroll
-  for (int q = 0; q < RPT; ++q) {
-    const float v = a[q][j];
-    key[q] = (live[q] && v == v) ? (__float_as_int(v) & 0x7fffffff) : -1;
+  for (int q = 0; q < TS_RPT; ++q) {
+    const float v = r.a[q][J];
+    key[q] = (r.live[q] && v == v) ? (__float_as_int(v) & 0x7fffffff) : -1;
+    kbest = max(kbest, key[q]);
   }
-  int best = key[0];
+  const int mx = wave_max_all(kbest);
+  unsigned long long msk = 0;
+  int wq = 0;
 #pragma unroll
-  for (int q = 1; q < RPT; ++q) best = max(best, key[q]);
-  const int mx = wave_max_i32(best);
-  int wq = -1, wl = 0;
-#pragma unroll
-  for (int q = RPT - 1; q >= 0; --q) {
-    const unsigned long long msk = mx >= 0 ? __ballot(key[q] == mx) : __ballot(live[q]);
-    if (msk) {
+  for (int q = TS_RPT - 1; q >= 0; --q) {
+    const unsigned long long m = __ballot(mx >= 0 ? key[q] == mx : r.live[q]);
+    if (m) {
+      msk = m;
       wq = q;
-      wl = __builtin_ctzll(msk);
     }
   }
-  if (wq < 0) return false;  // no rows left
-  if (lane == 0) win[J] = wq * 64 + wl;
-  float pvr = a[0][j];
-#pragma unroll
-  for (int q = 1; q < RPT; ++q) pvr = wq == q ? a[q][j] : pvr;
-  const float pv = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pvr), wl));
-#pragma unroll
-  for (int q = 0; q < RPT; ++q) {
-    if (q == wq && lane == wl) {
-      live[q] = false;
-      if (KEEP) {
-        // the row's current values as picked, in slot order (columns from J
-        // on are exact; nothing in this playoff reads them back)
-        float4 *row4 = reinterpret_cast<float4 *>(rb) + J * (TS_RS / 4);
-#pragma unroll
-        for (int i = 0; i < TS_W / 4; ++i)
-          row4[i] = make_float4(a[q][4 * i], a[q][4 * i + 1], a[q][4 * i + 2], a[q][4 * i + 3]);
-      }
-    }
+  const int wl = msk ? (int)__builtin_ctzll(msk) : 0;
+  // the candidate parks its row: one guarded block per row index q (as
+  // separate blocks, not alternatives of one branch, the four stay apart and
+  // every register index stays static)
+  ts_parks<J>(r, x.row[par][wave], msk != 0 && lane == wl, wq,
+              std::make_integer_sequence<int, TS_RPT>{});
+  if (lane == wl) {
+    // the candidate (lane 0 when there is none) parks the key
+    const unsigned rank = msk == 0 ? 0u : (mx >= 0 ? ((unsigned)mx >> 9) + 2u : 1u);
+    x.key[par][wave] = (rank << 10) | (unsigned)(1023 - (wq * TS_NT + wave * 64 + wl));
   }
-  // |pivot| < eps: libmat's skip (no scaling, no update); the multiplier
-  // then is the value itself
-  const bool elim = !(fabsf(pv) < eps);
-  const float inv = elim ? ts_recip(pv) : 1.0f;
-  float l[RPT];
-#pragma unroll
-  for (int q = 0; q < RPT; ++q) {
-    l[q] = a[q][j] * inv;
-    a[q][j] = l[q];
-    if (KEEP) mult[(q * 64 + lane) * (TS_W + 1) + J] = l[q];
-  }
-  if (elim) {
-    // chunks holding only finished columns are skipped (uniform branches;
-    // independent ones, so every chunk updates its registers in place)
-    if (4 * (j / 4) < live_n) ts_chunk<RPT, j, j / 4>(a, l, wq, wl);
-    if (4 < live_n) ts_chunk<RPT, j, 1>(a, l, wq, wl);
-    if (8 < live_n) ts_chunk<RPT, j, 2>(a, l, wq, wl);
-    if (12 < live_n) ts_chunk<RPT, j, 3>(a, l, wq, wl);
-    if (16 < live_n) ts_chunk<RPT, j, 4>(a, l, wq, wl);
-    if (20 < live_n) ts_chunk<RPT, j, 5>(a, l, wq, wl);
-    if (24 < live_n) ts_chunk<RPT, j, 6>(a, l, wq, wl);
-    if (28 < live_n) ts_chunk<RPT, j, 7>(a, l, wq, wl);
-  }
-  return true;
-}
-
-// A playoff on the wavefront's candidate rows a[q][.] (RPT per lane;
-// live[q]: a real row): Gaussian elimination with partial pivoting but no
-// interchanges.  Step J picks the live row of largest |value| in column J
-// (first (q, lane) on ties; NaN never wins, and a column of NaN takes the
-// first live row), takes it out of the competition, and applies its
-// multiplier and rank-1 update to every row (rows already out are updated
-// too: only their values at the time they were picked are used).
-// Columns go in groups of four: within a group every slot is static, and
-// between groups the registers rotate by four, so group g holds column
-// (4g + s) mod 32 in slot s while the loop over groups stays a runtime loop
-// (fully unrolled steps would not fit the instruction cache).  Slots of
-// finished columns take garbage updates; nothing reads them.
-// Outputs: win[J] = the winner's slot q*64+lane (-1 once the rows run out)
-// and, with KEEP, rb[J] = its row as picked in the slot order of its group
-// (exact from column J on) and mult[slot][J] = every row's multiplier of
-// column J (row stride TS_W + 1).
-template <int RPT, bool KEEP>
-__device__ __forceinline__ void ts_play(float (&a)[RPT][TS_W], bool (&live)[RPT], int w, float eps,
-                                        int *win, float *rb, float *mult) {
-  const int lane = threadIdx.x & 63;
-  if (lane < TS_W) win[lane] = -1;
-  wave_lds_sync();
-#pragma unroll 1
-  for (int g = 0; 4 * g < w; ++g) {
-    const int J = 4 * g, live_n = w - 4 * g;  // slots < live_n: unfinished columns
-    const bool go = ts_step<RPT, 0, KEEP>(a, live, J, live_n, eps, win, rb, mult) && J + 1 < w &&
-                    ts_step<RPT, 1, KEEP>(a, live, J + 1, live_n, eps, win, rb, mult) && J + 2 < w &&
-                    ts_step<RPT, 2, KEEP>(a, live, J + 2, live_n, eps, win, rb, mult) && J + 3 < w &&
-                    ts_step<RPT, 3, KEEP>(a, live, J + 3, live_n, eps, win, rb, mult);
-    (void)go;
-#pragma unroll
-    for (int q = 0; q < RPT; ++q) {
-      const float t0 = a[q][0], t1 = a[q][1], t2 = a[q][2], t3 = a[q][3];
-#pragma unroll
-      for (int sl = 0; sl + 4 < TS_W; ++sl) a[q][sl] = a[q][sl + 4];
-      a[q][TS_W - 4] = t0;
-      a[q][TS_W - 3] = t1;
-      a[q][TS_W - 2] = t2;
-      a[q][TS_W - 1] = t3;
-    }
-  }
-  wave_lds_sync();
-}
-
-// The factored pivot rows of a KEEP playoff in natural column order:
-// Fn[J][c] = multiplier of column c < J (from mult, the winner's row), else
-// the winner's value as picked (from rb).  One wavefront; w <= 32.
-__device__ __forceinline__ void ts_factored_rows(const int *win, const float *rb, const float *mult,
-                                                 int w, float *Fn) {
-  const int lane = threadIdx.x & 63;
-  for (int x = lane; x < w * TS_W; x += 64) {
-    const int J = x / TS_W, c = x % TS_W;
-    const int ws = win[J];
-    float v = 0.0f;
-    if (ws >= 0)
-      v = c < J ? mult[ws * (TS_W + 1) + c] : rb[J * TS_RS + ((c - (J & ~3)) & (TS_W - 1))];
-    Fn[J * TS_RS + c] = v;
-  }
-  wave_lds_sync();
-}
-
-// Polls one published row (TS_REC tagged words) until every word carries
-// the tag; returns false on timeout.  Payloads go to out[0 .. TS_REC).
-__device__ __forceinline__ bool ts_fetch(const uint64_t *src, unsigned tag, uint32_t *out) {
-  uint64_t wd[TS_REC];
-#pragma unroll
-  for (int h = 0; h < TS_REC; ++h)
-    wd[h] = __hip_atomic_load(src + h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  long spins = 0;
-  while (true) {
-    bool ok = true;
-#pragma unroll
-    for (int h = 0; h < TS_REC; ++h) ok = ok && (uint32_t)(wd[h] >> 32) == tag;
-    if (ok) break;
-    if (++spins > SPIN_LIMIT) return false;
-    __builtin_amdgcn_s_sleep(1);
-#pragma unroll
-    for (int h = 0; h < TS_REC; ++h)
-      if ((uint32_t)(wd[h] >> 32) != tag)
-        wd[h] = __hip_atomic_load(src + h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-#pragma unroll
-  for (int h = 0; h < TS_REC; ++h) out[h] = (uint32_t)wd[h];
-  return true;
-}
-
-// x := x U^-1 for NR rows held in registers, from the prepared pivot rows
-// uc (LDS, row stride TS_RS): uc[J][0] = 1/u_JJ and uc[J][s] = u_J,J+s for
-// the unfinished columns, zero elsewhere; a pivot with |u_JJ| < eps has
-// uc[J][0] = 1 and a zero row, which reproduces the elimination's skip (no
-// scaling, no update).  Operation order as in the elimination: x_J *= 1/u_JJ,
-// then x_c -= x_J u_Jc.  Same register rotation as the playoff: on return
-// column c < w is in slot (c - w) mod 32.
-template <int NR>
-__device__ __forceinline__ void ts_subst(float (&x)[NR][TS_W], const float *uc, int w) {
-  const float4 *uc4 = reinterpret_cast<const float4 *>(uc);
-#pragma unroll 1
-  for (int J = 0; J < w; ++J) {
-    float u[TS_W];
-#pragma unroll
-    for (int i = 0; i < TS_W / 4; ++i) {
-      const float4 v = uc4[J * (TS_RS / 4) + i];
-      u[4 * i] = v.x;
-      u[4 * i + 1] = v.y;
-      u[4 * i + 2] = v.z;
-      u[4 * i + 3] = v.w;
-    }
-#pragma unroll
-    for (int r = 0; r < NR; ++r) {
-      const float l = x[r][0] * u[0];
-#pragma unroll
-      for (int c = 1; c < TS_W; ++c) x[r][c - 1] = fmaf(-u[c], l, x[r][c]);
-      x[r][TS_W - 1] = l;
-    }
-  }
-}
-
-__device__ __forceinline__ void ts_load_row(float (&x)[TS_W], const float *row) {
-  const float4 *s = reinterpret_cast<const float4 *>(row);
-#pragma unroll
-  for (int i = 0; i < TS_W / 4; ++i) {
-    const float4 v = s[i];
-    x[4 * i] = v.x;
-    x[4 * i + 1] = v.y;
-    x[4 * i + 2] = v.z;
-    x[4 * i + 3] = v.w;
-  }
-}
-
-// RPT0: rows per lane in the first playoff (a workgroup holds 256*RPT0
-// rows); RPTF: rows per lane in the first cross-workgroup round (G > 4).
-template <int RPT0, int RPTF>
-__global__ __launch_bounds__(TS_NT) void tslu_kernel(TsluP p) {
-  constexpr int ROWS = TS_NT * RPT0;
-  const int t = threadIdx.x, lane = t & 63, wave = t >> 6, b = blockIdx.x;
-  const int w = p.w, G = p.G;
-  const int base = p.r0 + b * ROWS;
-  const int rend = p.r0 + p.m;
-
-  __shared__ __align__(16) float orig[ROWS * TS_RS];    // own rows, original values
-  __shared__ __align__(16) float rb[4][TS_W * TS_RS];   // per-wavefront pivot rows
-  __shared__ __align__(16) float cand[4 * TS_W * TS_RS];  // second-round candidates
-  __shared__ float mult[2 * 64 * (TS_W + 1)];              // last playoff's multipliers
-  __shared__ __align__(16) float Fn[TS_W * TS_RS];         // factored pivot rows
-  __shared__ int win[4][TS_W];
-  __shared__ int cid[4 * TS_W];  // candidate row indices of the current round
-  __shared__ int fin[TS_W];      // pivot rows, in order
-  __shared__ int s_pairs[PAIR_STRIDE];
-  __shared__ unsigned char skip[ROWS];
-  __shared__ int s_abort;
-
-  if (t == 0) s_abort = 0;
-  // 1. own rows: registers for the playoff, LDS for the original values
-  float a[RPT0][TS_W];
-  bool live[RPT0];
-#pragma unroll
-  for (int q = 0; q < RPT0; ++q) {
-    const int li = (wave * RPT0 + q) * 64 + lane;
-    const int row = base + li;
-    live[q] = row < rend;
-    const float *src = p.W + (long)row * p.ld + p.c0;
-#pragma unroll
-    for (int c = 0; c < TS_W; ++c) a[q][c] = (live[q] && c < w) ? src[c] : 0.0f;
-    float4 *o = reinterpret_cast<float4 *>(orig + li * TS_RS);
-#pragma unroll
-    for (int i = 0; i < TS_W / 4; ++i)
-      o[i] = make_float4(a[q][4 * i], a[q][4 * i + 1], a[q][4 * i + 2], a[q][4 * i + 3]);
-    skip[li] = (row < p.r0 + w) ? 1 : 0;  // the window: written by workgroup 0
-  }
-  ts_play<RPT0, false>(a, live, w, p.eps, win[wave], rb[wave], mult);
   __syncthreads();
-
-  // 2. the workgroup's playoff: wavefront 0 over the 4*w first-round winners
-  if (wave == 0) {
-    float a1[2][TS_W];
-    bool l1[2];
+  const uint4 kq = *reinterpret_cast<const uint4 *>(x.key[par]);
+  const unsigned best =
+      __builtin_amdgcn_readfirstlane(max(max(kq.x, kq.y), max(kq.z, kq.w)));
+  const bool found = (best >> 10) != 0;
+  const int slot = 1023 - (int)(best & 1023u);
+  const int ww = (slot >> 6) & 3, wqq = slot >> 8, wlw = slot & 63;
+  mywin = t == J ? (found ? slot : -1) : mywin;
+  // the winner's row in whole 16-byte chunks (the empty asm marks every
+  // element as used, so each chunk stays one ds_read_b128 even where its
+  // first elements are not needed)
+  float u[TS_W];
+  const ts_f4 *src = reinterpret_cast<const ts_f4 *>(x.row[par][ww]);
+  ts_f4 qv[TS_W / 4];
 #pragma unroll
-    for (int q = 0; q < 2; ++q) {
-      const int ci = q * 64 + lane;
-      const int v = ci / TS_W, j = ci % TS_W;
-      const int s = j < w ? win[v][j] : -1;
-      const int li = v * RPT0 * 64 + s;
-      l1[q] = s >= 0;
-      cid[ci] = s >= 0 ? base + li : -1;
-      if (s >= 0) ts_load_row(a1[q], orig + li * TS_RS);
+  for (int c4 = J / 4; c4 < TS_W / 4; ++c4) qv[c4] = src[c4];
+#pragma unroll
+  for (int c4 = J / 4; c4 < TS_W / 4; ++c4) {
+    asm volatile("" : "+v"(qv[c4]));
+    u[4 * c4] = qv[c4].x;
+    u[4 * c4 + 1] = qv[c4].y;
+    u[4 * c4 + 2] = qv[c4].z;
+    u[4 * c4 + 3] = qv[c4].w;
+  }
+  const bool mine = found && wave == ww && lane == wlw;
+  const float pv = u[J];
+  const bool elim = found && !(fabsf(pv) < eps);
+  const float inv = elim ? ts_recip(pv) : 1.0f;
+#pragma unroll
+  for (int q = 0; q < TS_RPT; ++q) {
+    if (!qlive[q]) continue;
+    const bool me = mine && q == wqq;
+    r.pick[q] = me ? J : r.pick[q];
+    r.live[q] = r.live[q] && !me;
+    const float l = r.a[q][J] * inv;
+    const float lm = (r.live[q] && elim) ? l : 0.0f;
+    r.a[q][J] = r.live[q] ? l : r.a[q][J];
+#pragma unroll
+    for (int c = J + 1; c < TS_W; ++c) r.a[q][c] = fmaf(-u[c], lm, r.a[q][c]);
+  }
+}
+
+template <bool FULL, int... J>
+__device__ __forceinline__ void ts_steps(TsRows &r, const bool (&qlive)[TS_RPT], int w, float eps,
+                                         TsXchg &x, int &mywin, std::integer_sequence<int, J...>) {
+  (ts_step<J, FULL>(r, qlive, w, eps, x, mywin), ...);
+}
+
+// A playoff of the workgroup's TS_PR candidate slots (r.live marks the real
+// ones); on return win[0 .. w) holds the winner slots.
+__device__ __forceinline__ void ts_play(TsRows &r, int w, float eps, TsXchg &x, int *win) {
+  bool qlive[TS_RPT];
+#pragma unroll
+  for (int q = 0; q < TS_RPT; ++q) {
+    r.pick[q] = -1;
+    qlive[q] = __ballot(r.live[q]) != 0;
+  }
+  int mywin = -1;
+  if (w == TS_W)
+    ts_steps<true>(r, qlive, w, eps, x, mywin, std::make_integer_sequence<int, TS_W>{});
+  else
+    ts_steps<false>(r, qlive, w, eps, x, mywin, std::make_integer_sequence<int, TS_W>{});
+  if (threadIdx.x < TS_W) win[threadIdx.x] = mywin;
+  __syncthreads();
+}
+
+// x := x U^-1 for the lane's row, from the final round's pivot rows in LDS,
+// row J shifted to start at its diagonal: us[J][0] = 1/u_JJ (1 where
+// |u_JJ| < eps) and us[J][k] = u_J,J+k for k >= 1 (zero where |u_JJ| < eps
+// or J + k >= w), which reproduces the elimination: x_J *= 1/u_JJ, then
+// x_c -= x_J u_Jc.  Rows J >= w are the identity's (x_J *= 1, no update),
+// so every step runs and no branch keeps the loads of later steps from
+// being issued early.
+template <int J>
+__device__ __forceinline__ void ts_sub(float (&x)[TS_W], const float *us) {
+  constexpr int K = TS_W - J;  // entries J .. 31
+  const float4 *row = reinterpret_cast<const float4 *>(us + J * TS_RS);
+  float q[(K + 3) / 4 * 4];
+#pragma unroll
+  for (int k4 = 0; k4 < (K + 3) / 4; ++k4) {
+    const float4 v = row[k4];
+    q[4 * k4] = v.x;
+    q[4 * k4 + 1] = v.y;
+    q[4 * k4 + 2] = v.z;
+    q[4 * k4 + 3] = v.w;
+  }
+  const float l = x[J] * q[0];
+  x[J] = l;
+#pragma unroll
+  for (int k = 1; k < K; ++k) x[J + k] = fmaf(-q[k], l, x[J + k]);
+}
+template <int... J>
+__device__ __forceinline__ void ts_subs(float (&x)[TS_W], const float *un,
+                                        std::integer_sequence<int, J...>) {
+  (ts_sub<J>(x, un), ...);
+}
+
+// Rows of the leaf between registers and memory: 16-byte vectors when every
+// row start is aligned (vec), columns >= w as zeros.
+__device__ __forceinline__ void ts_load_row(float (&x)[TS_W], const float *row, int w, bool vec) {
+  if (vec && w == TS_W) {
+#pragma unroll
+    for (int c4 = 0; c4 < TS_W / 4; ++c4) {
+      const float4 q = *reinterpret_cast<const float4 *>(row + 4 * c4);
+      x[4 * c4] = q.x;
+      x[4 * c4 + 1] = q.y;
+      x[4 * c4 + 2] = q.z;
+      x[4 * c4 + 3] = q.w;
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < TS_W; ++c) x[c] = c < w ? row[c] : 0.0f;
+  }
+}
+__device__ __forceinline__ void ts_store_row(float *row, const float (&x)[TS_W], int w, bool vec) {
+  if (vec && w == TS_W) {
+#pragma unroll
+    for (int c4 = 0; c4 < TS_W / 4; ++c4)
+      *reinterpret_cast<float4 *>(row + 4 * c4) =
+          make_float4(x[4 * c4], x[4 * c4 + 1], x[4 * c4 + 2], x[4 * c4 + 3]);
+  } else {
+#pragma unroll
+    for (int c = 0; c < TS_W; ++c)
+      if (c < w) row[c] = x[c];
+  }
+}
+
+__global__ __launch_bounds__(TS_NT) void tslu_kernel(TsluP p) {
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6, b = blockIdx.x;
+  const int w = p.w;
+  const int base = p.r0 + b * TS_NT;
+  const int rend = p.r0 + p.m;
+  const int myrow = base + t;
+  const bool mine = myrow < rend;
+  // 16-byte row accesses when the leaf's rows start 16-byte aligned
+  const bool vec = ((p.ld & 3) == 0) && ((p.c0 & 3) == 0) &&
+                   ((reinterpret_cast<uintptr_t>(p.W) & 15) == 0);
+
+  __shared__ TsXchg xg;
+  // own rows (original values), then the final pivot rows
+  __shared__ __align__(16) float orig[(TS_NT + TS_W) * TS_RS];
+  __shared__ __align__(16) float un[TS_W * TS_RS];  // final pivot rows, for the substitution
+  __shared__ int cid[TS_PR];     // candidate row of every slot in the current playoff
+  __shared__ int win[TS_W];      // winner slots of the current playoff
+  __shared__ int fin[TS_W];      // pivot rows, in order
+  __shared__ int windst[TS_W];   // where each window row goes (-1: into the window)
+  __shared__ int s_pairs[PAIR_STRIDE];
+  __shared__ unsigned char skip[TS_NT];
+  __shared__ int s_go;
+
+  // 1. own rows, original values, for the substitution
+  {
+    float x[TS_W];
+    if (mine) ts_load_row(x, p.W + (long)myrow * p.ld + p.c0, w, vec);
+    else
+#pragma unroll
+      for (int c = 0; c < TS_W; ++c) x[c] = 0.0f;
+#pragma unroll
+    for (int c4 = 0; c4 < TS_W / 4; ++c4)
+      *reinterpret_cast<float4 *>(orig + t * TS_RS + 4 * c4) =
+          make_float4(x[4 * c4], x[4 * c4 + 1], x[4 * c4 + 2], x[4 * c4 + 3]);
+    skip[t] = 0;
+  }
+
+  // 2. the tree: the leaf workgroups play their rows, publish, and carry on as
+  //    the next level's node when last to arrive (one playoff site, so its
+  //    code is generated once)
+  if (b < p.P) {
+    TsRows r;
+#pragma unroll
+    for (int q = 0; q < TS_RPT; ++q) {
+      const int row = p.r0 + b * TS_PR + q * TS_NT + t;
+      r.live[q] = row < rend;
+      cid[q * TS_NT + t] = r.live[q] ? row : -1;
+      if (r.live[q]) ts_load_row(r.a[q], p.W + (long)row * p.ld + p.c0, w, vec);
       else
 #pragma unroll
-        for (int c = 0; c < TS_W; ++c) a1[q][c] = 0.0f;
+        for (int c = 0; c < TS_W; ++c) r.a[q][c] = 0.0f;
     }
-    if (G == 1) {
-      ts_play<2, true>(a1, l1, w, p.eps, win[0], rb[0], mult);
-      ts_factored_rows(win[0], rb[0], mult, w, Fn);
-    } else {
-      ts_play<2, false>(a1, l1, w, p.eps, win[0], rb[0], mult);
-    }
-    if (lane < TS_W) {
-      const int s = win[0][lane];
-      fin[lane] = s >= 0 ? cid[s] : -1;
-    }
-  }
-  __syncthreads();
-
-  if (G > 1) {
-    // 3a. publish the workgroup's winners: index and original values
-    uint64_t *mine = p.rec + (long)b * TS_W * TS_REC;
-    for (int x = t; x < TS_W * TS_REC; x += TS_NT) {
-      const int j = x / TS_REC, h = x % TS_REC;
-      const int row = fin[j];
-      uint32_t pay;
-      if (h == 0) pay = (uint32_t)row;
-      else pay = row >= 0 ? __float_as_uint(orig[(row - base) * TS_RS + h - 1]) : 0u;
-      __hip_atomic_store(mine + x, tagged(p.tag, pay), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    __syncthreads();  // fin is rewritten below
-    bool timeout = false;
-    if (G <= 4) {
-      // 3b. one round: wavefront 0 plays all G*w published rows
+    int level = 0, node = b, nodes = p.P;
+    while (true) {
+      const bool top = nodes == 1;
+      ts_play(r, w, p.eps, xg, win);
+      if (top) {
+        // the final round: factored pivot rows and their indices
+#pragma unroll
+        for (int q = 0; q < TS_RPT; ++q) {
+          if (r.pick[q] >= 0) {
+            float *d = p.F + r.pick[q] * TS_W;
+#pragma unroll
+            for (int c4 = 0; c4 < TS_W / 4; ++c4)
+              *reinterpret_cast<float4 *>(d + 4 * c4) = make_float4(
+                  r.a[q][4 * c4], r.a[q][4 * c4 + 1], r.a[q][4 * c4 + 2], r.a[q][4 * c4 + 3]);
+            p.fin[r.pick[q]] = cid[q * TS_NT + t];
+          }
+        }
+        if (t >= w && t < TS_W) p.fin[t] = -1;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __syncthreads();
+        if (t == 0)
+          __hip_atomic_store(p.done, tagged(p.tag, 1u), __ATOMIC_RELEASE,
+                             __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+      if (t < TS_W) {
+        const int s = t < w ? win[t] : -1;
+        p.pub[((long)level * TS_GMAX + node) * TS_W + t] = s >= 0 ? cid[s] : -1;
+      }
+      const int parent = node / TS_FAN;
+      const int members = min(TS_FAN, nodes - parent * TS_FAN);
       if (wave == 0) {
-        float a2[2][TS_W];
-        bool l2[2];
-#pragma unroll
-        for (int q = 0; q < 2; ++q) {
-          const int ci = q * 64 + lane;
-          const int g = ci / TS_W, j = ci % TS_W;
-          l2[q] = false;
-          cid[ci] = -1;
-#pragma unroll
-          for (int c = 0; c < TS_W; ++c) a2[q][c] = 0.0f;
-          if (g < G && j < w) {
-            uint32_t wd[TS_REC];
-            if (ts_fetch(p.rec + ((long)g * TS_W + j) * TS_REC, p.tag, wd)) {
-              const int row = (int)wd[0];
-              l2[q] = row >= 0;
-              cid[ci] = row;
-#pragma unroll
-              for (int c = 0; c < TS_W; ++c) a2[q][c] = __uint_as_float(wd[1 + c]);
-            } else {
-              timeout = true;
-            }
-          }
-        }
-        if (__any(timeout)) {
-          if (lane == 0) {
-            s_abort = 1;
-            __hip_atomic_store(p.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          }
-        } else {
-          ts_play<2, true>(a2, l2, w, p.eps, win[0], rb[0], mult);
-          ts_factored_rows(win[0], rb[0], mult, w, Fn);
-          if (lane < TS_W) {
-            const int s = win[0][lane];
-            fin[lane] = s >= 0 ? cid[s] : -1;
-          }
-        }
-      }
-    } else {
-      // 3b. two rounds: wavefront v plays the rows of workgroups
-      //     [v*K, v*K+K), then wavefront 0 plays the 4*w survivors
-      const int K = (G + 3) / 4;
-      const int g0 = wave * K;
-      float a2[RPTF][TS_W];
-      bool l2[RPTF];
-#pragma unroll
-      for (int q = 0; q < RPTF; ++q) {
-        const int ci = q * 64 + lane;
-        const int g = g0 + ci / TS_W, j = ci % TS_W;
-        l2[q] = false;
-#pragma unroll
-        for (int c = 0; c < TS_W; ++c) a2[q][c] = 0.0f;
-        if (ci < K * TS_W && g < G && j < w) {
-          uint32_t wd[TS_REC];
-          if (ts_fetch(p.rec + ((long)g * TS_W + j) * TS_REC, p.tag, wd)) {
-            l2[q] = (int)wd[0] >= 0;
-#pragma unroll
-            for (int c = 0; c < TS_W; ++c) a2[q][c] = __uint_as_float(wd[1 + c]);
-          } else {
-            timeout = true;
-          }
-        }
-      }
-      if (__any(timeout)) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         if (lane == 0) {
-          s_abort = 1;
-          __hip_atomic_store(p.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-      } else {
-        ts_play<RPTF, false>(a2, l2, w, p.eps, win[wave], rb[wave], mult);
-        // survivors' original values again from the published copies
-        // (already complete), into the second-round candidates
-        if (lane < TS_W) {
-          const int s = lane < w ? win[wave][lane] : -1;
-          const int ci = wave * TS_W + lane;
-          float *dst = cand + ci * TS_RS;
-          if (s >= 0) {
-            const int g = g0 + s / TS_W, j = s % TS_W;
-            const uint64_t *src = p.rec + ((long)g * TS_W + j) * TS_REC;
-            cid[ci] = (int)(uint32_t)__hip_atomic_load(src, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-#pragma unroll
-            for (int c = 0; c < TS_W; ++c)
-              dst[c] = __uint_as_float((uint32_t)__hip_atomic_load(src + 1 + c, __ATOMIC_RELAXED,
-                                                                   __HIP_MEMORY_SCOPE_AGENT));
-          } else {
-            cid[ci] = -1;
-#pragma unroll
-            for (int c = 0; c < TS_W; ++c) dst[c] = 0.0f;
-          }
+          const unsigned old = __builtin_amdgcn_atomic_inc32(
+              p.cnt + (long)level * TS_GMAX + parent, (unsigned)(members - 1), __ATOMIC_ACQ_REL,
+              "agent");
+          s_go = old == (unsigned)(members - 1);
         }
       }
       __syncthreads();
-      if (!s_abort && wave == 0) {
-        float a3[2][TS_W];
-        bool l3[2];
+      if (!s_go) break;
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      // this workgroup plays node (level + 1, parent): its members' winners,
+      // member k's j-th winner in slot 32k + j
+      const int m0 = parent * TS_FAN;
 #pragma unroll
-        for (int q = 0; q < 2; ++q) {
-          const int ci = q * 64 + lane;
-          l3[q] = cid[ci] >= 0;
-          ts_load_row(a3[q], cand + ci * TS_RS);
-        }
-        ts_play<2, true>(a3, l3, w, p.eps, win[0], rb[0], mult);
-        ts_factored_rows(win[0], rb[0], mult, w, Fn);
-        if (lane < TS_W) {
-          const int s = win[0][lane];
-          fin[lane] = s >= 0 ? cid[s] : -1;
-        }
+      for (int q = 0; q < TS_RPT; ++q) {
+        const int sl = q * TS_NT + t;
+        const int mem = sl / TS_W, j = sl % TS_W;
+        int id = -1;
+        if (mem < members && j < w) id = p.pub[((long)level * TS_GMAX + m0 + mem) * TS_W + j];
+        cid[sl] = id;
+        r.live[q] = id >= 0;
+        if (id >= 0) ts_load_row(r.a[q], p.W + (long)id * p.ld + p.c0, w, vec);
+        else
+#pragma unroll
+          for (int c = 0; c < TS_W; ++c) r.a[q][c] = 0.0f;
       }
+      ++level;
+      node = parent;
+      nodes = (nodes + TS_FAN - 1) / TS_FAN;
     }
-    __syncthreads();
-    if (s_abort) return;
   }
 
-  // 4. every workgroup knows the pivot rows fin[0 .. w) and their factored
-  //    values F (natural column order)
-  const float *F = Fn;
-  if (t < w) {
-    const int row = fin[t];
-    if (row >= base && row < base + ROWS) skip[row - base] = 1;  // moves into the window
-  }
-  if (b == 0 && wave == 0) {
-    // libmat's sequential interchanges: step kk swaps row c0+kk with the
-    // current position of pivot row kk
-    int posw = lane < w ? fin[lane] : -1;
-    int mypiv = 0;
-    for (int kk = 0; kk < w; ++kk) {
-      const int k = p.c0 + kk;
-      const int pr = __builtin_amdgcn_readlane(posw, kk);
-      if (lane == kk) mypiv = pr;
-      if (posw == k) posw = pr;
-      if (lane == kk) posw = k;
-    }
-    if (lane < w) p.ipiv[p.c0 + lane] = mypiv;
-    swap_pairs(mypiv, p.c0, w, s_pairs);
-  }
-  // the pivot rows prepared for the substitution (into cand, free now)
-  float *uc = cand;
-  for (int x = t; x < TS_W * TS_W; x += TS_NT) {
-    const int j = x / TS_W, sl = x % TS_W;
-    if (j < w) {
-      const float pv = F[j * TS_RS + j];
-      const bool skipj = fabsf(pv) < p.eps;
-      float v;
-      if (sl == 0) v = skipj ? 1.0f : ts_recip(pv);
-      else v = (!skipj && sl < w - j) ? F[j * TS_RS + j + sl] : 0.0f;
-      uc[j * TS_RS + sl] = v;
+  // 3. wait for the final round
+  if (t == 0) {
+    long spins = 0;
+    s_go = 1;
+    while (__hip_atomic_load(p.done, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) !=
+           tagged(p.tag, 1u)) {
+      if (++spins > SPIN_LIMIT) {
+        s_go = 0;
+        __hip_atomic_store(p.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
     }
   }
   __syncthreads();
-  // own rows: L21 = A21 U11^-1 from the original values
-  {
-    float x[RPT0][TS_W];
-    bool mine[RPT0], any = false;
-#pragma unroll
-    for (int q = 0; q < RPT0; ++q) {
-      const int li = (wave * RPT0 + q) * 64 + lane;
-      ts_load_row(x[q], orig + li * TS_RS);
-      mine[q] = base + li < rend && !skip[li];
-      any = any || mine[q];
-    }
-    if (__any(any)) {
-      ts_subst<RPT0>(x, uc, w);
-#pragma unroll
-      for (int q = 0; q < RPT0; ++q) {
-        if (mine[q]) {
-          float *dst = p.W + (long)(base + (wave * RPT0 + q) * 64 + lane) * p.ld + p.c0;
-#pragma unroll
-          for (int sl = 0; sl < TS_W; ++sl) {
-            const int c = (sl + w) & (TS_W - 1);
-            if (c < w) dst[c] = x[q][sl];
-          }
-        }
-      }
+  if (!s_go) return;
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  // the final pivot rows (factored) and their indices, once from memory
+  float *Fs = orig + TS_NT * TS_RS;  // [TS_W][TS_RS]
+  for (int e = t; e < TS_W * TS_W; e += TS_NT) Fs[(e / TS_W) * TS_RS + e % TS_W] = p.F[e];
+  if (t < TS_W) {
+    fin[t] = t < w ? p.fin[t] : -1;
+    windst[t] = -1;
+  }
+  __syncthreads();
+  // prepared for the substitution (see ts_sub; entries past a row's end are
+  // read but never used)
+  for (int e = t; e < TS_W * TS_W; e += TS_NT) {
+    const int J = e / TS_W, c = e % TS_W;
+    if (c >= J) {
+      const float pjj = Fs[J * TS_RS + J];
+      const bool skipj = !(J < w) || fabsf(pjj) < p.eps;
+      float v = 0.0f;
+      if (c == J) v = skipj ? 1.0f : ts_recip(pjj);
+      else if (c < w && !skipj) v = Fs[J * TS_RS + c];
+      un[J * TS_RS + (c - J)] = v;
     }
   }
+  if (t < w) {
+    const int row = fin[t];
+    if (row >= base && row < base + TS_NT) skip[row - base] = 1;  // moves into the window
+  }
+  __syncthreads();
+
+  // 4. L21 = A21 U11^-1 from the original values, for the own rows below
+  //    the window that stay, and in workgroup 0 for the window rows, which
+  //    either move into the window as pivot rows or to a pivot row's old
+  //    place
+  const bool winrow = b == 0 && t < w;
+  const bool stays = mine && !skip[t] && myrow >= p.r0 + w;
+  float x[TS_W];
+  if (stays || winrow) {
+#pragma unroll
+    for (int c4 = 0; c4 < TS_W / 4; ++c4) {
+      const float4 q = *reinterpret_cast<const float4 *>(orig + t * TS_RS + 4 * c4);
+      x[4 * c4] = q.x;
+      x[4 * c4 + 1] = q.y;
+      x[4 * c4 + 2] = q.z;
+      x[4 * c4 + 3] = q.w;
+    }
+    ts_subs(x, un, std::make_integer_sequence<int, TS_W>{});
+  }
+  if (b == 0 && wave == 0) {
+    // libmat's sequential interchanges: step kk swaps row c0+kk with the
+    // current position of pivot row kk.  Lane i < w follows window position
+    // c0+i, lane 32+j the original place of pivot row j when that lies below
+    // the window (the only places the swaps touch): occ = the original row
+    // there now.
+    const int fj = lane >= TS_W && lane - TS_W < w ? fin[lane - TS_W] : -1;
+    int pos = -1;
+    if (lane < w) pos = p.c0 + lane;
+    else if (fj >= p.c0 + w) pos = fj;
+    int occ = pos;
+    const int finl = lane < w ? fin[lane] : -1;
+    int mypiv = 0;
+    for (int kk = 0; kk < w; ++kk) {
+      const int f = __builtin_amdgcn_readlane(finl, kk);
+      const unsigned long long m = __ballot(pos >= 0 && occ == f);
+      const int sl = (int)__builtin_ctzll(m);
+      const int okk = __builtin_amdgcn_readlane(occ, kk);
+      const int osl = __builtin_amdgcn_readlane(occ, sl);
+      const int psl = __builtin_amdgcn_readlane(pos, sl);
+      mypiv = lane == kk ? psl : mypiv;
+      occ = lane == kk ? osl : (lane == sl ? okk : occ);
+    }
+    if (lane < w) p.ipiv[p.c0 + lane] = mypiv;
+    // the row pairs (dst receives original row src) and where each window
+    // row goes: a moved window row below the window is displaced, one inside
+    // it is a pivot row (no L21 row)
+    const bool moved = pos >= 0 && occ != pos;
+    const unsigned long long mm = __ballot(moved);
+    if (moved) {
+      const int at = __popcll(mm & ((1ull << lane) - 1ull));
+      s_pairs[1 + at] = pos;
+      s_pairs[1 + PAIR_MAX + at] = occ;
+      if (pos >= p.c0 + w && occ < p.c0 + w) windst[occ - p.c0] = pos;
+    }
+    if (lane == 0) s_pairs[0] = __popcll(mm);
+  }
+  __syncthreads();
+  if (stays) ts_store_row(p.W + (long)myrow * p.ld + p.c0, x, w, vec);
+  if (winrow && windst[t] >= 0) ts_store_row(p.W + (long)windst[t] * p.ld + p.c0, x, w, vec);
   if (b == 0) {
     // the window gets the factored pivot rows
-    for (int x = t; x < w * TS_W; x += TS_NT) {
-      const int j = x / TS_W, c = x % TS_W;
-      if (c < w) p.W[(long)(p.r0 + j) * p.ld + p.c0 + c] = F[j * TS_RS + c];
-    }
-    // window rows displaced into a pivot row's old place (outside the
-    // window): their L21 rows from their original values
-    const int np = s_pairs[0];
-    if (wave == 0) {
-      const int dst = lane < np ? s_pairs[1 + lane] : -1;
-      const int src = lane < np ? s_pairs[1 + PAIR_MAX + lane] : -1;
-      // (the source of such a move is always a window row; the bounds keep
-      // the LDS read inside the workgroup's rows regardless)
-      const bool out = dst >= p.r0 + w && src >= p.r0 && src < p.r0 + w;
-      if (__any(out)) {
-        float x[1][TS_W];
-        ts_load_row(x[0], orig + (out ? src - p.r0 : 0) * TS_RS);
-        ts_subst<1>(x, uc, w);
-        if (out) {
-          float *d = p.W + (long)dst * p.ld + p.c0;
-#pragma unroll
-          for (int sl = 0; sl < TS_W; ++sl) {
-            const int c = (sl + w) & (TS_W - 1);
-            if (c < w) d[c] = x[0][sl];
-          }
-        }
-      }
+    for (int e = t; e < w * TS_W; e += TS_NT) {
+      const int J = e / TS_W, c = e % TS_W;
+      if (c < w) p.W[(long)(p.r0 + J) * p.ld + p.c0 + c] = Fs[J * TS_RS + c];
     }
     // the leaf's row pairs for the later row moves
-    for (int x = t; x < PAIR_STRIDE; x += TS_NT) p.pairs[x] = s_pairs[x];
+    for (int e = t; e < PAIR_STRIDE; e += TS_NT) p.pairs[e] = s_pairs[e];
   }
-}
-
-template <int RPT0, int RPTF>
-void launch_tslu(hipStream_t s, const TsluP &p) {
-  hipLaunchKernelGGL((tslu_kernel<RPT0, RPTF>), dim3(p.G), dim3(TS_NT), 0, s, p);
 }
 
 }  // namespace
 
-// Largest panel height the tournament kernel takes (32 workgroups of 512 rows).
-int tslu_max_rows() { return 32 * TS_NT * 2; }
+// Largest panel height the tournament kernel takes (one workgroup per 256
+// rows, all of them resident at once).
+int tslu_max_rows() { return TS_GMAX * TS_NT; }
 
 // Tournament-pivoted factorization of the leaf W[c0:n, c0:c0+w] (w <= 32,
 // n - c0 <= tslu_max_rows()): writes ipiv[c0 .. c0+w) (sequential
@@ -642,24 +574,32 @@ void tslu_leaf(hipStream_t s, int n, float *W, long ld, int c0, int w, int *ipiv
   p.ipiv = ipiv;
   p.pairs = pairs + (long)c0 * PAIR_STRIDE;
   p.err = sync_error_word(s);
-  // a fresh allocation holds stale bytes, and a stale word whose high half
-  // equals this launch's tag (small integers here) would pass as published:
-  // clear it (tag 0 is never used) before the first kernel polls it
-  const size_t rec_bytes = (size_t)32 * TS_W * TS_REC * 8;
+  // one slot: tickets (must start at zero; every complete launch leaves them
+  // at zero again), the done word (a stale word whose high half equals this
+  // launch's tag would pass as raised: cleared when fresh, tag 0 is never
+  // used), published winners and the final round
+  const size_t cnt_bytes = (size_t)TS_LMAX * TS_GMAX * sizeof(unsigned);
+  const size_t done_off = cnt_bytes;
+  const size_t pub_off = done_off + 256;
+  const size_t pub_bytes = (size_t)TS_LMAX * TS_GMAX * TS_W * sizeof(int);
+  const size_t f_off = pub_off + pub_bytes;
+  const size_t fin_off = f_off + (size_t)TS_W * TS_W * sizeof(float);
+  const size_t bytes = fin_off + TS_W * sizeof(int);
   bool fresh = false;
-  p.rec = (uint64_t *)ws.get("tslu.rec", rec_bytes, &fresh);
-  if (fresh) fill_u32(s, (unsigned *)p.rec, (long)(rec_bytes / 4), 0u);
+  char *buf = (char *)ws.get("tslu.ws", bytes, &fresh);
+  if (fresh) fill_u32(s, (unsigned *)buf, (long)(pub_off / 4), 0u);
+  p.cnt = (unsigned *)buf;
+  p.done = (uint64_t *)(buf + done_off);
+  p.pub = (int *)(buf + pub_off);
+  p.F = (float *)(buf + f_off);
+  p.fin = (int *)(buf + fin_off);
   unsigned tag = epoch.fetch_add(1, std::memory_order_relaxed) + 1;
   if (tag == 0) tag = epoch.fetch_add(1, std::memory_order_relaxed) + 1;
   p.tag = tag;
-  const int rpt0 = m > 32 * TS_NT ? 2 : 1;
-  p.G = ceil_div(m, TS_NT * rpt0);
-  const int K = (p.G + 3) / 4;
+  p.G = ceil_div(m, TS_NT);
+  p.P = ceil_div(m, TS_PR);
   KScope ks(s, "panel", 2.0 * m * w * sizeof(float));
-  if (rpt0 == 2) launch_tslu<2, 4>(s, p);
-  else if (K <= 2) launch_tslu<1, 1>(s, p);
-  else if (K <= 4) launch_tslu<1, 2>(s, p);
-  else launch_tslu<1, 4>(s, p);
+  hipLaunchKernelGGL(tslu_kernel, dim3(p.G), dim3(TS_NT), 0, s, p);
   MG_HIP(hipGetLastError());
 }
 
