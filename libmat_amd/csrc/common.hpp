@@ -145,9 +145,12 @@ constexpr double kLowpMinWork = 2.0e9;
 // A, B given in fp32 and multiplied on BF16 MFMA after an exact split into
 // `passes` (1, 3 or 6) bf16 terms; accumulation in fp32.  lower_only: only
 // row >= col of C is written (tiles above the diagonal are not launched).
+// grid_cap > 0 limits the launch to that many workgroups (rounded down to a
+// multiple of 8), each walking several tiles: work beside a critical path on
+// another stream leaves compute-unit slots for it.
 void gemm_bf16x(hipStream_t s, long M, long N, long K, float alpha,
                 View<const float> A, View<const float> B, View<float> C,
-                int passes, bool lower_only = false);
+                int passes, bool lower_only = false, int grid_cap = 0);
 
 // FP8 e4m3 GEMM with one scale per 128x128 tile of each operand (computed
 // on the fly), fp32 accumulation: C += alpha * A * B.

@@ -178,6 +178,7 @@ struct LpP {
   float alpha;
   int tiles_m, tiles_n, tiles;
   int lower;
+  int vgrid;  // workgroup ids of the tile map (tiles rounded up to 8)
 };
 
 // Workgroup id -> (tm, tn).  The ids are regrouped by XCD; lower products
@@ -249,14 +250,18 @@ void gemm_bf16x_kernel(LpP p) {
   constexpr int BM = LP_BM, BN = LP_BN, BK = 32;
   constexpr int TILE = BM * BK;  // bf16 per plane and operand (8 KB)
   __shared__ __align__(16) unsigned short sm[2][NP][TILE];  // [A, B][plane]
-  int tm, tn;
-  if (!lp_map_tile(blockIdx.x, gridDim.x, p.tiles, p.tiles_m, p.tiles_n, tm, tn, p.lower)) return;
-  const int bm0 = tm * BM, bn0 = tn * BN;
-  if (p.lower && bm0 + BM - 1 < bn0) return;
   const int t = threadIdx.x, lane = t & 63;
   const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
   const int wm = wave & 1, wn = wave >> 1;
   const int g = lane >> 5, li = lane & 31;
+  // one tile per workgroup, or (a grid capped below the tile count, with a
+  // multiple of 8 workgroups) tiles id, id + grid, ...: every id keeps its
+  // XCD, so the tile map's XCD grouping holds
+  for (int vid = blockIdx.x; vid < p.vgrid; vid += gridDim.x) {
+  int tm, tn;
+  if (!lp_map_tile(vid, p.vgrid, p.tiles, p.tiles_m, p.tiles_n, tm, tn, p.lower)) continue;
+  const int bm0 = tm * BM, bn0 = tn * BN;
+  if (p.lower && bm0 + BM - 1 < bn0) continue;
 
   f32x16 acc[2][2];
 #pragma unroll
@@ -350,6 +355,7 @@ void gemm_bf16x_kernel(LpP p) {
         if (in) cb[(long)jo * p.ldc] = cv[r] + p.alpha * acc[a][b][r];
       }
     }
+  }
   }
 }
 
@@ -453,11 +459,12 @@ static int lp_tile_count(int tiles_m, int tiles_n, int lower) {
 }
 
 void gemm_bf16x(hipStream_t s, long M, long N, long K, float alpha, View<const float> A,
-                View<const float> B, View<float> C, int passes, bool lower_only) {
+                View<const float> B, View<float> C, int passes, bool lower_only, int grid_cap) {
   if (M <= 0 || N <= 0 || K <= 0) return;
   if (C.rs != 1) {
     MG_CHECK(C.cs == 1 && !lower_only, "gemm_bf16x: unsupported C layout");
-    gemm_bf16x(s, N, M, K, alpha, B.t(), A.t(), View<float>{C.p, 1, C.rs}, passes, false);
+    gemm_bf16x(s, N, M, K, alpha, B.t(), A.t(), View<float>{C.p, 1, C.rs}, passes, false,
+               grid_cap);
     return;
   }
   MG_CHECK(passes == 1 || passes == 3 || passes == 6, "gemm_bf16x: passes must be 1, 3 or 6");
@@ -492,6 +499,8 @@ void gemm_bf16x(hipStream_t s, long M, long N, long K, float alpha, View<const f
   p.tiles = lp_tile_count(p.tiles_m, p.tiles_n, p.lower);
   int grid = p.tiles;
   if (grid >= 8) grid = (grid + 7) / 8 * 8;
+  p.vgrid = grid;
+  if (grid_cap >= 8 && grid > grid_cap) grid = grid_cap / 8 * 8;
   KScope ks(s, "gemm_bf16x", 2.0 * passes * M * N * K);
   if (passes == 1) hipLaunchKernelGGL((gemm_bf16x_kernel<1, 1>), dim3(grid), dim3(256), 0, s, p);
   else if (passes == 3) hipLaunchKernelGGL((gemm_bf16x_kernel<2, 3>), dim3(grid), dim3(256), 0, s, p);

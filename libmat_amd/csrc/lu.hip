@@ -13,6 +13,7 @@
 // mat.h:4502-4557), one pivot search per step over the trailing
 // submatrix, reproducing the reference's row-major scan order.
 #include <algorithm>
+#include <cstdlib>
 #include <mutex>
 #include <vector>
 
@@ -34,17 +35,30 @@ int lu_leaf_max(int m, int elem_size);  // widest leaf the panel kernel takes fo
 
 template <typename T>
 static void lu_gemm_sub(hipStream_t s, long M, long N, long K, View<const T> A, View<const T> B,
-                        View<T> C, Mul mul) {
+                        View<T> C, Mul mul, int grid_cap = 0) {
   if constexpr (std::is_same<T, float>::value) {
     // small products: native fp32 MFMA is as fast as packing for the split
     // passes (and exact to fp32); the split pays off on large updates only
     if ((mul == Mul::Bf16x1 || mul == Mul::Bf16x3 || mul == Mul::Bf16x6) &&
         (double)M * N * K >= kLowpMinWork) {
-      gemm_bf16x(s, M, N, K, -1.0f, A, B, C, (int)mul);
+      gemm_bf16x(s, M, N, K, -1.0f, A, B, C, (int)mul, false, grid_cap);
       return;
     }
   }
   gemm<T>(s, M, N, K, T(-1), A, B, T(1), C, false);
+}
+
+// Workgroups of the side stream's split-BF16 trailing updates per compute
+// unit (MATGPU_SIDE_WG_PER_CU, a fraction allowed, 0 = uncapped): capped, so
+// that the kernels of the critical path (panels, small solves and products)
+// find free slots instead of waiting for GEMM tiles to finish; a workgroup
+// of the split GEMM takes 48 KB of LDS, three fit on a unit.
+static int side_grid_cap() {
+  static const double per_cu = [] {
+    const char *e = std::getenv("MATGPU_SIDE_WG_PER_CU");
+    return e ? std::max(0.0, std::atof(e)) : 1.5;
+  }();
+  return (int)(per_cu * num_cus());
 }
 
 // Factor columns [c0, c0+w) (rows c0..n-1, all earlier updates applied)
@@ -86,7 +100,7 @@ static void block_swaps(hipStream_t s, T *W, long ld, const int *pairs, int b0, 
 // the columns [c0, c1).
 template <typename T>
 static void block_update(hipStream_t s, int n, T *W, long ld, int b0, int b1, int c0, int c1,
-                         const LuOpts &o) {
+                         const LuOpts &o, int grid_cap = 0) {
   if (c1 <= c0) return;
   const int nb = b1 - b0;
   View<const T> L11{W + (long)b0 * ld + b0, ld, 1};
@@ -94,7 +108,7 @@ static void block_update(hipStream_t s, int n, T *W, long ld, int b0, int b1, in
   trsm_left<T>(s, true, true, nb, c1 - c0, L11, A12, o.mul);
   View<const T> L21{W + (long)b1 * ld + b0, ld, 1};
   View<T> A22{W + (long)b1 * ld + c0, ld, 1};
-  lu_gemm_sub<T>(s, n - b1, c1 - c0, nb, L21, cview(A12), A22, o.mul);
+  lu_gemm_sub<T>(s, n - b1, c1 - c0, nb, L21, cview(A12), A22, o.mul, grid_cap);
 }
 
 // Right-looking blocked LU with look-ahead over NB-wide column blocks.
@@ -146,9 +160,9 @@ static void getrf_lookahead(hipStream_t s, int n, T *W, long ld, int *ipiv, int 
     MG_HIP(hipStreamWaitEvent(u, evP[k], 0));
     const int c3 = std::min(n, c2 + NB);
     block_swaps<T>(u, W, ld, pairs, b0, b1, leaf_of(b0), c2, n);
-    block_update<T>(u, n, W, ld, b0, b1, c2, c3, o);
+    block_update<T>(u, n, W, ld, b0, b1, c2, c3, o, side_grid_cap());
     MG_HIP(hipEventRecord(evU[k], u));
-    block_update<T>(u, n, W, ld, b0, b1, c3, n, o);
+    block_update<T>(u, n, W, ld, b0, b1, c3, n, o, side_grid_cap());
     block_swaps<T>(u, W, ld, pairs, b0, b1, leaf_of(b0), 0, b0);
   }
   hipEvent_t done = pooled_event(2 * nblk + 1);

@@ -104,6 +104,24 @@ __device__ __forceinline__ float ts_recip(float x) { return __builtin_amdgcn_rcp
 
 typedef float ts_f4 __attribute__((ext_vector_type(4)));
 
+// Words one workgroup hands to another within a launch (winners, the final
+// round) go through agent-scope atomic loads and stores, as every exchange
+// between workgroups in this library does: the compute units of different
+// XCDs cache device memory in different L2s, and these accesses are the
+// ones coherent across them without relying on cache maintenance.
+__device__ __forceinline__ void ts_put(int *p, int v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int ts_get(const int *p) {
+  return __hip_atomic_load(const_cast<int *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void ts_putf(float *p, float v) {
+  ts_put(reinterpret_cast<int *>(p), __float_as_int(v));
+}
+__device__ __forceinline__ float ts_getf(const float *p) {
+  return __int_as_float(ts_get(reinterpret_cast<const int *>(p)));
+}
+
 // Per-step exchange of a workgroup playoff, double-buffered by step parity
 // (one barrier per step: a buffer is rewritten two steps later, after every
 // wavefront has passed the barrier that follows its reads).
@@ -334,7 +352,8 @@ __global__ __launch_bounds__(TS_NT) void tslu_kernel(TsluP p) {
   __shared__ int windst[TS_W];   // where each window row goes (-1: into the window)
   __shared__ int s_pairs[PAIR_STRIDE];
   __shared__ unsigned char skip[TS_NT];
-  __shared__ int s_go;
+  __shared__ int s_go;  // this workgroup plays the next tree level
+  __shared__ int s_ok;  // the final round arrived (no time-out)
 
   // 1. own rows, original values, for the substitution
   {
@@ -376,13 +395,11 @@ __global__ __launch_bounds__(TS_NT) void tslu_kernel(TsluP p) {
           if (r.pick[q] >= 0) {
             float *d = p.F + r.pick[q] * TS_W;
 #pragma unroll
-            for (int c4 = 0; c4 < TS_W / 4; ++c4)
-              *reinterpret_cast<float4 *>(d + 4 * c4) = make_float4(
-                  r.a[q][4 * c4], r.a[q][4 * c4 + 1], r.a[q][4 * c4 + 2], r.a[q][4 * c4 + 3]);
-            p.fin[r.pick[q]] = cid[q * TS_NT + t];
+            for (int c = 0; c < TS_W; ++c) ts_putf(d + c, r.a[q][c]);
+            ts_put(p.fin + r.pick[q], cid[q * TS_NT + t]);
           }
         }
-        if (t >= w && t < TS_W) p.fin[t] = -1;
+        if (t >= w && t < TS_W) ts_put(p.fin + t, -1);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         __syncthreads();
         if (t == 0)
@@ -392,7 +409,7 @@ __global__ __launch_bounds__(TS_NT) void tslu_kernel(TsluP p) {
       }
       if (t < TS_W) {
         const int s = t < w ? win[t] : -1;
-        p.pub[((long)level * TS_GMAX + node) * TS_W + t] = s >= 0 ? cid[s] : -1;
+        ts_put(p.pub + ((long)level * TS_GMAX + node) * TS_W + t, s >= 0 ? cid[s] : -1);
       }
       const int parent = node / TS_FAN;
       const int members = min(TS_FAN, nodes - parent * TS_FAN);
@@ -416,7 +433,7 @@ __global__ __launch_bounds__(TS_NT) void tslu_kernel(TsluP p) {
         const int sl = q * TS_NT + t;
         const int mem = sl / TS_W, j = sl % TS_W;
         int id = -1;
-        if (mem < members && j < w) id = p.pub[((long)level * TS_GMAX + m0 + mem) * TS_W + j];
+        if (mem < members && j < w) id = ts_get(p.pub + ((long)level * TS_GMAX + m0 + mem) * TS_W + j);
         cid[sl] = id;
         r.live[q] = id >= 0;
         if (id >= 0) ts_load_row(r.a[q], p.W + (long)id * p.ld + p.c0, w, vec);
@@ -430,14 +447,15 @@ __global__ __launch_bounds__(TS_NT) void tslu_kernel(TsluP p) {
     }
   }
 
-  // 3. wait for the final round
+  // 3. wait for the final round (a flag of its own: a wavefront that left
+  //    the tree may not have read s_go yet when thread 0 gets here)
   if (t == 0) {
     long spins = 0;
-    s_go = 1;
+    s_ok = 1;
     while (__hip_atomic_load(p.done, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) !=
            tagged(p.tag, 1u)) {
       if (++spins > SPIN_LIMIT) {
-        s_go = 0;
+        s_ok = 0;
         __hip_atomic_store(p.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         break;
       }
@@ -445,13 +463,13 @@ __global__ __launch_bounds__(TS_NT) void tslu_kernel(TsluP p) {
     }
   }
   __syncthreads();
-  if (!s_go) return;
+  if (!s_ok) return;
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   // the final pivot rows (factored) and their indices, once from memory
   float *Fs = orig + TS_NT * TS_RS;  // [TS_W][TS_RS]
-  for (int e = t; e < TS_W * TS_W; e += TS_NT) Fs[(e / TS_W) * TS_RS + e % TS_W] = p.F[e];
+  for (int e = t; e < TS_W * TS_W; e += TS_NT) Fs[(e / TS_W) * TS_RS + e % TS_W] = ts_getf(p.F + e);
   if (t < TS_W) {
-    fin[t] = t < w ? p.fin[t] : -1;
+    fin[t] = t < w ? ts_get(p.fin + t) : -1;
     windst[t] = -1;
   }
   __syncthreads();
