@@ -95,6 +95,12 @@ hipStream_t default_stream();
 // MATGPU_PANEL_CUS=k (default 0: no mask) and a runtime that supports CU
 // masks, it leaves k compute units to the caller's stream.
 hipStream_t side_stream();
+// Wave priority (s_setprio) for kernels launched on stream s: 0 on a side
+// stream, 3 elsewhere.  Kernels of the caller's stream carry a
+// factorization's critical path (panels, small solves and products); when
+// they share SIMDs with the side stream's trailing-update GEMMs, the
+// higher priority lets their instructions issue first.
+int wave_prio(hipStream_t s);
 // Reusable events (timing disabled) of the current device, by index.
 hipEvent_t pooled_event(int i);
 // Held by a look-ahead factorization (LU, Cholesky, QR) while it enqueues:

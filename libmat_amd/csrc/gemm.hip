@@ -62,6 +62,7 @@ struct GemmP {
   T alpha, beta;
   int tiles_m, tiles_n, tiles;
   int lower;
+  int prio;  // wave priority (wave_prio of the launch stream)
   // tile count: all tiles, or for lower products with square tiles only the
   // tiles on or below the diagonal (the ids dealt to the XCDs are all work)
 };
@@ -181,6 +182,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmP<T> p) {
   constexpr int XCD = 8;
   __shared__ T As[2][BK * (BM + PAD)];
   __shared__ T Bs[2][BK * (BN + PAD)];
+  if (p.prio) __builtin_amdgcn_s_setprio(3);
   if constexpr (MAP == 0) {
     int tm, tn;
     if (!map_tile(blockIdx.x, gridDim.x, p.tiles, p.tiles_m, p.tiles_n, tm, tn)) return;
@@ -312,6 +314,7 @@ static void launch_tiles(hipStream_t s, GemmP<T> &p, bool vec, int grid_cap) {
   constexpr bool can_persist = BM == 128 && BN == 128;
   const bool persist = can_persist && compact && grid_cap >= 8 && grid > grid_cap;
   if (persist) grid = grid_cap / 8 * 8;
+  p.prio = wave_prio(s);
   KScope ks(s, "gemm", (p.lower ? 1.0 : 2.0) * p.M * p.N * p.K);
 #define MG_GEMM_LAUNCH(MAP_)                                                                        \
   {                                                                                                 \

@@ -71,6 +71,7 @@ struct PanelP {
   uint64_t *krec;        // [2][KREC] tagged words: row k
   int *err;              // set to 1 on a poll timeout
   int *pairs;            // this leaf's interchanges as row pairs (see swap_pairs)
+  int prio;              // wave priority (wave_prio of the launch stream)
 };
 template <typename T>
 __device__ inline void better(T &bv, int &bi, T v, int i) {
@@ -265,6 +266,7 @@ __device__ __forceinline__ bool panel_idle() {
 
 template <typename T, int WMAX, int RPT, bool LOCAL>
 __global__ __launch_bounds__(PanelNT<T>::value) void panel_gepp_kernel(PanelP<T> p) {
+  if (p.prio) __builtin_amdgcn_s_setprio(3);
   constexpr int PANEL_NT = PanelNT<T>::value;
   constexpr int PANEL_NW = PANEL_NT / 64;
   constexpr int RPW = PANEL_NT * RPT;
@@ -607,6 +609,7 @@ constexpr int WREC_MAX = 256;  // sizing of the exchange record buffer (64 workg
 // The results are the same bits as panel_gepp_kernel's.
 template <typename T, int WMAX, int RPT, bool LOCAL>
 __global__ __launch_bounds__(PanelNT<T>::value) void panel_gepp_la_kernel(PanelP<T> p) {
+  if (p.prio) __builtin_amdgcn_s_setprio(3);
   constexpr int PANEL_NT = PanelNT<T>::value;
   constexpr int PANEL_NW = PANEL_NT / 64;
   constexpr int RPW = PANEL_NT * RPT;
@@ -933,7 +936,8 @@ __global__ __launch_bounds__(PanelNT<T>::value) void panel_gepp_la_kernel(PanelP
 // permutation is applied in one pass.
 template <typename T, int NP>
 __global__ __launch_bounds__(256) void rowswap_kernel(T *W, long ld, int cl, int ch, const int *pairs,
-                                                      int *perm, int w, int skip0) {
+                                                      int *perm, int w, int skip0, int prio) {
+  if (prio) __builtin_amdgcn_s_setprio(3);
   static_assert(NP <= PAIR_MAX, "pairs hold at most PAIR_MAX rows");
   __shared__ int dst[PAIR_MAX], src[PAIR_MAX];
   __shared__ int npair;
@@ -1049,9 +1053,11 @@ void lu_rowswap(hipStream_t s, T *W, long ld, const int *pairs, int *perm, int k
   const dim3 grid(count > 0 ? ceil_div(count, 256) : 1);
   const int *pr = pairs + (long)k0 * PAIR_STRIDE;
   if (w <= 32)
-    hipLaunchKernelGGL((rowswap_kernel<T, 64>), grid, dim3(256), 0, s, W, ld, cl, ch, pr, perm, w, skip0);
+    hipLaunchKernelGGL((rowswap_kernel<T, 64>), grid, dim3(256), 0, s, W, ld, cl, ch, pr, perm, w, skip0,
+                       wave_prio(s));
   else
-    hipLaunchKernelGGL((rowswap_kernel<T, 128>), grid, dim3(256), 0, s, W, ld, cl, ch, pr, perm, w, skip0);
+    hipLaunchKernelGGL((rowswap_kernel<T, 128>), grid, dim3(256), 0, s, W, ld, cl, ch, pr, perm, w, skip0,
+                       wave_prio(s));
   MG_HIP(hipGetLastError());
 }
 
@@ -1064,7 +1070,8 @@ constexpr int BLK_LEAVES = 16;
 template <typename T>
 __global__ __launch_bounds__(256) void rowswap_block_kernel(T *W, long ld, int cl, int ch,
                                                             const int *pairs, int b0, int leaf,
-                                                            int nleaf) {
+                                                            int nleaf, int prio) {
+  if (prio) __builtin_amdgcn_s_setprio(3);
   constexpr int NP = 64;  // leaves of at most 32 columns
   __shared__ int dst[BLK_LEAVES][NP], src[BLK_LEAVES][NP];
   __shared__ int npair[BLK_LEAVES];
@@ -1110,7 +1117,7 @@ void lu_rowswap_block(hipStream_t s, T *W, long ld, const int *pairs, int b0, in
   MG_CHECK(ch <= b0 || cl >= b1, "lu_rowswap_block: the columns must lie outside the block");
   KScope ks(s, "rowswap");
   hipLaunchKernelGGL((rowswap_block_kernel<T>), dim3(ceil_div(ch - cl, 256)), dim3(256), 0, s, W,
-                     ld, cl, ch, pairs, b0, leaf, nleaf);
+                     ld, cl, ch, pairs, b0, leaf, nleaf, wave_prio(s));
   MG_HIP(hipGetLastError());
 }
 
@@ -1138,6 +1145,7 @@ static void lu_leaf_w(hipStream_t s, int n, T *W, long ld, int c0, int w, int *i
   p.eps = eps;
   p.ipiv = ipiv;
   p.pairs = pairs + (long)c0 * PAIR_STRIDE;
+  p.prio = wave_prio(s);
   // the exchange polls one workgroup per lane of a wavefront
   const int maxwg = 64;
   // a fresh allocation holds stale bytes, and a stale word whose high half

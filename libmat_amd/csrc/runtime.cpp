@@ -1,5 +1,6 @@
 // Host runtime: error strings, per-stream device workspaces, the library's
 // own stream for the synchronous drop-in entry points.
+#include <array>
 #include <atomic>
 #include <cstdlib>
 #include <map>
@@ -159,6 +160,12 @@ void timing_query(const char *family, long *launches, double *ms, double *work) 
   if (work) *work = w;
 }
 
+// every device's side stream, for wave_prio (no device query per launch)
+static std::array<std::atomic<hipStream_t>, 64> &side_streams() {
+  static std::array<std::atomic<hipStream_t>, 64> a{};
+  return a;
+}
+
 hipStream_t side_stream() {
   static std::mutex m;
   static std::map<int, hipStream_t> streams;
@@ -185,7 +192,18 @@ hipStream_t side_stream() {
   }
   if (!st) MG_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
   streams[dev] = st;
+  for (auto &slot : side_streams())
+    if (slot.load() == nullptr) {
+      slot.store(st);
+      break;
+    }
   return st;
+}
+
+int wave_prio(hipStream_t s) {
+  for (auto &slot : side_streams())
+    if (slot.load() == s && s != nullptr) return 0;
+  return 3;
 }
 
 hipEvent_t pooled_event(int i) {

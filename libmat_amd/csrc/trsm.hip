@@ -56,8 +56,9 @@ __device__ __forceinline__ void trsm_group(T (&x)[TRSM_TB], const T *col, T xc) 
 // Non-unit leaves divide by the diagonal, as the reference's TRSVs do.
 template <typename T, bool LOWER, bool UNIT>
 __global__ __launch_bounds__(TRSM_NT) void trsm_leaf_kernel(int n, int nrhs, View<const T> Tm,
-                                                            View<T> B) {
+                                                            View<T> B, int prio) {
   constexpr int TB = TRSM_TB;
+  if (prio) __builtin_amdgcn_s_setprio(3);
   __shared__ __align__(16) T Sk[TB * TB];
   {
     // every load issued before the LDS stores (one memory latency, not TB)
@@ -109,7 +110,8 @@ __global__ __launch_bounds__(TRSM_NT) void trsm_leaf_kernel(int n, int nrhs, Vie
 // lane does R FMAs where the one-lane-per-column leaf does up to 64.
 template <typename T, bool LOWER, bool UNIT, int G, int R>
 __global__ __launch_bounds__(256) void trsm_split_kernel(int n, int nrhs, View<const T> Tm,
-                                                         View<T> B) {
+                                                         View<T> B, int prio) {
+  if (prio) __builtin_amdgcn_s_setprio(3);
   constexpr int CW = 64 / G;                   // columns per wavefront
   constexpr int NP = R * G;                    // rows held (solve order)
   constexpr int LS = NP + 16 / (int)sizeof(T);  // padded column stride, 16-byte aligned
@@ -283,16 +285,17 @@ void trsm_left(hipStream_t s, bool lower, bool unit, int n, int nrhs, View<const
     }
     return;
   }
+  const int prio = wave_prio(s);
   if (n > 16 && (n <= 64 ? nrhs <= 4096 : n <= 128 && nrhs <= 256)) {
     KScope ks(s, "trsm");
     const dim3 blk(256);
 #define MG_SPLIT(G_, R_)                                                                            \
   {                                                                                                 \
     const dim3 grid(ceil_div(nrhs, 4 * (64 / G_)));                                               \
-    if (lower && unit) hipLaunchKernelGGL((trsm_split_kernel<T, true, true, G_, R_>), grid, blk, 0, s, n, nrhs, Tm, B);   \
-    else if (lower) hipLaunchKernelGGL((trsm_split_kernel<T, true, false, G_, R_>), grid, blk, 0, s, n, nrhs, Tm, B);     \
-    else if (unit) hipLaunchKernelGGL((trsm_split_kernel<T, false, true, G_, R_>), grid, blk, 0, s, n, nrhs, Tm, B);      \
-    else hipLaunchKernelGGL((trsm_split_kernel<T, false, false, G_, R_>), grid, blk, 0, s, n, nrhs, Tm, B);              \
+    if (lower && unit) hipLaunchKernelGGL((trsm_split_kernel<T, true, true, G_, R_>), grid, blk, 0, s, n, nrhs, Tm, B, prio);   \
+    else if (lower) hipLaunchKernelGGL((trsm_split_kernel<T, true, false, G_, R_>), grid, blk, 0, s, n, nrhs, Tm, B, prio);     \
+    else if (unit) hipLaunchKernelGGL((trsm_split_kernel<T, false, true, G_, R_>), grid, blk, 0, s, n, nrhs, Tm, B, prio);      \
+    else hipLaunchKernelGGL((trsm_split_kernel<T, false, false, G_, R_>), grid, blk, 0, s, n, nrhs, Tm, B, prio);              \
   }
     if (n <= 32) MG_SPLIT(4, 8) else if (n <= 64) MG_SPLIT(8, 8) else MG_SPLIT(16, 8)
 #undef MG_SPLIT
@@ -303,10 +306,10 @@ void trsm_left(hipStream_t s, bool lower, bool unit, int n, int nrhs, View<const
     // small: one launch of the register-resident leaf
     dim3 grid(ceil_div(nrhs, TRSM_NT)), blk(TRSM_NT);
     KScope ks(s, "trsm");
-    if (lower && unit) hipLaunchKernelGGL((trsm_leaf_kernel<T, true, true>), grid, blk, 0, s, n, nrhs, Tm, B);
-    else if (lower) hipLaunchKernelGGL((trsm_leaf_kernel<T, true, false>), grid, blk, 0, s, n, nrhs, Tm, B);
-    else if (unit) hipLaunchKernelGGL((trsm_leaf_kernel<T, false, true>), grid, blk, 0, s, n, nrhs, Tm, B);
-    else hipLaunchKernelGGL((trsm_leaf_kernel<T, false, false>), grid, blk, 0, s, n, nrhs, Tm, B);
+    if (lower && unit) hipLaunchKernelGGL((trsm_leaf_kernel<T, true, true>), grid, blk, 0, s, n, nrhs, Tm, B, prio);
+    else if (lower) hipLaunchKernelGGL((trsm_leaf_kernel<T, true, false>), grid, blk, 0, s, n, nrhs, Tm, B, prio);
+    else if (unit) hipLaunchKernelGGL((trsm_leaf_kernel<T, false, true>), grid, blk, 0, s, n, nrhs, Tm, B, prio);
+    else hipLaunchKernelGGL((trsm_leaf_kernel<T, false, false>), grid, blk, 0, s, n, nrhs, Tm, B, prio);
     MG_HIP(hipGetLastError());
     return;
   }

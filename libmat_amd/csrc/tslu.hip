@@ -75,6 +75,7 @@ struct TsluP {
   unsigned tag;   // distinct per launch, never 0
   int G;          // workgroups (one per 256 rows)
   int P;          // leaf workgroups of the tree (one per TS_PR rows; = G here)
+  int prio;       // wave priority (wave_prio of the launch stream)
 };
 
 // Wavefront max of int keys, in every lane: DPP within each 16-lane row (one
@@ -332,6 +333,7 @@ __device__ __forceinline__ void ts_store_row(float *row, const float (&x)[TS_W],
 }
 
 __global__ __launch_bounds__(TS_NT) void tslu_kernel(TsluP p) {
+  if (p.prio) __builtin_amdgcn_s_setprio(3);
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6, b = blockIdx.x;
   const int w = p.w;
   const int base = p.r0 + b * TS_NT;
@@ -616,6 +618,7 @@ void tslu_leaf(hipStream_t s, int n, float *W, long ld, int c0, int w, int *ipiv
   p.tag = tag;
   p.G = ceil_div(m, TS_NT);
   p.P = ceil_div(m, TS_PR);
+  p.prio = wave_prio(s);
   KScope ks(s, "panel", 2.0 * m * w * sizeof(float));
   hipLaunchKernelGGL(tslu_kernel, dim3(p.G), dim3(TS_NT), 0, s, p);
   MG_HIP(hipGetLastError());
