@@ -19,7 +19,7 @@
 //  * Tiles are mapped XCD-aware: consecutive workgroup ids land on different
 //    XCDs, so ids are regrouped so that each XCD's L2 sees a compact band of
 //    C tiles (grouped along M) that shares A rows and B columns.
-#include "common.hpp"
+#include "ops.hpp"
 
 namespace mg {
 
@@ -407,6 +407,101 @@ void gemm(hipStream_t s, long M, long N, long K, T alpha, View<const T> A, View<
   else if (!a_mn && !b_mn) launch_mode<T, false, false>(s, p, vec, in_place, grid_cap);
   else launch_mode<T, false, true>(s, p, vec, in_place, grid_cap);
 }
+
+// ------------------------------------------------ folded tile rows ----
+// Tp_r = D_r T(r, c0 : c0 + cols) for every block row r of a triangle in
+// one launch (the folded tiles of the chained triangular solve, trsv.hip):
+// block row r = blockIdx.y, tile blockIdx.x of its product.  Each block row
+// is computed exactly as gemm() computes it on its own: C_r^T = T_r^T D_r^T
+// into the column-major view of the row-major Tp_r, with the 64 x 64 tiles
+// and deep slabs gemm() picks for such products (fewer 128 x 128 tiles than
+// 256 and K >= 32); rows that gemm() would tile otherwise are left to it.
+template <typename T>
+struct FoldP {
+  const T *Tm;
+  long trs, tcs;  // the triangle's view: element (i, j) at Tm[i * trs + j * tcs]
+  const T *D;     // inverted TB x TB diagonal blocks, row-major, in order
+  T *Tp;
+  int n, TB, lower, prio;
+};
+
+template <typename T>
+__host__ __device__ inline bool fold_row_batched(int rrows, int cols) {
+  return rrows >= 32 && (long)((cols + 127) / 128) * ((rrows + 127) / 128) < 256;
+}
+
+template <typename T, int BM, int BN, int BK, bool A_MN, bool VEC>
+__global__ __launch_bounds__(256) void fold_rows_kernel(FoldP<T> f) {
+  constexpr int PAD = 16 / (int)sizeof(T);
+  __shared__ T As[2][BK * (BM + PAD)];
+  __shared__ T Bs[2][BK * (BN + PAD)];
+  const int r = blockIdx.y;
+  const int ro = r * f.TB, rrows = min(f.TB, f.n - ro);
+  const int c0 = f.lower ? 0 : ro + f.TB;
+  const int cols = f.lower ? ro : f.n - c0;
+  if (cols <= 0 || !fold_row_batched<T>(rrows, cols)) return;
+  const int tiles_n = (rrows + BN - 1) / BN;
+  const int id = blockIdx.x;
+  if (id >= ((cols + BM - 1) / BM) * tiles_n) return;
+  if (f.prio) __builtin_amdgcn_s_setprio(3);
+  GemmP<T> p{};
+  p.A = f.Tm + (long)ro * f.trs + (long)c0 * f.tcs;  // A'(i, k) = T(ro + k, c0 + i)
+  p.lda = A_MN ? f.trs : f.tcs;
+  p.B = f.D + (long)r * f.TB * f.TB;  // B'(k, j) = D_r(j, k)
+  p.ldb = f.TB;
+  p.C = f.Tp + fold_off(f.lower != 0, r, f.n, f.TB);
+  p.ldc = fold_ld(f.lower != 0, r, f.n, f.TB);
+  p.M = cols;
+  p.N = rrows;
+  p.K = rrows;
+  p.alpha = T(1);
+  p.beta = T(0);
+  gemm_tile<T, BM, BN, BK, A_MN, false, VEC>(p, id / tiles_n, id % tiles_n, As, Bs);
+}
+
+template <typename T>
+void gemm_fold_rows(hipStream_t s, bool lower, int n, int TB, View<const T> Tm, const T *D, T *Tp) {
+  const int nb = ceil_div(n, TB);
+  if (nb <= 1) return;
+  const bool a_mn = Tm.cs == 1;
+  MG_CHECK(a_mn || Tm.rs == 1, "gemm_fold_rows: the triangle needs a unit stride");
+  constexpr int BM = 64, BN = 64, BK = sizeof(T) == 4 ? 64 : 32;
+  constexpr int VW = 16 / (int)sizeof(T);
+  const long lda = a_mn ? Tm.rs : Tm.cs;
+  // every block row's operands start at multiples of TB (>= VW) elements
+  const bool vec = (uintptr_t)Tm.p % 16 == 0 && lda % VW == 0 && (uintptr_t)D % 16 == 0 &&
+                   TB % VW == 0;
+  int maxt = 0;
+  double work = 0.0;
+  for (int r = 0; r < nb; ++r) {
+    const int ro = r * TB, rrows = std::min(TB, n - ro);
+    const int c0 = lower ? 0 : ro + TB;
+    const int cols = lower ? ro : n - c0;
+    if (cols <= 0) continue;
+    if (!fold_row_batched<T>(rrows, cols)) {
+      // a short last block row or a very wide one: gemm() tiles it differently
+      View<const T> Dr{D + (long)r * TB * TB, TB, 1};
+      View<T> C{Tp + fold_off(lower, r, n, TB), fold_ld(lower, r, n, TB), 1};
+      gemm<T>(s, rrows, cols, rrows, T(1), Dr, Tm.sub(ro, c0), T(0), C, false);
+      continue;
+    }
+    maxt = std::max(maxt, (int)(ceil_div(cols, BM) * ceil_div(rrows, BN)));
+    work += 2.0 * rrows * cols * rrows;
+  }
+  if (maxt == 0) return;
+  FoldP<T> f{Tm.p, Tm.rs, Tm.cs, D, Tp, n, TB, lower ? 1 : 0, wave_prio(s)};
+  KScope ks(s, "gemm", work);
+  const dim3 grid(maxt, nb);
+  if (a_mn && vec) hipLaunchKernelGGL((fold_rows_kernel<T, BM, BN, BK, true, true>), grid, dim3(256), 0, s, f);
+  else if (a_mn) hipLaunchKernelGGL((fold_rows_kernel<T, BM, BN, BK, true, false>), grid, dim3(256), 0, s, f);
+  else if (vec) hipLaunchKernelGGL((fold_rows_kernel<T, BM, BN, BK, false, true>), grid, dim3(256), 0, s, f);
+  else hipLaunchKernelGGL((fold_rows_kernel<T, BM, BN, BK, false, false>), grid, dim3(256), 0, s, f);
+  MG_HIP(hipGetLastError());
+}
+
+template void gemm_fold_rows<float>(hipStream_t, bool, int, int, View<const float>, const float *, float *);
+template void gemm_fold_rows<double>(hipStream_t, bool, int, int, View<const double>, const double *,
+                                     double *);
 
 template void gemm<float>(hipStream_t, long, long, long, float, View<const float>,
                           View<const float>, float, View<float>, int, int);

@@ -4,6 +4,25 @@
 
 namespace mg {
 
+// Folded tiles T'(r,k) = D_r T(r,k): block row r of the lower triangle is a
+// row-major TB x (r TB) matrix holding T'(r, 0..r-1); of the upper triangle
+// a row-major TB x (n - (r+1) TB) matrix holding T'(r, r+1..nb-1).
+__host__ __device__ inline long fold_off(bool lower, int r, int n, int TB) {
+  if (lower) return (long)TB * TB * ((long)r * (r - 1) / 2);
+  return (long)TB * ((long)r * n - (long)TB * r * (r + 1) / 2);
+}
+__host__ __device__ inline long fold_ld(bool lower, int r, int n, int TB) {
+  return lower ? (long)r * TB : (long)n - (long)(r + 1) * TB;
+}
+
+// Tp := D_r T(r, k) for every off-diagonal tile of block row r, all block
+// rows of one triangle in one launch (gemm.hip).  D: the inverted TB x TB
+// diagonal blocks, row-major, one after another; Tp laid out by fold_off /
+// fold_ld.  Tiles, slab depth and the order of the sums are those of one
+// gemm() call per block row, so the result is bit-identical to it.
+template <typename T>
+void gemm_fold_rows(hipStream_t s, bool lower, int n, int TB, View<const T> Tm, const T *D, T *Tp);
+
 // ----------------------------------------------------------------- LU ----
 // Row-major working layout: element (i,j) of the n x n matrix at W[i*ld+j].
 // Partial pivoting with libmat's rules: the pivot of column k is the first

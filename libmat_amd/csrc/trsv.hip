@@ -252,17 +252,6 @@ __device__ __forceinline__ uint32_t chain_word(T v, int h) {
   }
 }
 
-// Folded tiles T'(r,k) = D_r T(r,k): block row r of the lower triangle is a
-// row-major TB x (r TB) matrix holding T'(r, 0..r-1); of the upper triangle
-// a row-major TB x (n - (r+1) TB) matrix holding T'(r, r+1..nb-1).
-__host__ __device__ inline long fold_off(bool lower, int r, int n, int TB) {
-  if (lower) return (long)TB * TB * ((long)r * (r - 1) / 2);
-  return (long)TB * ((long)r * n - (long)TB * r * (r + 1) / 2);
-}
-__host__ __device__ inline long fold_ld(bool lower, int r, int n, int TB) {
-  return lower ? (long)r * TB : (long)n - (long)(r + 1) * TB;
-}
-
 // FOLD: tiles are the folded T'(r,k) and the workgroup starts from D_r b_r,
 // so the product with the diagonal block leaves the chain's critical path.
 // fp32 tile product on MFMA for the chained solve: acc (+|-)= T(16w.., :)
@@ -590,20 +579,11 @@ size_t tri_fold_size(bool lower, int n) {
   return (size_t)(lower ? fold_off(true, nb, n, TB) : fold_off(false, nb - 1, n, TB));
 }
 
-// Tp := D_r T(r, k) for every off-diagonal tile, one MFMA GEMM per block row.
+// Tp := D_r T(r, k) for every off-diagonal tile: all block rows of the
+// triangle in one launch of MFMA tile products.
 template <typename T>
 void tri_fold_tiles(hipStream_t s, bool lower, int n, View<const T> Tm, const T *D, T *Tp) {
-  constexpr int TB = TriTB<T>::TB;
-  const int nb = (n + TB - 1) / TB;
-  for (int r = 0; r < nb; ++r) {
-    const int ro = r * TB, rrows = std::min(TB, n - ro);
-    const int c0 = lower ? 0 : ro + TB;
-    const int cols = lower ? ro : n - c0;
-    if (cols <= 0) continue;
-    View<const T> Dr{D + (long)r * TB * TB, TB, 1};
-    View<T> C{Tp + fold_off(lower, r, n, TB), fold_ld(lower, r, n, TB), 1};
-    gemm<T>(s, rrows, cols, rrows, T(1), Dr, Tm.sub(ro, c0), T(0), C, false);
-  }
+  gemm_fold_rows<T>(s, lower, n, TriTB<T>::TB, Tm, D, Tp);
 }
 
 // X := op(T)^-1 X, X column-major n x nrhs; D from tri_invert_blocks and,
