@@ -179,6 +179,9 @@ struct LpP {
   int tiles_m, tiles_n, tiles;
   int lower;
   int vgrid;  // workgroup ids of the tile map (tiles rounded up to 8)
+  // persistent grid (fewer workgroups than ids): per-XCD claim counters,
+  // else null (one tile per workgroup)
+  unsigned *ctr;
 };
 
 // Workgroup id -> (tm, tn).  The ids are regrouped by XCD; lower products
@@ -255,11 +258,31 @@ void gemm_bf16x_kernel(LpP p) {
   const int wm = wave & 1, wn = wave >> 1;
   const int g = lane >> 5, li = lane & 31;
   // one tile per workgroup, or (a grid capped below the tile count, with a
-  // multiple of 8 workgroups) tiles id, id + grid, ...: every id keeps its
-  // XCD, so the tile map's XCD grouping holds
-  for (int vid = blockIdx.x; vid < p.vgrid; vid += gridDim.x) {
+  // multiple of 8 workgroups) tiles claimed one at a time: workgroup b takes
+  // the next id of XCD b % 8's share of the tile map (so the map's XCD
+  // grouping holds), and workgroups that start late, behind other kernels
+  // on their compute units, simply take fewer.  The counter wraps to zero
+  // on its last claim (every workgroup ends on exactly one claim past the
+  // share), ready for the next launch.
+  __shared__ int s_claim;
+  const int xcd = blockIdx.x % 8, per = p.vgrid / 8;
+  for (int vid = blockIdx.x, turn = 0;; vid += gridDim.x, ++turn) {
   int tm, tn;
-  if (!lp_map_tile(vid, p.vgrid, p.tiles, p.tiles_m, p.tiles_n, tm, tn, p.lower)) continue;
+  if (p.ctr) {
+    __syncthreads();  // every lane has read the previous claim
+    if (t == 0)
+      s_claim = (int)__builtin_amdgcn_atomic_inc32(p.ctr + xcd, (unsigned)(per + gridDim.x / 8 - 1),
+                                                   __ATOMIC_RELAXED, "agent");
+    __syncthreads();
+    const int j = s_claim;
+    if (j >= per) break;
+    const int id = xcd * per + j;
+    if (id >= p.tiles) continue;
+    lp_tile_of(id, p.tiles_m, p.tiles_n, tm, tn, p.lower);
+  } else {
+    if (turn > 0) break;
+    if (!lp_map_tile(vid, p.vgrid, p.tiles, p.tiles_m, p.tiles_n, tm, tn, p.lower)) continue;
+  }
   const int bm0 = tm * BM, bn0 = tn * BN;
   if (p.lower && bm0 + BM - 1 < bn0) continue;
 
@@ -500,7 +523,12 @@ void gemm_bf16x(hipStream_t s, long M, long N, long K, float alpha, View<const f
   int grid = p.tiles;
   if (grid >= 8) grid = (grid + 7) / 8 * 8;
   p.vgrid = grid;
-  if (grid_cap >= 8 && grid > grid_cap) grid = grid_cap / 8 * 8;
+  if (grid_cap >= 8 && grid > grid_cap) {
+    grid = grid_cap / 8 * 8;
+    bool fresh = false;
+    p.ctr = (unsigned *)ws.get("lp.ctr", 8 * sizeof(unsigned), &fresh);
+    if (fresh) fill_u32(s, p.ctr, 8, 0u);
+  }
   KScope ks(s, "gemm_bf16x", 2.0 * passes * M * N * K);
   if (passes == 1) hipLaunchKernelGGL((gemm_bf16x_kernel<1, 1>), dim3(grid), dim3(256), 0, s, p);
   else if (passes == 3) hipLaunchKernelGGL((gemm_bf16x_kernel<2, 3>), dim3(grid), dim3(256), 0, s, p);

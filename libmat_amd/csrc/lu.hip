@@ -23,7 +23,8 @@ namespace mg {
 
 template <typename T>
 void lu_leaf(hipStream_t s, int n, T *W, long ld, int c0, int w, int *ipiv, int *perm, int *pairs,
-             T eps, int cl, int ch, bool tournament);
+             T eps, int cl, int ch, bool tournament, T *linv);
+int tslu_max_rows();  // tallest panel of the tournament leaf (tslu.hip)
 template <typename T>
 void lu_rowswap(hipStream_t s, T *W, long ld, const int *pairs, int *perm, int k0, int w, int cl,
                 int ch);
@@ -61,30 +62,47 @@ static int side_grid_cap() {
   return (int)(per_cu * num_cus());
 }
 
+// X := L^-1 X for the unit lower triangle of rows [r0, r0 + nt): from the
+// tournament leaves' diagonal-block inverses when there are (linv, block
+// solves in one launch), else by substitution.
+template <typename T>
+static void trsm_unit_lower(hipStream_t s, int nt, int nrhs, View<const T> L, View<T> X, T *linv,
+                            int r0, Mul mul) {
+  if constexpr (std::is_same<T, float>::value) {
+    if (linv && nt <= 256) {
+      trsm_lower_unit_linv(s, nt, nrhs, L, linv + (long)r0 * 32, X);
+      return;
+    }
+  }
+  trsm_left<T>(s, true, true, nt, nrhs, L, X, mul);
+}
+
 // Factor columns [c0, c0+w) (rows c0..n-1, all earlier updates applied)
 // with leaves of at most `leaf` columns (every leaf starts at c0 + a
 // multiple of `leaf`); the leaves' row interchanges reach columns [cl, ch).
+// linv: where the tournament leaves put their diagonal blocks' inverses
+// (32-column leaves), or null.
 template <typename T>
 static void getrf_rec(hipStream_t s, int n, T *W, long ld, int c0, int w, int *ipiv, int *perm,
-                      int *pairs, T eps, const LuOpts &o, int leaf, int cl, int ch) {
+                      int *pairs, T eps, const LuOpts &o, int leaf, int cl, int ch, T *linv) {
   if (w <= leaf) {
-    lu_leaf<T>(s, n, W, ld, c0, w, ipiv, perm, pairs, eps, cl, ch, o.tournament);
+    lu_leaf<T>(s, n, W, ld, c0, w, ipiv, perm, pairs, eps, cl, ch, o.tournament, linv);
     return;
   }
   int w1 = ((w / 2 + leaf - 1) / leaf) * leaf;
   if (w1 >= w) w1 = w - leaf;
   const int w2 = w - w1;
-  getrf_rec<T>(s, n, W, ld, c0, w1, ipiv, perm, pairs, eps, o, leaf, cl, ch);
+  getrf_rec<T>(s, n, W, ld, c0, w1, ipiv, perm, pairs, eps, o, leaf, cl, ch, linv);
   // U12 = L11^-1 A12 (unit lower)
   View<const T> L11{W + (long)c0 * ld + c0, ld, 1};
   View<T> A12{W + (long)c0 * ld + c0 + w1, ld, 1};
-  trsm_left<T>(s, true, true, w1, w2, L11, A12, o.mul);
+  trsm_unit_lower<T>(s, w1, w2, L11, A12, linv, c0, o.mul);
   // A22 -= L21 U12
   const int m2 = n - c0 - w1;
   View<const T> L21{W + (long)(c0 + w1) * ld + c0, ld, 1};
   View<T> A22{W + (long)(c0 + w1) * ld + c0 + w1, ld, 1};
   lu_gemm_sub<T>(s, m2, w2, w1, L21, cview(A12), A22, o.mul);
-  getrf_rec<T>(s, n, W, ld, c0 + w1, w2, ipiv, perm, pairs, eps, o, leaf, cl, ch);
+  getrf_rec<T>(s, n, W, ld, c0 + w1, w2, ipiv, perm, pairs, eps, o, leaf, cl, ch, linv);
 }
 
 // Row interchanges of the factored block [b0, b1) (its leaves in order)
@@ -100,12 +118,12 @@ static void block_swaps(hipStream_t s, T *W, long ld, const int *pairs, int b0, 
 // the columns [c0, c1).
 template <typename T>
 static void block_update(hipStream_t s, int n, T *W, long ld, int b0, int b1, int c0, int c1,
-                         const LuOpts &o, int grid_cap = 0) {
+                         const LuOpts &o, T *linv, int grid_cap = 0) {
   if (c1 <= c0) return;
   const int nb = b1 - b0;
   View<const T> L11{W + (long)b0 * ld + b0, ld, 1};
   View<T> A12{W + (long)b0 * ld + c0, ld, 1};
-  trsm_left<T>(s, true, true, nb, c1 - c0, L11, A12, o.mul);
+  trsm_unit_lower<T>(s, nb, c1 - c0, L11, A12, linv, b0, o.mul);
   View<const T> L21{W + (long)b1 * ld + b0, ld, 1};
   View<T> A22{W + (long)b1 * ld + c0, ld, 1};
   lu_gemm_sub<T>(s, n - b1, c1 - c0, nb, L21, cview(A12), A22, o.mul, grid_cap);
@@ -126,7 +144,7 @@ static void block_update(hipStream_t s, int n, T *W, long ld, int b0, int b1, in
 // and interchanges of different blocks reach any column in pivot order.
 template <typename T>
 static void getrf_lookahead(hipStream_t s, int n, T *W, long ld, int *ipiv, int *perm, int *pairs,
-                            T eps, const LuOpts &o, int NB) {
+                            T eps, const LuOpts &o, int NB, T *linv) {
   // the side stream and the event pool are per device: one enqueue at a time
   std::lock_guard<std::mutex> lk(lookahead_mutex());
   hipStream_t u = side_stream();
@@ -143,7 +161,7 @@ static void getrf_lookahead(hipStream_t s, int n, T *W, long ld, int *ipiv, int 
   // leaf width of a block: the panel kernel's limit for the block's height
   auto leaf_of = [&](int b0) { return std::min(o.leaf, lu_leaf_max(n - b0, (int)sizeof(T))); };
   getrf_rec<T>(s, n, W, ld, 0, std::min(NB, n), ipiv, perm, pairs, eps, o, leaf_of(0), 0,
-               std::min(NB, n));
+               std::min(NB, n), linv);
   for (int k = 0; k < nblk; ++k) {
     const int b0 = k * NB, b1 = std::min(n, b0 + NB);
     const int c2 = std::min(n, b1 + NB);
@@ -152,17 +170,18 @@ static void getrf_lookahead(hipStream_t s, int n, T *W, long ld, int *ipiv, int 
       // caller: block k+1's columns, then its panel
       if (k >= 1) MG_HIP(hipStreamWaitEvent(s, evU[k - 1], 0));
       block_swaps<T>(s, W, ld, pairs, b0, b1, leaf_of(b0), b1, c2);
-      block_update<T>(s, n, W, ld, b0, b1, b1, c2, o);
-      getrf_rec<T>(s, n, W, ld, b1, c2 - b1, ipiv, perm, pairs, eps, o, leaf_of(b1), b1, c2);
+      block_update<T>(s, n, W, ld, b0, b1, b1, c2, o, linv);
+      getrf_rec<T>(s, n, W, ld, b1, c2 - b1, ipiv, perm, pairs, eps, o, leaf_of(b1), b1, c2,
+                   linv);
     }
     // side: the rest of the trailing matrix, block k+2's columns first (the
     // only part the caller's next step waits for), then the columns on the left
     MG_HIP(hipStreamWaitEvent(u, evP[k], 0));
     const int c3 = std::min(n, c2 + NB);
     block_swaps<T>(u, W, ld, pairs, b0, b1, leaf_of(b0), c2, n);
-    block_update<T>(u, n, W, ld, b0, b1, c2, c3, o, side_grid_cap());
+    block_update<T>(u, n, W, ld, b0, b1, c2, c3, o, linv, side_grid_cap());
     MG_HIP(hipEventRecord(evU[k], u));
-    block_update<T>(u, n, W, ld, b0, b1, c3, n, o, side_grid_cap());
+    block_update<T>(u, n, W, ld, b0, b1, c3, n, o, linv, side_grid_cap());
     block_swaps<T>(u, W, ld, pairs, b0, b1, leaf_of(b0), 0, b0);
   }
   hipEvent_t done = pooled_event(2 * nblk + 1);
@@ -186,12 +205,18 @@ void getrf_rowmajor_perm(hipStream_t s, int n, T *W, long ld, int *ipiv, int *pe
   MG_HIP(hipGetLastError());
   // every leaf's interchanges as row pairs, written by its panel kernel
   int *pairs = (int *)workspace(s).get("getrf.pairs", (size_t)n * kPairStride * sizeof(int));
+  // the tournament leaves (fp32, 32 columns) also write the inverses of
+  // their unit lower diagonal blocks, and the solves with the factored
+  // blocks' L then run as block steps (trsm_lower_unit_linv)
+  T *linv = nullptr;
+  if (std::is_same<T, float>::value && o.tournament && o.leaf == 32 && n <= tslu_max_rows())
+    linv = (T *)workspace(s).get("getrf.linv", (size_t)ceil_div(n, 32) * 32 * 32 * sizeof(T));
   const int NB = lookahead_block();
   if (NB > 0 && n >= 4 * NB && NB % o.leaf == 0)
-    getrf_lookahead<T>(s, n, W, ld, ipiv, perm, pairs, eps, o, NB);
+    getrf_lookahead<T>(s, n, W, ld, ipiv, perm, pairs, eps, o, NB, linv);
   else
     getrf_rec<T>(s, n, W, ld, 0, n, ipiv, perm, pairs, eps, o,
-                 std::min(o.leaf, lu_leaf_max(n, (int)sizeof(T))), 0, n);
+                 std::min(o.leaf, lu_leaf_max(n, (int)sizeof(T))), 0, n, linv);
 }
 
 template <typename T>

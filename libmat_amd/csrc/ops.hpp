@@ -81,6 +81,11 @@ void potrf_lower(hipStream_t s, int n, T *A, long lda, int *info, Mul mul = Mul:
 template <typename T>
 void trsm_left(hipStream_t s, bool lower, bool unit, int n, int nrhs, View<const T> Tm,
                View<T> B, Mul mul = Mul::Native);
+// X := L^-1 B for a unit lower triangle of at most 256 rows whose 32 x 32
+// diagonal blocks' inverses are given (row-major, block i at linv + 1024 i,
+// as the tournament leaves write them); L and B row-major.
+void trsm_lower_unit_linv(hipStream_t s, int nt, int nrhs, View<const float> L, const float *linv,
+                          View<float> B);
 
 // ------------------------------------------------------------ utilities --
 template <typename T>

@@ -1177,7 +1177,7 @@ static void lu_leaf_w(hipStream_t s, int n, T *W, long ld, int c0, int w, int *i
 
 int tslu_max_rows();
 void tslu_leaf(hipStream_t s, int n, float *W, long ld, int c0, int w, int *ipiv, int *pairs,
-               float eps);
+               float eps, float *linv);
 
 int lu_leaf_max(int m, int elem_size) {
   return (elem_size == 8 && m > 64 * PanelNT<double>::value) ? 32 : 64;
@@ -1188,15 +1188,16 @@ int lu_leaf_max(int m, int elem_size) {
 // (and to perm).  Leaves up to 32 columns use the 32-column kernel.
 template <typename T>
 void lu_leaf(hipStream_t s, int n, T *W, long ld, int c0, int w, int *ipiv, int *perm, int *pairs,
-             T eps, int cl, int ch, bool tournament) {
+             T eps, int cl, int ch, bool tournament, T *linv) {
   MG_CHECK(w >= 1 && w <= 64, "lu_leaf: leaf width must be in [1, 64]");
   bool done = false;
   if constexpr (std::is_same<T, float>::value) {
     if (tournament && w <= 32 && n - c0 <= tslu_max_rows()) {
-      tslu_leaf(s, n, W, ld, c0, w, ipiv, pairs, eps);
+      tslu_leaf(s, n, W, ld, c0, w, ipiv, pairs, eps, linv);
       done = true;
     }
   }
+  MG_CHECK(done || linv == nullptr, "lu_leaf: diagonal-block inverses need the tournament leaf");
   if (!done) {
     if (w <= 32) lu_leaf_w<T, 32>(s, n, W, ld, c0, w, ipiv, pairs, eps);
     else lu_leaf_w<T, 64>(s, n, W, ld, c0, w, ipiv, pairs, eps);
@@ -1205,9 +1206,9 @@ void lu_leaf(hipStream_t s, int n, T *W, long ld, int c0, int w, int *ipiv, int 
 }
 
 template void lu_leaf<float>(hipStream_t, int, float *, long, int, int, int *, int *, int *, float, int,
-                             int, bool);
+                             int, bool, float *);
 template void lu_leaf<double>(hipStream_t, int, double *, long, int, int, int *, int *, int *, double,
-                              int, int, bool);
+                              int, int, bool, double *);
 template void lu_rowswap<float>(hipStream_t, float *, long, const int *, int *, int, int, int, int);
 template void lu_rowswap<double>(hipStream_t, double *, long, const int *, int *, int, int, int, int);
 template void lu_rowswap_block<float>(hipStream_t, float *, long, const int *, int, int, int, int,

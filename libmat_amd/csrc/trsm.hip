@@ -320,6 +320,191 @@ void trsm_left(hipStream_t s, bool lower, bool unit, int n, int nrhs, View<const
   trsm_inv_rec<T>(s, lower, n, nrhs, Tm, B, D, 0, tb, mul);
 }
 
+// ---------------------------------------- block solves with inverses ----
+// X := L^-1 B for a unit lower triangle L of nt <= 256 rows whose 32 x 32
+// diagonal blocks come with their inverses (the tournament leaves write
+// them, linv + 1024 i for block i), both L and B row-major.  Block by block,
+// in one workgroup per 16 right-hand-side columns:
+//   Y_i = B_i - L(i, 0 : 32 i) X(0 : 32 i),   X_i = L_ii^-1 Y_i,
+// so the chain is nt / 32 block steps with workgroup barriers instead of nt
+// dependent row steps.  The product of each step is split over the four
+// wavefronts by K (partials summed in wavefront order, deterministic); a
+// lane owns 2 rows x 4 columns of it.  Block row i + 1 of L is loaded into
+// registers while step i computes and goes to LDS once step i's product is
+// done; L_ii^-1 likewise one step ahead.  LDS stays under 64 KB, so a
+// workgroup fits on a compute unit beside two split-BF16 GEMM workgroups of
+// the side stream.
+constexpr int LV_R = 256;           // rows of the largest triangle
+constexpr int LV_C = 16;            // right-hand-side columns per workgroup
+constexpr int LV_XLD = LV_C + 4;    // LDS row stride of X and the partials
+constexpr int LV_LBC = LV_R - 32;   // widest block row of L (224 columns)
+constexpr int LV_LBLD = LV_LBC + 4;
+constexpr int LV_LSLD = 36;
+
+template <bool VEC>
+__device__ __forceinline__ float4 lv_load4(const float *p, int valid) {
+  if (VEC && valid >= 4) return *reinterpret_cast<const float4 *>(p);
+  float4 v = {0.0f, 0.0f, 0.0f, 0.0f};
+  if (valid > 0) v.x = p[0];
+  if (valid > 1) v.y = p[1];
+  if (valid > 2) v.z = p[2];
+  if (valid > 3) v.w = p[3];
+  return v;
+}
+
+template <bool VEC>
+__global__ __launch_bounds__(256) void trsm_linv_kernel(int nt, int nrhs, const float *L, long ldl,
+                                                        const float *Li, float *B, long ldb,
+                                                        int prio) {
+  if (prio) __builtin_amdgcn_s_setprio(3);
+  __shared__ __align__(16) float Xs[LV_R * LV_XLD];
+  __shared__ __align__(16) float Ps[4][32 * LV_XLD];
+  __shared__ __align__(16) float LB[32 * LV_LBLD];  // block row i of L
+  __shared__ __align__(16) float Ls[32 * LV_LSLD];  // L_ii^-1
+  const int t = threadIdx.x, wv = t >> 6, l = t & 63, rg = l >> 2, cg = l & 3;
+  const int c0 = blockIdx.x * LV_C;
+  const int nbk = (nt + 31) / 32;
+  // B rows [0, nt) (zero past nt and past nrhs)
+  for (int e = t; e < nbk * 32 * (LV_C / 4); e += 256) {
+    const int r = e / (LV_C / 4), c4 = (e % (LV_C / 4)) * 4;
+    const float4 v = r < nt ? lv_load4<VEC>(B + (long)r * ldb + c0 + c4, nrhs - c0 - c4)
+                            : float4{0.0f, 0.0f, 0.0f, 0.0f};
+    *reinterpret_cast<float4 *>(&Xs[r * LV_XLD + c4]) = v;
+  }
+  // prefetch registers: a block row of L (32 rows x up to 224 columns,
+  // float4 e = t + 256 u is row e / 56, columns 4 (e % 56) ..) and one
+  // float4 of an inverse block (row t / 8, columns 4 (t % 8) ..)
+  constexpr int NU = 32 * LV_LBC / 4 / 256;  // 7
+  float4 pl[NU], pi;
+  auto fetch_l = [&](int ib) {
+    const int cols = 32 * ib;
+#pragma unroll
+    for (int u = 0; u < NU; ++u) {
+      const int e = t + 256 * u, r = e / (LV_LBC / 4), c4 = (e % (LV_LBC / 4)) * 4;
+      pl[u] = (c4 < cols && 32 * ib + r < nt) ? lv_load4<VEC>(L + (long)(32 * ib + r) * ldl + c4, 4)
+                                              : float4{0.0f, 0.0f, 0.0f, 0.0f};
+    }
+  };
+  auto commit_l = [&](int ib) {
+    const int cols = 32 * ib;
+#pragma unroll
+    for (int u = 0; u < NU; ++u) {
+      const int e = t + 256 * u, r = e / (LV_LBC / 4), c4 = (e % (LV_LBC / 4)) * 4;
+      if (c4 < cols) *reinterpret_cast<float4 *>(&LB[r * LV_LBLD + c4]) = pl[u];
+    }
+  };
+  auto fetch_i = [&](int ib) { pi = *reinterpret_cast<const float4 *>(Li + (long)ib * 1024 + 4 * t); };
+  auto commit_i = [&]() { *reinterpret_cast<float4 *>(&Ls[(t >> 3) * LV_LSLD + 4 * (t & 7)]) = pi; };
+  fetch_i(0);
+  commit_i();
+  if (nbk > 1) fetch_l(1);
+  // this lane's rows / columns in the Y and X phases
+  const int yr = t >> 3, yc = (t & 7) * 2;
+  for (int i = 0; i < nbk; ++i) {
+    // Xs of the earlier blocks (and B) and block row i in place; the
+    // previous step's readers of Ls are past this barrier
+    __syncthreads();
+    if (i > 0) commit_i();  // L_ii^-1, fetched during the previous step
+    if (i + 1 < nbk) fetch_i(i + 1);
+    // partial products over this wavefront's quarter of k in [0, 32 i)
+    const int kq = 8 * i;  // = 32 i / 4
+    const int kb = wv * kq;
+    float acc[2][4];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) acc[a][c] = 0.0f;
+    for (int k = kb; k < kb + kq; k += 4) {
+      const float4 l0 = *reinterpret_cast<const float4 *>(&LB[(2 * rg) * LV_LBLD + k]);
+      const float4 l1 = *reinterpret_cast<const float4 *>(&LB[(2 * rg + 1) * LV_LBLD + k]);
+#pragma unroll
+      for (int kk = 0; kk < 4; ++kk) {
+        const float4 xv = *reinterpret_cast<const float4 *>(&Xs[(k + kk) * LV_XLD + 4 * cg]);
+        const float a0 = kk == 0 ? l0.x : kk == 1 ? l0.y : kk == 2 ? l0.z : l0.w;
+        const float a1 = kk == 0 ? l1.x : kk == 1 ? l1.y : kk == 2 ? l1.z : l1.w;
+        acc[0][0] = fmaf(a0, xv.x, acc[0][0]);
+        acc[0][1] = fmaf(a0, xv.y, acc[0][1]);
+        acc[0][2] = fmaf(a0, xv.z, acc[0][2]);
+        acc[0][3] = fmaf(a0, xv.w, acc[0][3]);
+        acc[1][0] = fmaf(a1, xv.x, acc[1][0]);
+        acc[1][1] = fmaf(a1, xv.y, acc[1][1]);
+        acc[1][2] = fmaf(a1, xv.z, acc[1][2]);
+        acc[1][3] = fmaf(a1, xv.w, acc[1][3]);
+      }
+    }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+      *reinterpret_cast<float4 *>(&Ps[wv][(2 * rg + a) * LV_XLD + 4 * cg]) =
+          float4{acc[a][0], acc[a][1], acc[a][2], acc[a][3]};
+    __syncthreads();
+    // every reader of LB is done: block row i + 1 goes in, i + 2 is loaded
+    if (i + 1 < nbk) {
+      commit_l(i + 1);
+      if (i + 2 < nbk) fetch_l(i + 2);
+    }
+    // Y_i = B_i - (partials in wavefront order), into Ps[0]
+    {
+      const float2 b = *reinterpret_cast<const float2 *>(&Xs[(32 * i + yr) * LV_XLD + yc]);
+      float2 sm = *reinterpret_cast<const float2 *>(&Ps[0][yr * LV_XLD + yc]);
+#pragma unroll
+      for (int q = 1; q < 4; ++q) {
+        const float2 pq = *reinterpret_cast<const float2 *>(&Ps[q][yr * LV_XLD + yc]);
+        sm.x += pq.x;
+        sm.y += pq.y;
+      }
+      *reinterpret_cast<float2 *>(&Ps[0][yr * LV_XLD + yc]) = float2{b.x - sm.x, b.y - sm.y};
+    }
+    __syncthreads();
+    // X_i = L_ii^-1 Y_i
+    {
+      float o0 = 0.0f, o1 = 0.0f;
+#pragma unroll 8
+      for (int j = 0; j < 32; ++j) {
+        const float lv = Ls[yr * LV_LSLD + j];
+        const float2 y = *reinterpret_cast<const float2 *>(&Ps[0][j * LV_XLD + yc]);
+        o0 = fmaf(lv, y.x, o0);
+        o1 = fmaf(lv, y.y, o1);
+      }
+      *reinterpret_cast<float2 *>(&Xs[(32 * i + yr) * LV_XLD + yc]) = float2{o0, o1};
+    }
+  }
+  __syncthreads();
+  for (int e = t; e < nbk * 32 * (LV_C / 4); e += 256) {
+    const int r = e / (LV_C / 4), c4 = (e % (LV_C / 4)) * 4;
+    if (r >= nt) continue;
+    const float4 v = *reinterpret_cast<const float4 *>(&Xs[r * LV_XLD + c4]);
+    float *dst = B + (long)r * ldb + c0 + c4;
+    const int valid = nrhs - c0 - c4;
+    if (VEC && valid >= 4) {
+      *reinterpret_cast<float4 *>(dst) = v;
+    } else {
+      if (valid > 0) dst[0] = v.x;
+      if (valid > 1) dst[1] = v.y;
+      if (valid > 2) dst[2] = v.z;
+      if (valid > 3) dst[3] = v.w;
+    }
+  }
+}
+
+void trsm_lower_unit_linv(hipStream_t s, int nt, int nrhs, View<const float> L, const float *linv,
+                          View<float> B) {
+  if (nt <= 0 || nrhs <= 0) return;
+  MG_CHECK(nt <= LV_R, "trsm_lower_unit_linv: at most 256 rows");
+  MG_CHECK(L.cs == 1 && B.cs == 1, "trsm_lower_unit_linv: row-major L and B");
+  MG_CHECK((uintptr_t)linv % 16 == 0, "trsm_lower_unit_linv: inverses must be 16-byte aligned");
+  const bool vec = (uintptr_t)L.p % 16 == 0 && L.rs % 4 == 0 && (uintptr_t)B.p % 16 == 0 &&
+                   B.rs % 4 == 0;
+  KScope ks(s, "trsm");
+  const dim3 grid(ceil_div(nrhs, LV_C)), blk(256);
+  if (vec)
+    hipLaunchKernelGGL(trsm_linv_kernel<true>, grid, blk, 0, s, nt, nrhs, L.p, L.rs, linv, B.p, B.rs,
+                       wave_prio(s));
+  else
+    hipLaunchKernelGGL(trsm_linv_kernel<false>, grid, blk, 0, s, nt, nrhs, L.p, L.rs, linv, B.p,
+                       B.rs, wave_prio(s));
+  MG_HIP(hipGetLastError());
+}
+
 template void trsm_left<float>(hipStream_t, bool, bool, int, int, View<const float>, View<float>, Mul);
 template void trsm_left<double>(hipStream_t, bool, bool, int, int, View<const double>, View<double>, Mul);
 

@@ -76,6 +76,8 @@ struct TsluP {
   int G;          // workgroups (one per 256 rows)
   int P;          // leaf workgroups of the tree (one per TS_PR rows; = G here)
   int prio;       // wave priority (wave_prio of the launch stream)
+  float *linv;    // if not null: the inverse of the leaf's unit lower 32 x 32
+                  // block (identity past w), row-major at linv + c0 * 32
 };
 
 // Wavefront max of int keys, in every lane: DPP within each 16-lane row (one
@@ -549,6 +551,24 @@ __global__ __launch_bounds__(TS_NT) void tslu_kernel(TsluP p) {
     }
     if (lane == 0) s_pairs[0] = __popcll(mm);
   }
+  if (b == 0 && wave == 1 && p.linv && lane < TS_W) {
+    // column `lane` of the inverse of the unit lower factored block by
+    // forward substitution, row by row (rows past w: identity), for the
+    // block triangular solves with this leaf's L11 (trsm.hip)
+    float xi[TS_W];
+#pragma unroll
+    for (int i = 0; i < TS_W; ++i) {
+      float v = i == lane ? 1.0f : 0.0f;
+      if (i < w) {
+#pragma unroll
+        for (int k = 0; k < i; ++k) v = fmaf(-Fs[i * TS_RS + k], xi[k], v);
+      }
+      xi[i] = v;
+    }
+    float *dst = p.linv + (long)p.c0 * TS_W + lane;
+#pragma unroll
+    for (int i = 0; i < TS_W; ++i) dst[i * TS_W] = xi[i];
+  }
   __syncthreads();
   if (stays) ts_store_row(p.W + (long)myrow * p.ld + p.c0, x, w, vec);
   if (winrow && windst[t] >= 0) ts_store_row(p.W + (long)windst[t] * p.ld + p.c0, x, w, vec);
@@ -575,7 +595,7 @@ int tslu_max_rows() { return TS_GMAX * TS_NT; }
 // pairs + c0 * PAIR_STRIDE.  The row moves outside the leaf's columns are
 // the caller's (lu_rowswap).
 void tslu_leaf(hipStream_t s, int n, float *W, long ld, int c0, int w, int *ipiv, int *pairs,
-               float eps) {
+               float eps, float *linv) {
   MG_CHECK(w >= 1 && w <= TS_W, "tslu_leaf: leaf width must be in [1, 32]");
   const int m = n - c0;
   MG_CHECK(m >= w && m <= tslu_max_rows(), "tslu_leaf: panel height out of range");
@@ -619,6 +639,7 @@ void tslu_leaf(hipStream_t s, int n, float *W, long ld, int c0, int w, int *ipiv
   p.G = ceil_div(m, TS_NT);
   p.P = ceil_div(m, TS_PR);
   p.prio = wave_prio(s);
+  p.linv = linv;
   KScope ks(s, "panel", 2.0 * m * w * sizeof(float));
   hipLaunchKernelGGL(tslu_kernel, dim3(p.G), dim3(TS_NT), 0, s, p);
   MG_HIP(hipGetLastError());
