@@ -166,6 +166,17 @@ __device__ inline void lower_tile(int id, int tiles_n, int &tm, int &tn) {
   }
 }
 
+// LDS slab buffers of a tile: two (the next slab is stored while this one is
+// read) for the large tiles; one for the 64 x 64 tiles, which the
+// factorizations' critical path runs beside the side stream's GEMMs: the
+// next slab waits in registers anyway, one barrier more per slab keeps a
+// workgroup's LDS at 35 KB instead of 70, so it fits on a compute unit
+// beside two split-BF16 workgroups.  The sums are the same either way.
+template <int BM, int BN>
+constexpr int slab_bufs() {
+  return BM == 64 && BN == 64 ? 1 : 2;
+}
+
 template <typename T, int BM, int BN, int BK, bool A_MN, bool B_MN, bool VEC>
 __device__ __forceinline__ void gemm_tile(const GemmP<T> &p, int tm, int tn, T (*As)[BK * (BM + 16 / sizeof(T))],
                                           T (*Bs)[BK * (BN + 16 / sizeof(T))]);
@@ -180,8 +191,8 @@ template <typename T, int BM, int BN, int BK, bool A_MN, bool B_MN, bool VEC, in
 __global__ __launch_bounds__(256) void gemm_kernel(GemmP<T> p) {
   constexpr int PAD = 16 / (int)sizeof(T);
   constexpr int XCD = 8;
-  __shared__ T As[2][BK * (BM + PAD)];
-  __shared__ T Bs[2][BK * (BN + PAD)];
+  __shared__ T As[slab_bufs<BM, BN>()][BK * (BM + PAD)];
+  __shared__ T Bs[slab_bufs<BM, BN>()][BK * (BN + PAD)];
   if (p.prio) __builtin_amdgcn_s_setprio(3);
   if constexpr (MAP == 0) {
     int tm, tn;
@@ -249,8 +260,9 @@ __device__ __forceinline__ void gemm_tile(const GemmP<T> &p, int tm, int tn, T (
   }
   __syncthreads();
 
+  constexpr int NBUF = slab_bufs<BM, BN>();
   for (int kt = 0; kt < nk; ++kt) {
-    const int cur = kt & 1;
+    const int cur = NBUF == 2 ? (kt & 1) : 0;
     const bool more = kt + 1 < nk;
     if (more) {
       la.load(p.A, p.lda, bm0, (kt + 1) * BK, p.M, p.K);
@@ -272,8 +284,9 @@ __device__ __forceinline__ void gemm_tile(const GemmP<T> &p, int tm, int tn, T (
         for (int b = 0; b < TN; ++b) acc[a][b] = F::mma(af[a], bf[b], acc[a][b]);
     }
     if (more) {
-      la.template store<LDA_S>(As[cur ^ 1]);
-      lb.template store<LDB_S>(Bs[cur ^ 1]);
+      if constexpr (NBUF == 1) __syncthreads();  // every reader of the one buffer is done
+      la.template store<LDA_S>(As[NBUF == 2 ? cur ^ 1 : 0]);
+      lb.template store<LDB_S>(Bs[NBUF == 2 ? cur ^ 1 : 0]);
     }
     __syncthreads();
   }
@@ -433,8 +446,8 @@ __host__ __device__ inline bool fold_row_batched(int rrows, int cols) {
 template <typename T, int BM, int BN, int BK, bool A_MN, bool VEC>
 __global__ __launch_bounds__(256) void fold_rows_kernel(FoldP<T> f) {
   constexpr int PAD = 16 / (int)sizeof(T);
-  __shared__ T As[2][BK * (BM + PAD)];
-  __shared__ T Bs[2][BK * (BN + PAD)];
+  __shared__ T As[slab_bufs<BM, BN>()][BK * (BM + PAD)];
+  __shared__ T Bs[slab_bufs<BM, BN>()][BK * (BN + PAD)];
   const int r = blockIdx.y;
   const int ro = r * f.TB, rrows = min(f.TB, f.n - ro);
   const int c0 = f.lower ? 0 : ro + f.TB;
