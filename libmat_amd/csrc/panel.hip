@@ -1177,7 +1177,7 @@ static void lu_leaf_w(hipStream_t s, int n, T *W, long ld, int c0, int w, int *i
 
 int tslu_max_rows();
 void tslu_leaf(hipStream_t s, int n, float *W, long ld, int c0, int w, int *ipiv, int *pairs,
-               float eps, float *linv);
+               float eps, float *linv, int *perm, int cl, int ch);
 
 int lu_leaf_max(int m, int elem_size) {
   return (elem_size == 8 && m > 64 * PanelNT<double>::value) ? 32 : 64;
@@ -1193,7 +1193,8 @@ void lu_leaf(hipStream_t s, int n, T *W, long ld, int c0, int w, int *ipiv, int 
   bool done = false;
   if constexpr (std::is_same<T, float>::value) {
     if (tournament && w <= 32 && n - c0 <= tslu_max_rows()) {
-      tslu_leaf(s, n, W, ld, c0, w, ipiv, pairs, eps, linv);
+      // the tournament leaf moves the rows of [cl, ch) and perm itself
+      tslu_leaf(s, n, W, ld, c0, w, ipiv, pairs, eps, linv, perm, cl, ch);
       done = true;
     }
   }
@@ -1201,8 +1202,8 @@ void lu_leaf(hipStream_t s, int n, T *W, long ld, int c0, int w, int *ipiv, int 
   if (!done) {
     if (w <= 32) lu_leaf_w<T, 32>(s, n, W, ld, c0, w, ipiv, pairs, eps);
     else lu_leaf_w<T, 64>(s, n, W, ld, c0, w, ipiv, pairs, eps);
+    lu_rowswap<T>(s, W, ld, pairs, perm, c0, w, cl, ch);
   }
-  lu_rowswap<T>(s, W, ld, pairs, perm, c0, w, cl, ch);
 }
 
 template void lu_leaf<float>(hipStream_t, int, float *, long, int, int, int *, int *, int *, float, int,

@@ -78,6 +78,8 @@ struct TsluP {
   int prio;       // wave priority (wave_prio of the launch stream)
   float *linv;    // if not null: the inverse of the leaf's unit lower 32 x 32
                   // block (identity past w), row-major at linv + c0 * 32
+  int *perm;      // if not null: the row-index vector, moved with the rows
+  int cl, ch;     // the leaf's interchanges also reach columns [cl, ch) outside it
 };
 
 // Wavefront max of int keys, in every lane: DPP within each 16-lane row (one
@@ -580,6 +582,35 @@ __global__ __launch_bounds__(TS_NT) void tslu_kernel(TsluP p) {
     }
     // the leaf's row pairs for the later row moves
     for (int e = t; e < PAIR_STRIDE; e += TS_NT) p.pairs[e] = s_pairs[e];
+    // the interchanges reach the other columns [cl, ch) (one thread per
+    // column: all of its reads, then all of its writes) and perm
+    // (pairs past np copy row src[0] onto itself and are stored first, so
+    // the loops run without branches: every load is issued before any store)
+    const int np = s_pairs[0];
+    const bool inside = p.c0 >= p.cl && p.c0 + w <= p.ch;  // the leaf's own columns are skipped
+    const int nother = (p.ch - p.cl) - (inside ? w : 0);
+    const int s0 = s_pairs[1 + PAIR_MAX];
+    unsigned src[2 * TS_W], dstr[2 * TS_W];
+#pragma unroll
+    for (int x = 0; x < 2 * TS_W; ++x) {
+      src[x] = (unsigned)(x < np ? s_pairs[1 + PAIR_MAX + x] : s0);
+      dstr[x] = (unsigned)(x < np ? s_pairs[1 + x] : s0);
+    }
+    const unsigned uld = (unsigned)p.ld;
+    for (int o = t; o < nother; o += TS_NT) {
+      const int c = p.cl + o + ((inside && p.cl + o >= p.c0) ? w : 0);
+      float *Wc = p.W + c;
+      float v[2 * TS_W];
+#pragma unroll
+      for (int x = 0; x < 2 * TS_W; ++x) v[x] = Wc[(unsigned long)src[x] * uld];
+#pragma unroll
+      for (int x = 2 * TS_W - 1; x >= 0; --x) Wc[(unsigned long)dstr[x] * uld] = v[x];
+    }
+    if (p.perm) {
+      const int pv = t < np ? p.perm[s_pairs[1 + PAIR_MAX + t]] : 0;
+      __syncthreads();
+      if (t < np) p.perm[s_pairs[1 + t]] = pv;
+    }
   }
 }
 
@@ -595,7 +626,7 @@ int tslu_max_rows() { return TS_GMAX * TS_NT; }
 // pairs + c0 * PAIR_STRIDE.  The row moves outside the leaf's columns are
 // the caller's (lu_rowswap).
 void tslu_leaf(hipStream_t s, int n, float *W, long ld, int c0, int w, int *ipiv, int *pairs,
-               float eps, float *linv) {
+               float eps, float *linv, int *perm, int cl, int ch) {
   MG_CHECK(w >= 1 && w <= TS_W, "tslu_leaf: leaf width must be in [1, 32]");
   const int m = n - c0;
   MG_CHECK(m >= w && m <= tslu_max_rows(), "tslu_leaf: panel height out of range");
@@ -640,6 +671,9 @@ void tslu_leaf(hipStream_t s, int n, float *W, long ld, int c0, int w, int *ipiv
   p.P = ceil_div(m, TS_PR);
   p.prio = wave_prio(s);
   p.linv = linv;
+  p.perm = perm;
+  p.cl = cl;
+  p.ch = ch;
   KScope ks(s, "panel", 2.0 * m * w * sizeof(float));
   hipLaunchKernelGGL(tslu_kernel, dim3(p.G), dim3(TS_NT), 0, s, p);
   MG_HIP(hipGetLastError());
