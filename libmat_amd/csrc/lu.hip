@@ -23,7 +23,7 @@ namespace mg {
 
 template <typename T>
 void lu_leaf(hipStream_t s, int n, T *W, long ld, int c0, int w, int *ipiv, int *perm, int *pairs,
-             T eps, int cl, int ch, bool tournament, T *linv);
+             T eps, int cl, int ch, bool tournament, T *linv, int prev_c0 = -1);
 int tslu_max_rows();  // tallest panel of the tournament leaf (tslu.hip)
 template <typename T>
 void lu_rowswap(hipStream_t s, T *W, long ld, const int *pairs, int *perm, int k0, int w, int cl,
@@ -92,6 +92,13 @@ static void getrf_rec(hipStream_t s, int n, T *W, long ld, int c0, int w, int *i
   int w1 = ((w / 2 + leaf - 1) / leaf) * leaf;
   if (w1 >= w) w1 = w - leaf;
   const int w2 = w - w1;
+  if (linv && leaf == 32 && w1 == leaf) {
+    // a node of two 32-column tournament leaves: the second one also solves
+    // U12 = L11^-1 A12 and updates its own rows (A22 -= L21 U12) in its launch
+    lu_leaf<T>(s, n, W, ld, c0, w1, ipiv, perm, pairs, eps, cl, ch, o.tournament, linv);
+    lu_leaf<T>(s, n, W, ld, c0 + w1, w2, ipiv, perm, pairs, eps, cl, ch, o.tournament, linv, c0);
+    return;
+  }
   getrf_rec<T>(s, n, W, ld, c0, w1, ipiv, perm, pairs, eps, o, leaf, cl, ch, linv);
   // U12 = L11^-1 A12 (unit lower)
   View<const T> L11{W + (long)c0 * ld + c0, ld, 1};

@@ -1177,7 +1177,7 @@ static void lu_leaf_w(hipStream_t s, int n, T *W, long ld, int c0, int w, int *i
 
 int tslu_max_rows();
 void tslu_leaf(hipStream_t s, int n, float *W, long ld, int c0, int w, int *ipiv, int *pairs,
-               float eps, float *linv, int *perm, int cl, int ch);
+               float eps, float *linv, int *perm, int cl, int ch, int prev_c0);
 
 int lu_leaf_max(int m, int elem_size) {
   return (elem_size == 8 && m > 64 * PanelNT<double>::value) ? 32 : 64;
@@ -1188,17 +1188,18 @@ int lu_leaf_max(int m, int elem_size) {
 // (and to perm).  Leaves up to 32 columns use the 32-column kernel.
 template <typename T>
 void lu_leaf(hipStream_t s, int n, T *W, long ld, int c0, int w, int *ipiv, int *perm, int *pairs,
-             T eps, int cl, int ch, bool tournament, T *linv) {
+             T eps, int cl, int ch, bool tournament, T *linv, int prev_c0) {
   MG_CHECK(w >= 1 && w <= 64, "lu_leaf: leaf width must be in [1, 64]");
   bool done = false;
   if constexpr (std::is_same<T, float>::value) {
     if (tournament && w <= 32 && n - c0 <= tslu_max_rows()) {
       // the tournament leaf moves the rows of [cl, ch) and perm itself
-      tslu_leaf(s, n, W, ld, c0, w, ipiv, pairs, eps, linv, perm, cl, ch);
+      tslu_leaf(s, n, W, ld, c0, w, ipiv, pairs, eps, linv, perm, cl, ch, prev_c0);
       done = true;
     }
   }
-  MG_CHECK(done || linv == nullptr, "lu_leaf: diagonal-block inverses need the tournament leaf");
+  MG_CHECK(done || (linv == nullptr && prev_c0 < 0),
+           "lu_leaf: diagonal-block inverses and fused updates need the tournament leaf");
   if (!done) {
     if (w <= 32) lu_leaf_w<T, 32>(s, n, W, ld, c0, w, ipiv, pairs, eps);
     else lu_leaf_w<T, 64>(s, n, W, ld, c0, w, ipiv, pairs, eps);
@@ -1207,9 +1208,9 @@ void lu_leaf(hipStream_t s, int n, T *W, long ld, int c0, int w, int *ipiv, int 
 }
 
 template void lu_leaf<float>(hipStream_t, int, float *, long, int, int, int *, int *, int *, float, int,
-                             int, bool, float *);
+                             int, bool, float *, int);
 template void lu_leaf<double>(hipStream_t, int, double *, long, int, int, int *, int *, int *, double,
-                              int, int, bool, double *);
+                              int, int, bool, double *, int);
 template void lu_rowswap<float>(hipStream_t, float *, long, const int *, int *, int, int, int, int);
 template void lu_rowswap<double>(hipStream_t, double *, long, const int *, int *, int, int, int, int);
 template void lu_rowswap_block<float>(hipStream_t, float *, long, const int *, int, int, int, int,

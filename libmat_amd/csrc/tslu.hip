@@ -80,6 +80,10 @@ struct TsluP {
                   // block (identity past w), row-major at linv + c0 * 32
   int *perm;      // if not null: the row-index vector, moved with the rows
   int cl, ch;     // the leaf's interchanges also reach columns [cl, ch) outside it
+  // second leaf of a 64-column node (prev_c0 >= 0): the node's first leaf
+  // starts at prev_c0 (its inverse at linv + prev_c0 * 32); this launch also
+  // does the node's U12 = L11^-1 A12 and A22 -= L21 U12 for its columns
+  int prev_c0;
 };
 
 // Wavefront max of int keys, in every lane: DPP within each 16-lane row (one
@@ -361,34 +365,92 @@ __global__ __launch_bounds__(TS_NT) void tslu_kernel(TsluP p) {
   __shared__ int s_go;  // this workgroup plays the next tree level
   __shared__ int s_ok;  // the final round arrived (no time-out)
 
-  // 1. own rows, original values, for the substitution
+  // 0. second leaf of a 64-column node: U12 = L11^-1 A12 of the node's
+  //    first leaf (A12 = its window rows in this leaf's columns; the order of
+  //    the block solve's X_0 = L_00^-1 B_0), kept in registers for the
+  //    write-back and in LDS (the final pivot rows' place, unused until
+  //    then) for the update of the own rows below
+  const bool fused = p.prev_c0 >= 0;
+  float u4[4] = {0.0f, 0.0f, 0.0f, 0.0f};  // U12 row t / 8, columns 4 (t % 8) ..
+  float *U12s = orig + TS_NT * TS_RS;
+  if (fused) {
+    const float *lprev = p.linv + (long)p.prev_c0 * TS_W;
+    for (int e = t; e < TS_W * TS_W; e += TS_NT) {
+      const int i = e / TS_W, c = e % TS_W;
+      un[i * TS_RS + c] = lprev[e];
+      U12s[i * TS_RS + c] = c < w ? p.W[(long)(p.prev_c0 + i) * p.ld + p.c0 + c] : 0.0f;
+    }
+    __syncthreads();
+    const int i = t >> 3, cq = 4 * (t & 7);
+#pragma unroll 8
+    for (int j = 0; j < TS_W; ++j) {
+      const float lv = un[i * TS_RS + j];
+      const float4 a = *reinterpret_cast<const float4 *>(U12s + j * TS_RS + cq);
+      u4[0] = fmaf(lv, a.x, u4[0]);
+      u4[1] = fmaf(lv, a.y, u4[1]);
+      u4[2] = fmaf(lv, a.z, u4[2]);
+      u4[3] = fmaf(lv, a.w, u4[3]);
+    }
+    __syncthreads();
+    *reinterpret_cast<float4 *>(U12s + i * TS_RS + cq) = make_float4(u4[0], u4[1], u4[2], u4[3]);
+    __syncthreads();
+  }
+
+  // 1. own rows, original values, for the substitution (in a fused launch
+  //    after A22 -= L21 U12, written back for the tree's reads)
   {
     float x[TS_W];
     if (mine) ts_load_row(x, p.W + (long)myrow * p.ld + p.c0, w, vec);
     else
 #pragma unroll
       for (int c = 0; c < TS_W; ++c) x[c] = 0.0f;
+    if (fused && mine) {
+      float lp[TS_W], acc[TS_W];
+      ts_load_row(lp, p.W + (long)myrow * p.ld + p.prev_c0, TS_W, vec);
+#pragma unroll
+      for (int c = 0; c < TS_W; ++c) acc[c] = 0.0f;
+#pragma unroll
+      for (int j = 0; j < TS_W; ++j) {
+#pragma unroll
+        for (int c4 = 0; c4 < TS_W / 4; ++c4) {
+          const float4 u = *reinterpret_cast<const float4 *>(U12s + j * TS_RS + 4 * c4);
+          acc[4 * c4] = fmaf(lp[j], u.x, acc[4 * c4]);
+          acc[4 * c4 + 1] = fmaf(lp[j], u.y, acc[4 * c4 + 1]);
+          acc[4 * c4 + 2] = fmaf(lp[j], u.z, acc[4 * c4 + 2]);
+          acc[4 * c4 + 3] = fmaf(lp[j], u.w, acc[4 * c4 + 3]);
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < TS_W; ++c) x[c] = c < w ? x[c] - acc[c] : 0.0f;
+      ts_store_row(p.W + (long)myrow * p.ld + p.c0, x, w, vec);
+    }
 #pragma unroll
     for (int c4 = 0; c4 < TS_W / 4; ++c4)
       *reinterpret_cast<float4 *>(orig + t * TS_RS + 4 * c4) =
           make_float4(x[4 * c4], x[4 * c4 + 1], x[4 * c4 + 2], x[4 * c4 + 3]);
     skip[t] = 0;
   }
+  // the own rows (original, or updated) are in LDS for the leaf playoff
+  __syncthreads();
 
   // 2. the tree: the leaf workgroups play their rows, publish, and carry on as
   //    the next level's node when last to arrive (one playoff site, so its
   //    code is generated once)
   if (b < p.P) {
     TsRows r;
+    static_assert(TS_RPT == 1, "the leaf playoff reads its rows from the LDS copy");
+    {
+      const int row = p.r0 + b * TS_PR + t;  // = myrow
+      r.live[0] = row < rend;
+      cid[t] = r.live[0] ? row : -1;
 #pragma unroll
-    for (int q = 0; q < TS_RPT; ++q) {
-      const int row = p.r0 + b * TS_PR + q * TS_NT + t;
-      r.live[q] = row < rend;
-      cid[q * TS_NT + t] = r.live[q] ? row : -1;
-      if (r.live[q]) ts_load_row(r.a[q], p.W + (long)row * p.ld + p.c0, w, vec);
-      else
-#pragma unroll
-        for (int c = 0; c < TS_W; ++c) r.a[q][c] = 0.0f;
+      for (int c4 = 0; c4 < TS_W / 4; ++c4) {
+        const float4 q = *reinterpret_cast<const float4 *>(orig + t * TS_RS + 4 * c4);
+        r.a[0][4 * c4] = q.x;
+        r.a[0][4 * c4 + 1] = q.y;
+        r.a[0][4 * c4 + 2] = q.z;
+        r.a[0][4 * c4 + 3] = q.w;
+      }
     }
     int level = 0, node = b, nodes = p.P;
     while (true) {
@@ -471,6 +533,14 @@ __global__ __launch_bounds__(TS_NT) void tslu_kernel(TsluP p) {
   __syncthreads();
   if (!s_ok) return;
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  if (fused && b == 0) {
+    // U12 into the node's first leaf rows: every workgroup read A12 before
+    // its leaf playoff, and the final round followed every leaf playoff
+    const int i = t >> 3, cq = 4 * (t & 7);
+#pragma unroll
+    for (int cc = 0; cc < 4; ++cc)
+      if (cq + cc < w) p.W[(long)(p.prev_c0 + i) * p.ld + p.c0 + cq + cc] = u4[cc];
+  }
   // the final pivot rows (factored) and their indices, once from memory
   float *Fs = orig + TS_NT * TS_RS;  // [TS_W][TS_RS]
   for (int e = t; e < TS_W * TS_W; e += TS_NT) Fs[(e / TS_W) * TS_RS + e % TS_W] = ts_getf(p.F + e);
@@ -626,7 +696,9 @@ int tslu_max_rows() { return TS_GMAX * TS_NT; }
 // pairs + c0 * PAIR_STRIDE.  The row moves outside the leaf's columns are
 // the caller's (lu_rowswap).
 void tslu_leaf(hipStream_t s, int n, float *W, long ld, int c0, int w, int *ipiv, int *pairs,
-               float eps, float *linv, int *perm, int cl, int ch) {
+               float eps, float *linv, int *perm, int cl, int ch, int prev_c0) {
+  MG_CHECK(prev_c0 < 0 || (linv && prev_c0 + TS_W == c0),
+           "tslu_leaf: a fused leaf follows its node's first leaf, with the inverses");
   MG_CHECK(w >= 1 && w <= TS_W, "tslu_leaf: leaf width must be in [1, 32]");
   const int m = n - c0;
   MG_CHECK(m >= w && m <= tslu_max_rows(), "tslu_leaf: panel height out of range");
@@ -674,6 +746,7 @@ void tslu_leaf(hipStream_t s, int n, float *W, long ld, int c0, int w, int *ipiv
   p.perm = perm;
   p.cl = cl;
   p.ch = ch;
+  p.prev_c0 = prev_c0;
   KScope ks(s, "panel", 2.0 * m * w * sizeof(float));
   hipLaunchKernelGGL(tslu_kernel, dim3(p.G), dim3(TS_NT), 0, s, p);
   MG_HIP(hipGetLastError());
