@@ -264,17 +264,20 @@ void gemm_bf16x_kernel(LpP p) {
   // on their compute units, simply take fewer.  The counter wraps to zero
   // on its last claim (every workgroup ends on exactly one claim past the
   // share), ready for the next launch.
-  __shared__ int s_claim;
+  // (the claim goes to the lanes through the slab buffer, idle between
+  // tiles: no LDS beyond the 48 KB of slabs)
+  int *s_claim = reinterpret_cast<int *>(&sm[0][0][0]);
   const int xcd = blockIdx.x % 8, per = p.vgrid / 8;
   for (int vid = blockIdx.x, turn = 0;; vid += gridDim.x, ++turn) {
   int tm, tn;
   if (p.ctr) {
-    __syncthreads();  // every lane has read the previous claim
+    __syncthreads();  // every lane is done with the slab buffer (and the previous claim)
     if (t == 0)
-      s_claim = (int)__builtin_amdgcn_atomic_inc32(p.ctr + xcd, (unsigned)(per + gridDim.x / 8 - 1),
-                                                   __ATOMIC_RELAXED, "agent");
+      *s_claim = (int)__builtin_amdgcn_atomic_inc32(p.ctr + xcd, (unsigned)(per + gridDim.x / 8 - 1),
+                                                    __ATOMIC_RELAXED, "agent");
     __syncthreads();
-    const int j = s_claim;
+    const int j = *s_claim;
+    __syncthreads();  // read before the tile's first slab lands in the buffer
     if (j >= per) break;
     const int id = xcd * per + j;
     if (id >= p.tiles) continue;

@@ -358,7 +358,7 @@ __global__ __launch_bounds__(256) void trsm_linv_kernel(int nt, int nrhs, const 
                                                         int prio) {
   if (prio) __builtin_amdgcn_s_setprio(3);
   __shared__ __align__(16) float Xs[LV_R * LV_XLD];
-  __shared__ __align__(16) float Ps[4][32 * LV_XLD];
+  __shared__ __align__(16) float Ps[3][32 * LV_XLD];  // partials of wavefronts 1..3, then Y
   __shared__ __align__(16) float LB[32 * LV_LBLD];  // block row i of L
   __shared__ __align__(16) float Ls[32 * LV_LSLD];  // L_ii^-1
   const int t = threadIdx.x, wv = t >> 6, l = t & 63, rg = l >> 2, cg = l & 3;
@@ -432,27 +432,37 @@ __global__ __launch_bounds__(256) void trsm_linv_kernel(int nt, int nrhs, const 
         acc[1][3] = fmaf(a1, xv.w, acc[1][3]);
       }
     }
+    if (wv > 0) {
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
-      *reinterpret_cast<float4 *>(&Ps[wv][(2 * rg + a) * LV_XLD + 4 * cg]) =
-          float4{acc[a][0], acc[a][1], acc[a][2], acc[a][3]};
+      for (int a = 0; a < 2; ++a)
+        *reinterpret_cast<float4 *>(&Ps[wv - 1][(2 * rg + a) * LV_XLD + 4 * cg]) =
+            float4{acc[a][0], acc[a][1], acc[a][2], acc[a][3]};
+    }
     __syncthreads();
     // every reader of LB is done: block row i + 1 goes in, i + 2 is loaded
     if (i + 1 < nbk) {
       commit_l(i + 1);
       if (i + 2 < nbk) fetch_l(i + 2);
     }
-    // Y_i = B_i - (partials in wavefront order), into Ps[0]
-    {
-      const float2 b = *reinterpret_cast<const float2 *>(&Xs[(32 * i + yr) * LV_XLD + yc]);
-      float2 sm = *reinterpret_cast<const float2 *>(&Ps[0][yr * LV_XLD + yc]);
+    // Y_i = B_i - (partials in wavefront order), by wavefront 0 from its own
+    // partial in registers, into Ps[0]
+    if (wv == 0) {
 #pragma unroll
-      for (int q = 1; q < 4; ++q) {
-        const float2 pq = *reinterpret_cast<const float2 *>(&Ps[q][yr * LV_XLD + yc]);
-        sm.x += pq.x;
-        sm.y += pq.y;
+      for (int a = 0; a < 2; ++a) {
+        const int row = 2 * rg + a;
+        const float4 b = *reinterpret_cast<const float4 *>(&Xs[(32 * i + row) * LV_XLD + 4 * cg]);
+        float4 sm = float4{acc[a][0], acc[a][1], acc[a][2], acc[a][3]};
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          const float4 pq = *reinterpret_cast<const float4 *>(&Ps[q][row * LV_XLD + 4 * cg]);
+          sm.x += pq.x;
+          sm.y += pq.y;
+          sm.z += pq.z;
+          sm.w += pq.w;
+        }
+        *reinterpret_cast<float4 *>(&Ps[0][row * LV_XLD + 4 * cg]) =
+            float4{b.x - sm.x, b.y - sm.y, b.z - sm.z, b.w - sm.w};
       }
-      *reinterpret_cast<float2 *>(&Ps[0][yr * LV_XLD + yc]) = float2{b.x - sm.x, b.y - sm.y};
     }
     __syncthreads();
     // X_i = L_ii^-1 Y_i
