@@ -26,8 +26,15 @@
 //      the elimination: multiply by 1/pivot, then the rank-1 updates, and
 //      no scaling where |pivot| < eps); workgroup 0 writes [L11 \ U11] into
 //      the top w rows and the rows those displaced into the pivots' old
-//      places, and turns the pivot order into libmat's sequential
-//      interchanges (ipiv) and the leaf's row pairs.
+//      places, turns the pivot order into libmat's sequential interchanges
+//      (ipiv) and the leaf's row pairs, applies them to the other columns of
+//      the block ([cl, ch)) and to perm, and writes the inverse of the unit
+//      lower 32 x 32 block for the block triangular solves (trsm.hip).
+// The second leaf of a 64-column node also does the node's update first:
+// U12 = L11^-1 A12 from the first leaf's inverse, then every workgroup its
+// rows' A22 -= L21 U12, written back before its leaf playoff (the tree
+// nodes read the candidates' rows from memory after the arrival ticket,
+// which the release fence before it orders after those writes).
 // A playoff runs on all four SIMDs of a compute unit: every step, each
 // wavefront finds its candidate (DPP max + ballot) and parks the
 // candidate's row in LDS, one barrier, every wavefront takes the best of the
@@ -36,8 +43,9 @@
 // per four or more cycles, so 64 rows there cost about half as much per step
 // as 256 rows here).  Rows per lane are a parameter (TS_RPT): four rows per
 // lane took 2200 cycles a step, more than the level they save.
-// Every playoff reads original row values (the matrix is written only after
-// the final round), so no workgroup overwrites a row another one may read.
+// Every playoff reads the rows' values as they were when the launch began
+// (after the fused update): the leaf's columns are written only after the
+// final round, so no workgroup overwrites a row another one may still read.
 #include <algorithm>
 #include <atomic>
 #include <cstdint>
@@ -693,8 +701,11 @@ int tslu_max_rows() { return TS_GMAX * TS_NT; }
 // Tournament-pivoted factorization of the leaf W[c0:n, c0:c0+w] (w <= 32,
 // n - c0 <= tslu_max_rows()): writes ipiv[c0 .. c0+w) (sequential
 // interchanges, as libmat's) and the leaf's row pairs at
-// pairs + c0 * PAIR_STRIDE.  The row moves outside the leaf's columns are
-// the caller's (lu_rowswap).
+// pairs + c0 * PAIR_STRIDE, and applies the interchanges to the columns
+// [cl, ch) outside the leaf and to perm (if not null).  linv (if not null)
+// receives the inverse of the leaf's unit lower block at linv + c0 * 32;
+// prev_c0 >= 0 makes it the second leaf of a 64-column node (see the
+// kernel's comment), which needs linv.
 void tslu_leaf(hipStream_t s, int n, float *W, long ld, int c0, int w, int *ipiv, int *pairs,
                float eps, float *linv, int *perm, int cl, int ch, int prev_c0) {
   MG_CHECK(prev_c0 < 0 || (linv && prev_c0 + TS_W == c0),
