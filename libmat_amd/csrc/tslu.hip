@@ -549,6 +549,13 @@ __global__ __launch_bounds__(TS_NT) void tslu_kernel(TsluP p) {
     for (int cc = 0; cc < 4; ++cc)
       if (cq + cc < w) p.W[(long)(p.prev_c0 + i) * p.ld + p.c0 + cq + cc] = u4[cc];
   }
+  // The tail is spread over the workgroups so that no one of them carries
+  // all of it: workgroup 0 writes the window, ipiv and the row pairs; the
+  // last workgroup (mwg) applies the interchanges to the other columns and
+  // perm; the one before it (lwg) forms the inverse of L11.  Each of them
+  // also computes its own rows of L21.  (One workgroup: all of it.)
+  const int mwg = p.G - 1;
+  const int lwg = p.G >= 3 ? p.G - 2 : p.G - 1;
   // the final pivot rows (factored) and their indices, once from memory
   float *Fs = orig + TS_NT * TS_RS;  // [TS_W][TS_RS]
   for (int e = t; e < TS_W * TS_W; e += TS_NT) Fs[(e / TS_W) * TS_RS + e % TS_W] = ts_getf(p.F + e);
@@ -557,44 +564,7 @@ __global__ __launch_bounds__(TS_NT) void tslu_kernel(TsluP p) {
     windst[t] = -1;
   }
   __syncthreads();
-  // prepared for the substitution (see ts_sub; entries past a row's end are
-  // read but never used)
-  for (int e = t; e < TS_W * TS_W; e += TS_NT) {
-    const int J = e / TS_W, c = e % TS_W;
-    if (c >= J) {
-      const float pjj = Fs[J * TS_RS + J];
-      const bool skipj = !(J < w) || fabsf(pjj) < p.eps;
-      float v = 0.0f;
-      if (c == J) v = skipj ? 1.0f : ts_recip(pjj);
-      else if (c < w && !skipj) v = Fs[J * TS_RS + c];
-      un[J * TS_RS + (c - J)] = v;
-    }
-  }
-  if (t < w) {
-    const int row = fin[t];
-    if (row >= base && row < base + TS_NT) skip[row - base] = 1;  // moves into the window
-  }
-  __syncthreads();
-
-  // 4. L21 = A21 U11^-1 from the original values, for the own rows below
-  //    the window that stay, and in workgroup 0 for the window rows, which
-  //    either move into the window as pivot rows or to a pivot row's old
-  //    place
-  const bool winrow = b == 0 && t < w;
-  const bool stays = mine && !skip[t] && myrow >= p.r0 + w;
-  float x[TS_W];
-  if (stays || winrow) {
-#pragma unroll
-    for (int c4 = 0; c4 < TS_W / 4; ++c4) {
-      const float4 q = *reinterpret_cast<const float4 *>(orig + t * TS_RS + 4 * c4);
-      x[4 * c4] = q.x;
-      x[4 * c4 + 1] = q.y;
-      x[4 * c4 + 2] = q.z;
-      x[4 * c4 + 3] = q.w;
-    }
-    ts_subs(x, un, std::make_integer_sequence<int, TS_W>{});
-  }
-  if (b == 0 && wave == 0) {
+  if ((b == 0 || b == mwg) && wave == 0) {
     // libmat's sequential interchanges: step kk swaps row c0+kk with the
     // current position of pivot row kk.  Lane i < w follows window position
     // c0+i, lane 32+j the original place of pivot row j when that lies below
@@ -617,7 +587,7 @@ __global__ __launch_bounds__(TS_NT) void tslu_kernel(TsluP p) {
       mypiv = lane == kk ? psl : mypiv;
       occ = lane == kk ? osl : (lane == sl ? okk : occ);
     }
-    if (lane < w) p.ipiv[p.c0 + lane] = mypiv;
+    if (b == 0 && lane < w) p.ipiv[p.c0 + lane] = mypiv;
     // the row pairs (dst receives original row src) and where each window
     // row goes: a moved window row below the window is displaced, one inside
     // it is a pivot row (no L21 row)
@@ -631,40 +601,34 @@ __global__ __launch_bounds__(TS_NT) void tslu_kernel(TsluP p) {
     }
     if (lane == 0) s_pairs[0] = __popcll(mm);
   }
-  if (b == 0 && wave == 1 && p.linv && lane < TS_W) {
-    // column `lane` of the inverse of the unit lower factored block by
-    // forward substitution, row by row (rows past w: identity), for the
-    // block triangular solves with this leaf's L11 (trsm.hip)
-    float xi[TS_W];
-#pragma unroll
-    for (int i = 0; i < TS_W; ++i) {
-      float v = i == lane ? 1.0f : 0.0f;
-      if (i < w) {
-#pragma unroll
-        for (int k = 0; k < i; ++k) v = fmaf(-Fs[i * TS_RS + k], xi[k], v);
-      }
-      xi[i] = v;
+  // prepared for the substitution (see ts_sub; entries past a row's end are
+  // read but never used)
+  for (int e = t; e < TS_W * TS_W; e += TS_NT) {
+    const int J = e / TS_W, c = e % TS_W;
+    if (c >= J) {
+      const float pjj = Fs[J * TS_RS + J];
+      const bool skipj = !(J < w) || fabsf(pjj) < p.eps;
+      float v = 0.0f;
+      if (c == J) v = skipj ? 1.0f : ts_recip(pjj);
+      else if (c < w && !skipj) v = Fs[J * TS_RS + c];
+      un[J * TS_RS + (c - J)] = v;
     }
-    float *dst = p.linv + (long)p.c0 * TS_W + lane;
-#pragma unroll
-    for (int i = 0; i < TS_W; ++i) dst[i * TS_W] = xi[i];
+  }
+  if (t < w) {
+    const int row = fin[t];
+    if (row >= base && row < base + TS_NT) skip[row - base] = 1;  // moves into the window
   }
   __syncthreads();
-  if (stays) ts_store_row(p.W + (long)myrow * p.ld + p.c0, x, w, vec);
-  if (winrow && windst[t] >= 0) ts_store_row(p.W + (long)windst[t] * p.ld + p.c0, x, w, vec);
-  if (b == 0) {
-    // the window gets the factored pivot rows
-    for (int e = t; e < w * TS_W; e += TS_NT) {
-      const int J = e / TS_W, c = e % TS_W;
-      if (c < w) p.W[(long)(p.r0 + J) * p.ld + p.c0 + c] = Fs[J * TS_RS + c];
-    }
-    // the leaf's row pairs for the later row moves
-    for (int e = t; e < PAIR_STRIDE; e += TS_NT) p.pairs[e] = s_pairs[e];
+
+  if (b == mwg) {
     // the interchanges reach the other columns [cl, ch) (one thread per
-    // column: all of its reads, then all of its writes) and perm
-    // (pairs past np copy row src[0] onto itself and are stored first, so
-    // the loops run without branches: every load is issued before any store)
+    // column: all of its reads, then all of its writes) and perm (read
+    // before, written after the columns).  Pairs past np copy row src[0]
+    // onto itself and are stored first, so the loops run without branches:
+    // every load is issued before any store.  The columns outside the leaf
+    // are not touched by anything else in this launch.
     const int np = s_pairs[0];
+    const int pv = (p.perm && t < np) ? p.perm[s_pairs[1 + PAIR_MAX + t]] : 0;
     const bool inside = p.c0 >= p.cl && p.c0 + w <= p.ch;  // the leaf's own columns are skipped
     const int nother = (p.ch - p.cl) - (inside ? w : 0);
     const int s0 = s_pairs[1 + PAIR_MAX];
@@ -685,10 +649,60 @@ __global__ __launch_bounds__(TS_NT) void tslu_kernel(TsluP p) {
       for (int x = 2 * TS_W - 1; x >= 0; --x) Wc[(unsigned long)dstr[x] * uld] = v[x];
     }
     if (p.perm) {
-      const int pv = t < np ? p.perm[s_pairs[1 + PAIR_MAX + t]] : 0;
       __syncthreads();
       if (t < np) p.perm[s_pairs[1 + t]] = pv;
     }
+  }
+  if (b == lwg && wave == 1 && p.linv && lane < TS_W) {
+    // column `lane` of the inverse of the unit lower factored block by
+    // forward substitution (rows past w: identity), for the block triangular
+    // solves with this leaf's L11 (trsm.hip).  Column-oriented: once x_i is
+    // final it updates the rows below it, so row j still takes its terms in
+    // the order k = 0 .. j-1, and the steps are independent FMAs
+    float xi[TS_W];
+#pragma unroll
+    for (int i = 0; i < TS_W; ++i) xi[i] = i == lane ? 1.0f : 0.0f;
+#pragma unroll
+    for (int i = 0; i < TS_W - 1; ++i) {
+#pragma unroll
+      for (int j = i + 1; j < TS_W; ++j) {
+        const float f = j < w ? Fs[j * TS_RS + i] : 0.0f;
+        xi[j] = j < w ? fmaf(-f, xi[i], xi[j]) : xi[j];
+      }
+    }
+    float *dst = p.linv + (long)p.c0 * TS_W + lane;
+#pragma unroll
+    for (int i = 0; i < TS_W; ++i) dst[i * TS_W] = xi[i];
+  }
+
+  // 4. L21 = A21 U11^-1 from the original values, for the own rows below
+  //    the window that stay, and in workgroup 0 for the window rows, which
+  //    either move into the window as pivot rows or to a pivot row's old
+  //    place
+  const bool winrow = b == 0 && t < w;
+  const bool stays = mine && !skip[t] && myrow >= p.r0 + w;
+  if (stays || winrow) {
+    float x[TS_W];
+#pragma unroll
+    for (int c4 = 0; c4 < TS_W / 4; ++c4) {
+      const float4 q = *reinterpret_cast<const float4 *>(orig + t * TS_RS + 4 * c4);
+      x[4 * c4] = q.x;
+      x[4 * c4 + 1] = q.y;
+      x[4 * c4 + 2] = q.z;
+      x[4 * c4 + 3] = q.w;
+    }
+    ts_subs(x, un, std::make_integer_sequence<int, TS_W>{});
+    if (stays) ts_store_row(p.W + (long)myrow * p.ld + p.c0, x, w, vec);
+    if (winrow && windst[t] >= 0) ts_store_row(p.W + (long)windst[t] * p.ld + p.c0, x, w, vec);
+  }
+  if (b == 0) {
+    // the window gets the factored pivot rows
+    for (int e = t; e < w * TS_W; e += TS_NT) {
+      const int J = e / TS_W, c = e % TS_W;
+      if (c < w) p.W[(long)(p.r0 + J) * p.ld + p.c0 + c] = Fs[J * TS_RS + c];
+    }
+    // the leaf's row pairs for the later row moves
+    for (int e = t; e < PAIR_STRIDE; e += TS_NT) p.pairs[e] = s_pairs[e];
   }
 }
 
