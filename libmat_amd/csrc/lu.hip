@@ -52,12 +52,16 @@ static void lu_gemm_sub(hipStream_t s, long M, long N, long K, View<const T> A, 
 // Workgroups of the side stream's split-BF16 trailing updates per compute
 // unit (MATGPU_SIDE_WG_PER_CU, a fraction allowed, 0 = uncapped): capped, so
 // that the kernels of the critical path (panels, small solves and products)
-// find free slots instead of waiting for GEMM tiles to finish; a workgroup
-// of the split GEMM takes 48 KB of LDS, three fit on a unit.
+// find free slots instead of waiting for GEMM tiles to finish.  A workgroup
+// of the split GEMM takes 48 KB of LDS and 128 VGPRs per lane on each SIMD;
+// two of them leave a unit 64 KB of LDS and 256 VGPRs per SIMD, which every
+// critical-path kernel of the LU fits in (the tournament leaf 49 KB and 164
+// VGPRs, the block-inverse solve 62 KB); a third leaves too little LDS for
+// the block-inverse solve.
 static int side_grid_cap() {
   static const double per_cu = [] {
     const char *e = std::getenv("MATGPU_SIDE_WG_PER_CU");
-    return e ? std::max(0.0, std::atof(e)) : 1.5;
+    return e ? std::max(0.0, std::atof(e)) : 2.0;
   }();
   return (int)(per_cu * num_cus());
 }

@@ -369,6 +369,7 @@ __global__ __launch_bounds__(TS_NT) void tslu_kernel(TsluP p) {
   __shared__ int fin[TS_W];      // pivot rows, in order
   __shared__ int windst[TS_W];   // where each window row goes (-1: into the window)
   __shared__ int s_pairs[PAIR_STRIDE];
+  __shared__ unsigned mv_row[2][2 * TS_W];  // row moves: padded src, dst rows
   __shared__ unsigned char skip[TS_NT];
   __shared__ int s_go;  // this workgroup plays the next tree level
   __shared__ int s_ok;  // the final round arrived (no time-out)
@@ -627,26 +628,28 @@ __global__ __launch_bounds__(TS_NT) void tslu_kernel(TsluP p) {
     // onto itself and are stored first, so the loops run without branches:
     // every load is issued before any store.  The columns outside the leaf
     // are not touched by anything else in this launch.
+    // (The padded rows sit in LDS and are read where they are used: held
+    // in registers they lifted the kernel past 256 VGPRs, which kept it off
+    // compute units running two split-GEMM workgroups.)
     const int np = s_pairs[0];
     const int pv = (p.perm && t < np) ? p.perm[s_pairs[1 + PAIR_MAX + t]] : 0;
     const bool inside = p.c0 >= p.cl && p.c0 + w <= p.ch;  // the leaf's own columns are skipped
     const int nother = (p.ch - p.cl) - (inside ? w : 0);
-    const int s0 = s_pairs[1 + PAIR_MAX];
-    unsigned src[2 * TS_W], dstr[2 * TS_W];
-#pragma unroll
-    for (int x = 0; x < 2 * TS_W; ++x) {
-      src[x] = (unsigned)(x < np ? s_pairs[1 + PAIR_MAX + x] : s0);
-      dstr[x] = (unsigned)(x < np ? s_pairs[1 + x] : s0);
+    if (t < 2 * TS_W) {
+      const int s0 = s_pairs[1 + PAIR_MAX];
+      mv_row[0][t] = (unsigned)(t < np ? s_pairs[1 + PAIR_MAX + t] : s0);
+      mv_row[1][t] = (unsigned)(t < np ? s_pairs[1 + t] : s0);
     }
-    const unsigned uld = (unsigned)p.ld;
+    __syncthreads();
+    const unsigned long uld = (unsigned long)p.ld;
     for (int o = t; o < nother; o += TS_NT) {
       const int c = p.cl + o + ((inside && p.cl + o >= p.c0) ? w : 0);
       float *Wc = p.W + c;
       float v[2 * TS_W];
 #pragma unroll
-      for (int x = 0; x < 2 * TS_W; ++x) v[x] = Wc[(unsigned long)src[x] * uld];
+      for (int x = 0; x < 2 * TS_W; ++x) v[x] = Wc[mv_row[0][x] * uld];
 #pragma unroll
-      for (int x = 2 * TS_W - 1; x >= 0; --x) Wc[(unsigned long)dstr[x] * uld] = v[x];
+      for (int x = 2 * TS_W - 1; x >= 0; --x) Wc[mv_row[1][x] * uld] = v[x];
     }
     if (p.perm) {
       __syncthreads();
