@@ -50,9 +50,13 @@ const char *last_error();
 struct Workspace {
   std::mutex mu;
   std::unordered_map<std::string, std::pair<void *, size_t>> slots;
+  std::unordered_map<std::string, std::pair<void *, size_t>> hslots;  // pinned host
   // fresh (optional) is set when the slot was (re)allocated: its contents
   // are undefined
   void *get(const std::string &name, size_t bytes, bool *fresh = nullptr);
+  // Pinned, coherent host memory (hipHostMalloc), for words that kernels
+  // of this stream write and the host polls; contents undefined when new.
+  void *get_host(const std::string &name, size_t bytes);
   // the slot's buffer, or null when it was never allocated
   void *find(const std::string &name);
   ~Workspace();

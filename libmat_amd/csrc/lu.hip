@@ -521,21 +521,28 @@ void zero_upper(hipStream_t s, int n, T *A, long lda) {
   MG_HIP(hipGetLastError());
 }
 
-template <typename T>
-__global__ void gather_rows_kernel(int n, int nrhs, const T *src, long lds, const int *perm, T *dst,
-                                   long ldd) {
+template <typename Ti, typename To>
+__global__ void gather_rows_kernel(int n, int nrhs, const Ti *src, long lds, const int *perm,
+                                   To *dst, long ldd) {
   const long idx = (long)blockIdx.x * 256 + threadIdx.x;
   if (idx >= (long)n * nrhs) return;
   const int i = (int)(idx % n), c = (int)(idx / n);
-  dst[i + (long)c * ldd] = src[perm[i] + (long)c * lds];
+  dst[i + (long)c * ldd] = (To)src[perm[i] + (long)c * lds];
+}
+
+template <typename Ti, typename To>
+void gather_rows_cvt(hipStream_t s, int n, int nrhs, const Ti *src, long lds, const int *perm,
+                     To *dst, long ldd) {
+  if (n <= 0 || nrhs <= 0) return;
+  hipLaunchKernelGGL((gather_rows_kernel<Ti, To>), dim3(ceil_div((size_t)n * nrhs, 256)), dim3(256),
+                     0, s, n, nrhs, src, lds, perm, dst, ldd);
+  MG_HIP(hipGetLastError());
 }
 
 template <typename T>
 void gather_rows(hipStream_t s, int n, int nrhs, const T *src, long lds, const int *perm, T *dst,
                  long ldd) {
-  hipLaunchKernelGGL((gather_rows_kernel<T>), dim3(ceil_div((size_t)n * nrhs, 256)), dim3(256), 0, s,
-                     n, nrhs, src, lds, perm, dst, ldd);
-  MG_HIP(hipGetLastError());
+  gather_rows_cvt<T, T>(s, n, nrhs, src, lds, perm, dst, ldd);
 }
 
 template <typename Ti, typename To>
@@ -600,6 +607,8 @@ MG_INST(float)
 MG_INST(double)
 template void convert<double, float>(hipStream_t, long, const double *, float *);
 template void convert<float, double>(hipStream_t, long, const float *, double *);
+template void gather_rows_cvt<double, float>(hipStream_t, int, int, const double *, long,
+                                              const int *, float *, long);
 template void convert2d<double, float>(hipStream_t, int, int, const double *, long, float *, long);
 template void convert2d<float, double>(hipStream_t, int, int, const float *, long, double *, long);
 template void convert2d<double, double>(hipStream_t, int, int, const double *, long, double *, long);

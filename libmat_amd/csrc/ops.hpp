@@ -106,6 +106,10 @@ void zero_upper(hipStream_t s, int n, T *A, long lda);
 template <typename T>
 void gather_rows(hipStream_t s, int n, int nrhs, const T *src, long lds, const int *perm,
                  T *dst, long ldd);
+// The same with a conversion: dst[i + c*ldd] = (To)src[perm[i] + c*lds].
+template <typename Ti, typename To>
+void gather_rows_cvt(hipStream_t s, int n, int nrhs, const Ti *src, long lds, const int *perm,
+                     To *dst, long ldd);
 template <typename Ti, typename To>
 void convert(hipStream_t s, long count, const Ti *src, To *dst);
 template <typename Ti, typename To>

@@ -36,6 +36,24 @@ void *Workspace::get(const std::string &name, size_t bytes, bool *fresh) {
   return p;
 }
 
+void *Workspace::get_host(const std::string &name, size_t bytes) {
+  if (bytes == 0) bytes = 1;
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = hslots.find(name);
+  if (it != hslots.end() && it->second.second >= bytes) return it->second.first;
+  if (it != hslots.end()) {
+    // growing: queued work on this stream may still write the old buffer
+    MG_HIP(hipDeviceSynchronize());
+    MG_HIP(hipHostFree(it->second.first));
+    hslots.erase(it);
+  }
+  void *p = nullptr;
+  const size_t rounded = (bytes + 255) / 256 * 256;
+  MG_HIP(hipHostMalloc(&p, rounded, hipHostMallocCoherent));
+  hslots[name] = {p, rounded};
+  return p;
+}
+
 void *Workspace::find(const std::string &name) {
   std::lock_guard<std::mutex> lk(mu);
   auto it = slots.find(name);

@@ -16,6 +16,7 @@
 //   <= tol (infinity norms) and fall back to the plain fp64 factorization
 //   when refinement stalls (two steps without halving the error, growth,
 //   NaN, or the iteration cap).
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -357,19 +358,43 @@ static void transpose_convert(hipStream_t s, int rows, int cols, const Ti *src, 
   MG_HIP(hipGetLastError());
 }
 
+// The backward error's column maxima go to the host through a mailbox in
+// pinned memory: one workgroup copies them, and the multi-workgroup
+// kernels' error word, with system-scope stores, then raises a tag; the
+// host polls the tag.  (A copy to pageable memory and a stream
+// synchronization cost ~70 us per refinement step: a staged copy and the
+// wake-up of the waiting thread.)
+__global__ void berr_publish_kernel(const unsigned long long *dev, int count, const int *err,
+                                    unsigned long long *mbox, unsigned long long tag) {
+  for (int i = threadIdx.x; i < count; i += blockDim.x)
+    __hip_atomic_store(mbox + i, dev[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  if (threadIdx.x == 0)
+    __hip_atomic_store(mbox + count, (unsigned long long)(unsigned)*err, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_SYSTEM);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+    __hip_atomic_store(mbox + count + 1, tag, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+}
+
+static std::atomic<unsigned long long> g_mbox_tag{0};
+
 // Backward error per column, max over columns, from device vectors.
 struct BerrEval {
   hipStream_t s;
   int n, nrhs;
   double normA;
-  unsigned long long *dev;  // [3][nrhs]: |r|, |x|, |b|
-  std::vector<unsigned long long> host;
-  std::vector<double> bnorm;
+  unsigned long long *dev;   // [3][nrhs]: |r|, |x|, |b|
+  unsigned long long *mbox;  // pinned: [3][nrhs] copies, the error word, the tag
+  int *err;
 
   BerrEval(hipStream_t s_, int n_, int nrhs_, double normA_, const double *B, long ldb)
       : s(s_), n(n_), nrhs(nrhs_), normA(normA_) {
-    dev = (unsigned long long *)workspace(s).get("berr", (size_t)3 * nrhs * 8);
-    host.resize(3 * nrhs);
+    Workspace &ws = workspace(s);
+    dev = (unsigned long long *)ws.get("berr", (size_t)3 * nrhs * 8);
+    mbox = (unsigned long long *)ws.get_host("berr.mbox", ((size_t)3 * nrhs + 2) * 8);
+    err = sync_error_word(s);
     fill_u32(s, (unsigned *)dev, 6L * nrhs, 0u);
     col_amax(s, n, nrhs, B, ldb, dev + 2 * nrhs);
   }
@@ -377,14 +402,37 @@ struct BerrEval {
     fill_u32(s, (unsigned *)dev, 4L * nrhs, 0u);
     col_amax(s, n, nrhs, R, ldr, dev);
     col_amax(s, n, nrhs, X, ldx, dev + nrhs);
-    MG_HIP(hipMemcpyAsync(host.data(), dev, (size_t)3 * nrhs * 8, hipMemcpyDeviceToHost, s));
-    sync_and_check(s);
+    const int count = 3 * nrhs;
+    const unsigned long long tag = ++g_mbox_tag;
+    hipLaunchKernelGGL(berr_publish_kernel, dim3(1), dim3(64), 0, s, dev, count, err, mbox, tag);
+    MG_HIP(hipGetLastError());
+    // poll (a refinement step takes a few ms); past 50 ms, block on the
+    // stream instead, which also reports any error of the queued work
+    unsigned long long *tagp = mbox + count + 1;
+    const auto t0 = std::chrono::steady_clock::now();
+    for (long spin = 0; __atomic_load_n(tagp, __ATOMIC_ACQUIRE) != tag; ++spin) {
+      if ((spin & 1023) == 1023 &&
+          std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(50)) {
+        MG_HIP(hipStreamSynchronize(s));
+        MG_CHECK(__atomic_load_n(tagp, __ATOMIC_ACQUIRE) == tag,
+                 "backward error: the mailbox was not written");
+        break;
+      }
+    }
+    if (__atomic_load_n(mbox + count, __ATOMIC_ACQUIRE) != 0) {
+      MG_HIP(hipMemsetAsync(err, 0, sizeof(int), s));
+      MG_HIP(hipStreamSynchronize(s));
+      throw Error{-5, "a multi-workgroup kernel timed out waiting for another workgroup"};
+    }
     double worst = 0.0;
     for (int c = 0; c < nrhs; ++c) {
+      const unsigned long long hr = __atomic_load_n(mbox + c, __ATOMIC_RELAXED);
+      const unsigned long long hx = __atomic_load_n(mbox + nrhs + c, __ATOMIC_RELAXED);
+      const unsigned long long hb = __atomic_load_n(mbox + 2 * nrhs + c, __ATOMIC_RELAXED);
       double r, x, b;
-      memcpy(&r, &host[c], 8);
-      memcpy(&x, &host[nrhs + c], 8);
-      memcpy(&b, &host[2 * nrhs + c], 8);
+      memcpy(&r, &hr, 8);
+      memcpy(&x, &hx, 8);
+      memcpy(&b, &hb, 8);
       const double den = normA * x + b;
       double e;
       if (!(den == den) || !(r == r)) e = NAN;  // NaN anywhere: never converged
@@ -436,7 +484,6 @@ int gesv_mixed(hipStream_t s, int n, int nrhs, const double *A, long lda, const 
   int *ipiv = (int *)ws.get("gesvm.ipiv", (size_t)n4 * 4);
   int *perm = (int *)ws.get("gesvm.perm", (size_t)n4 * 4);
   double *R = (double *)ws.get("gesvm.R", (size_t)n * nrhs * 8);
-  float *D = (float *)ws.get("gesvm.D", (size_t)n4 * nrhs * 4);
   float *D2 = (float *)ws.get("gesvm.D2", (size_t)n4 * nrhs * 4);
   double *ubf = nullptr, *vbf = nullptr;
   const Mul mul = o.passes == 0 ? Mul::Native : (Mul)o.passes;
@@ -483,8 +530,7 @@ int gesv_mixed(hipStream_t s, int n, int nrhs, const double *A, long lda, const 
       tp.solve(s, nrhs, D2, n4);
       rbt_vec_n(s, n, n4, nrhs, vbf, D2, n4, X, ldx, assign);
     } else {
-      convert2d<double, float>(s, n, nrhs, Rm, ldr, D, n);
-      gather_rows<float>(s, n, nrhs, D, n, perm, D2, n);
+      gather_rows_cvt<double, float>(s, n, nrhs, Rm, ldr, perm, D2, n);
       tp.solve(s, nrhs, D2, n);
       if (assign) convert2d<float, double>(s, n, nrhs, D2, n, X, ldx);
       else
