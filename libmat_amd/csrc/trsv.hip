@@ -422,32 +422,45 @@ __global__ __launch_bounds__(ChainNT<T>::NT) void tri_chain_kernel(int n, int nr
         __syncthreads();
         chain_mfma_a(Sb, LDS, reinterpret_cast<float(&)[32]>(areg));
       }
-      // X_k from its published words
+      // X_k from its published words: each thread polls its first word
+      // alone until it carries this launch's tag (one load a thread in
+      // flight while the block is not there yet), then loads its other
+      // words together and polls only those still stale, so the block costs
+      // one more memory round trip after it lands instead of one per word
       const uint64_t *src = pub + (long)k * BW;
       bool timeout = false;
+      constexpr int PW = TB * NC / NT * NW;  // words per thread
+      uint64_t wd[PW];
+      auto wptr = [&](int x) {  // word x of this thread: u = x / NW, h = x % NW
+        return src + (long)(t + (x / NW) * NT) * NW + x % NW;
+      };
+      auto poll = [&](int x) {
+        long spins = 0;
+        while ((unsigned)(wd[x] >> 32) != tag) {
+          if (++spins > CHAIN_SPIN) {
+            timeout = true;
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+          wd[x] = __hip_atomic_load(wptr(x), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+      };
+      wd[0] = __hip_atomic_load(wptr(0), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      poll(0);
+#pragma unroll
+      for (int x = 1; x < PW; ++x)
+        wd[x] = __hip_atomic_load(wptr(x), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+      for (int x = 1; x < PW; ++x) poll(x);
 #pragma unroll
       for (int u = 0; u < TB * NC / NT; ++u) {
         const int e = t + u * NT;  // = c * TB + j
-        uint32_t w[NW];
-#pragma unroll
-        for (int h = 0; h < NW; ++h) {
-          uint64_t wd = __hip_atomic_load(src + (long)e * NW + h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          long spins = 0;
-          while ((unsigned)(wd >> 32) != tag) {
-            if (++spins > CHAIN_SPIN) {
-              timeout = true;
-              break;
-            }
-            __builtin_amdgcn_s_sleep(1);
-            wd = __hip_atomic_load(src + (long)e * NW + h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          }
-          w[h] = (uint32_t)wd;
-        }
         T val;
         if constexpr (NW == 1) {
-          val = __uint_as_float(w[0]);
+          val = __uint_as_float((uint32_t)wd[u]);
         } else {
-          val = __longlong_as_double((long long)(((unsigned long long)w[1] << 32) | w[0]));
+          val = __longlong_as_double(
+              (long long)(((unsigned long long)(uint32_t)wd[2 * u + 1] << 32) | (uint32_t)wd[2 * u]));
         }
         Xs[(e % TB) * NC + e / TB] = val;
       }
