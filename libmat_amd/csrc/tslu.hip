@@ -384,10 +384,22 @@ __global__ __launch_bounds__(TS_NT) void tslu_kernel(TsluP p) {
   float *U12s = orig + TS_NT * TS_RS;
   if (fused) {
     const float *lprev = p.linv + (long)p.prev_c0 * TS_W;
-    for (int e = t; e < TS_W * TS_W; e += TS_NT) {
-      const int i = e / TS_W, c = e % TS_W;
-      un[i * TS_RS + c] = lprev[e];
-      U12s[i * TS_RS + c] = c < w ? p.W[(long)(p.prev_c0 + i) * p.ld + p.c0 + c] : 0.0f;
+    {
+      // all loads issued before any is used (one memory round trip)
+      constexpr int FR = TS_W * TS_W / TS_NT;
+      float lv[FR], av[FR];
+#pragma unroll
+      for (int j = 0; j < FR; ++j) {
+        const int e = t + j * TS_NT, i = e / TS_W, c = e % TS_W;
+        lv[j] = lprev[e];
+        av[j] = c < w ? p.W[(long)(p.prev_c0 + i) * p.ld + p.c0 + c] : 0.0f;
+      }
+#pragma unroll
+      for (int j = 0; j < FR; ++j) {
+        const int e = t + j * TS_NT, i = e / TS_W, c = e % TS_W;
+        un[i * TS_RS + c] = lv[j];
+        U12s[i * TS_RS + c] = av[j];
+      }
     }
     __syncthreads();
     const int i = t >> 3, cq = 4 * (t & 7);
@@ -558,11 +570,25 @@ __global__ __launch_bounds__(TS_NT) void tslu_kernel(TsluP p) {
   const int mwg = p.G - 1;
   const int lwg = p.G >= 3 ? p.G - 2 : p.G - 1;
   // the final pivot rows (factored) and their indices, once from memory
+  // (every load issued before any is used: one memory round trip, where a
+  // loop of load-then-store took one per element)
   float *Fs = orig + TS_NT * TS_RS;  // [TS_W][TS_RS]
-  for (int e = t; e < TS_W * TS_W; e += TS_NT) Fs[(e / TS_W) * TS_RS + e % TS_W] = ts_getf(p.F + e);
-  if (t < TS_W) {
-    fin[t] = t < w ? ts_get(p.fin + t) : -1;
-    windst[t] = -1;
+  {
+    static_assert(TS_W * TS_W % TS_NT == 0, "whole rounds of F");
+    constexpr int FR = TS_W * TS_W / TS_NT;
+    float fv[FR];
+#pragma unroll
+    for (int j = 0; j < FR; ++j) fv[j] = ts_getf(p.F + t + j * TS_NT);
+    const int fi = t < w ? ts_get(p.fin + t) : -1;
+#pragma unroll
+    for (int j = 0; j < FR; ++j) {
+      const int e = t + j * TS_NT;
+      Fs[(e / TS_W) * TS_RS + e % TS_W] = fv[j];
+    }
+    if (t < TS_W) {
+      fin[t] = fi;
+      windst[t] = -1;
+    }
   }
   __syncthreads();
   if ((b == 0 || b == mwg) && wave == 0) {
