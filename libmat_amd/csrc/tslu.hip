@@ -15,9 +15,11 @@
 //      pivoting on its own rows (no interchanges: a chosen row leaves the
 //      competition and is no longer updated, so it ends as its row of the
 //      factored block);
-//   2. the workgroups publish their winners' row indices; the last of every
-//      group of eight to arrive (an atomic ticket) plays the group's 256
-//      winners again on their original values, and so on up the tree (two
+//   2. the workgroups publish their winners' row indices and rows (the
+//      values their playoff began with); the last of every group of eight
+//      to arrive (an atomic ticket) loads the group's 256 winners (index
+//      and row in one memory round trip) and plays them, and so on up the
+//      tree (two
 //      levels above the leaves for up to 16384 rows), until one workgroup
 //      plays the final round, writes the factored w x w block [L11 \ U11]
 //      and the pivot rows, and raises a flag;
@@ -32,9 +34,9 @@
 //      lower 32 x 32 block for the block triangular solves (trsm.hip).
 // The second leaf of a 64-column node also does the node's update first:
 // U12 = L11^-1 A12 from the first leaf's inverse, then every workgroup its
-// rows' A22 -= L21 U12, written back before its leaf playoff (the tree
-// nodes read the candidates' rows from memory after the arrival ticket,
-// which the release fence before it orders after those writes).
+// rows' A22 -= L21 U12 in LDS before its leaf playoff (the tree nodes get
+// the updated rows from the published winners, and every row of the leaf's
+// columns is written once at the end).
 // A playoff runs on all four SIMDs of a compute unit: every step, each
 // wavefront finds its candidate (DPP max + ballot) and parks the
 // candidate's row in LDS, one barrier, every wavefront takes the best of the
@@ -43,9 +45,14 @@
 // per four or more cycles, so 64 rows there cost about half as much per step
 // as 256 rows here).  Rows per lane are a parameter (TS_RPT): four rows per
 // lane took 2200 cycles a step, more than the level they save.
-// Every playoff reads the rows' values as they were when the launch began
+// Every playoff plays the rows' values as they were when the launch began
 // (after the fused update): the leaf's columns are written only after the
 // final round, so no workgroup overwrites a row another one may still read.
+// An agent-scope release writes the L2 back, which the side stream's GEMM
+// keeps dirty (a release fence in each of the four wavefronts instead of
+// one cost ~7 us per leaf, measured), so a tree hand-off has one: the
+// published rows are written by wavefront 0 alone, whose fence orders them
+// before the ticket, which is itself relaxed.
 #include <algorithm>
 #include <atomic>
 #include <cstdint>
@@ -77,6 +84,9 @@ struct TsluP {
   int *err;       // set to 1 on a poll timeout
   int *pub;       // [TS_LMAX][TS_GMAX][TS_W] winners' row indices per tree node
   unsigned *cnt;  // [TS_LMAX][TS_GMAX] arrival tickets (wrap to 0 after the last)
+  float *prow;    // [TS_LMAX][TS_GMAX][TS_W][TS_W] winners' rows per tree node (the
+                  // values their playoff began with; plain stores, ordered by the
+                  // release fence before the ticket)
   float *F;       // [TS_W][TS_W] factored pivot rows of the final round
   int *fin;       // [TS_W] pivot rows (original indices), in order
   uint64_t *done; // tagged word: the final round is written
@@ -418,7 +428,7 @@ __global__ __launch_bounds__(TS_NT) void tslu_kernel(TsluP p) {
   }
 
   // 1. own rows, original values, for the substitution (in a fused launch
-  //    after A22 -= L21 U12, written back for the tree's reads)
+  //    after A22 -= L21 U12)
   {
     float x[TS_W];
     if (mine) ts_load_row(x, p.W + (long)myrow * p.ld + p.c0, w, vec);
@@ -443,7 +453,6 @@ __global__ __launch_bounds__(TS_NT) void tslu_kernel(TsluP p) {
       }
 #pragma unroll
       for (int c = 0; c < TS_W; ++c) x[c] = c < w ? x[c] - acc[c] : 0.0f;
-      ts_store_row(p.W + (long)myrow * p.ld + p.c0, x, w, vec);
     }
 #pragma unroll
     for (int c4 = 0; c4 < TS_W / 4; ++c4)
@@ -500,13 +509,40 @@ __global__ __launch_bounds__(TS_NT) void tslu_kernel(TsluP p) {
         const int s = t < w ? win[t] : -1;
         ts_put(p.pub + ((long)level * TS_GMAX + node) * TS_W + t, s >= 0 ? cid[s] : -1);
       }
+      if (t < w && win[t] >= 0) {
+        // the winner's row for the next level, copied by wavefront 0 (whose
+        // release fence below orders it): a leaf's candidates are its own
+        // rows (LDS), a node's were read from the level below (still in
+        // this XCD's L2)
+        const int sw = win[t];
+        float4 *d = reinterpret_cast<float4 *>(
+            p.prow + (((long)level * TS_GMAX + node) * TS_W + t) * TS_W);
+        float4 q[TS_W / 4];
+        if (level == 0) {
+#pragma unroll
+          for (int c4 = 0; c4 < TS_W / 4; ++c4)
+            q[c4] = *reinterpret_cast<const float4 *>(orig + sw * TS_RS + 4 * c4);
+        } else {
+          const float4 *src = reinterpret_cast<const float4 *>(
+              p.prow + (((long)(level - 1) * TS_GMAX + node * TS_FAN + sw / TS_W) * TS_W +
+                        sw % TS_W) * TS_W);
+#pragma unroll
+          for (int c4 = 0; c4 < TS_W / 4; ++c4) q[c4] = src[c4];
+        }
+#pragma unroll
+        for (int c4 = 0; c4 < TS_W / 4; ++c4) d[c4] = q[c4];
+      }
       const int parent = node / TS_FAN;
       const int members = min(TS_FAN, nodes - parent * TS_FAN);
+      // wavefront 0 wrote everything the next level reads (indices and
+      // rows): its release fence, then the ticket (relaxed: the fence
+      // releases, the acquire fence below acquires; an acq_rel ticket
+      // would write the L2 back a second time)
       if (wave == 0) {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         if (lane == 0) {
           const unsigned old = __builtin_amdgcn_atomic_inc32(
-              p.cnt + (long)level * TS_GMAX + parent, (unsigned)(members - 1), __ATOMIC_ACQ_REL,
+              p.cnt + (long)level * TS_GMAX + parent, (unsigned)(members - 1), __ATOMIC_RELAXED,
               "agent");
           s_go = old == (unsigned)(members - 1);
         }
@@ -515,20 +551,26 @@ __global__ __launch_bounds__(TS_NT) void tslu_kernel(TsluP p) {
       if (!s_go) break;
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
       // this workgroup plays node (level + 1, parent): its members' winners,
-      // member k's j-th winner in slot 32k + j
+      // member k's j-th winner in slot 32k + j; the index and the row are
+      // loaded together (the row of an empty slot is not this launch's and
+      // is dropped)
       const int m0 = parent * TS_FAN;
 #pragma unroll
       for (int q = 0; q < TS_RPT; ++q) {
         const int sl = q * TS_NT + t;
         const int mem = sl / TS_W, j = sl % TS_W;
+        const bool slot = mem < members && j < w;
+        const long nd = (long)level * TS_GMAX + m0 + mem;
         int id = -1;
-        if (mem < members && j < w) id = ts_get(p.pub + ((long)level * TS_GMAX + m0 + mem) * TS_W + j);
-        cid[sl] = id;
-        r.live[q] = id >= 0;
-        if (id >= 0) ts_load_row(r.a[q], p.W + (long)id * p.ld + p.c0, w, vec);
-        else
+        if (slot) {
+          id = ts_get(p.pub + nd * TS_W + j);
+          ts_load_row(r.a[q], p.prow + (nd * TS_W + j) * TS_W, TS_W, true);
+        }
+        if (id < 0)
 #pragma unroll
           for (int c = 0; c < TS_W; ++c) r.a[q][c] = 0.0f;
+        cid[sl] = id;
+        r.live[q] = id >= 0;
       }
       ++level;
       node = parent;
@@ -781,7 +823,8 @@ void tslu_leaf(hipStream_t s, int n, float *W, long ld, int c0, int w, int *ipiv
   const size_t pub_bytes = (size_t)TS_LMAX * TS_GMAX * TS_W * sizeof(int);
   const size_t f_off = pub_off + pub_bytes;
   const size_t fin_off = f_off + (size_t)TS_W * TS_W * sizeof(float);
-  const size_t bytes = fin_off + TS_W * sizeof(int);
+  const size_t prow_off = fin_off + 256;
+  const size_t bytes = prow_off + (size_t)TS_LMAX * TS_GMAX * TS_W * TS_W * sizeof(float);
   bool fresh = false;
   char *buf = (char *)ws.get("tslu.ws", bytes, &fresh);
   if (fresh) fill_u32(s, (unsigned *)buf, (long)(pub_off / 4), 0u);
@@ -790,6 +833,7 @@ void tslu_leaf(hipStream_t s, int n, float *W, long ld, int c0, int w, int *ipiv
   p.pub = (int *)(buf + pub_off);
   p.F = (float *)(buf + f_off);
   p.fin = (int *)(buf + fin_off);
+  p.prow = (float *)(buf + prow_off);
   unsigned tag = epoch.fetch_add(1, std::memory_order_relaxed) + 1;
   if (tag == 0) tag = epoch.fetch_add(1, std::memory_order_relaxed) + 1;
   p.tag = tag;
