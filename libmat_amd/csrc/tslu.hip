@@ -377,6 +377,7 @@ __global__ __launch_bounds__(TS_NT) void tslu_kernel(TsluP p) {
   __shared__ int cid[TS_PR];     // candidate row of every slot in the current playoff
   __shared__ int win[TS_W];      // winner slots of the current playoff
   __shared__ int fin[TS_W];      // pivot rows, in order
+  __shared__ int s_fidx[TS_W];   // the final round's pivot rows, before publication
   __shared__ int windst[TS_W];   // where each window row goes (-1: into the window)
   __shared__ int s_pairs[PAIR_STRIDE];
   __shared__ unsigned mv_row[2][2 * TS_W];  // row moves: padded src, dst rows
@@ -487,22 +488,33 @@ __global__ __launch_bounds__(TS_NT) void tslu_kernel(TsluP p) {
       const bool top = nodes == 1;
       ts_play(r, w, p.eps, xg, win);
       if (top) {
-        // the final round: factored pivot rows and their indices
+        // the final round: factored pivot rows and their indices, gathered
+        // in LDS (the substitution's buffer, free until the tail) and
+        // written by wavefront 0 behind one release (see above)
+        static_assert(TS_RPT == 1, "one playing row per lane");
+        if (r.pick[0] >= 0) {
 #pragma unroll
-        for (int q = 0; q < TS_RPT; ++q) {
-          if (r.pick[q] >= 0) {
-            float *d = p.F + r.pick[q] * TS_W;
-#pragma unroll
-            for (int c = 0; c < TS_W; ++c) ts_putf(d + c, r.a[q][c]);
-            ts_put(p.fin + r.pick[q], cid[q * TS_NT + t]);
-          }
+          for (int c4 = 0; c4 < TS_W / 4; ++c4)
+            *reinterpret_cast<float4 *>(un + r.pick[0] * TS_RS + 4 * c4) =
+                make_float4(r.a[0][4 * c4], r.a[0][4 * c4 + 1], r.a[0][4 * c4 + 2],
+                            r.a[0][4 * c4 + 3]);
+          s_fidx[r.pick[0]] = cid[t];
         }
-        if (t >= w && t < TS_W) ts_put(p.fin + t, -1);
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         __syncthreads();
-        if (t == 0)
-          __hip_atomic_store(p.done, tagged(p.tag, 1u), __ATOMIC_RELEASE,
-                             __HIP_MEMORY_SCOPE_AGENT);
+        if (wave == 0) {
+          if (lane < TS_W) {
+            // every j < w has a winner (the final round has at least w rows)
+            float4 *d = reinterpret_cast<float4 *>(p.F + lane * TS_W);
+#pragma unroll
+            for (int c4 = 0; c4 < TS_W / 4; ++c4)
+              d[c4] = *reinterpret_cast<const float4 *>(un + lane * TS_RS + 4 * c4);
+            p.fin[lane] = lane < w ? s_fidx[lane] : -1;
+          }
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+          if (lane == 0)
+            __hip_atomic_store(p.done, tagged(p.tag, 1u), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        }
         break;
       }
       if (t < TS_W) {
