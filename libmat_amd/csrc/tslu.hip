@@ -595,7 +595,9 @@ __global__ __launch_bounds__(TS_NT) void tslu_kernel(TsluP p) {
   if (t == 0) {
     long spins = 0;
     s_ok = 1;
-    while (__hip_atomic_load(p.done, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) !=
+    // relaxed polls (an acquire poll invalidates caches on every read); the
+    // acquire fence after the barrier below orders what follows
+    while (__hip_atomic_load(p.done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) !=
            tagged(p.tag, 1u)) {
       if (++spins > SPIN_LIMIT) {
         s_ok = 0;
