@@ -70,6 +70,7 @@ constexpr int TS_RPT = 1;    // playing rows per lane
 constexpr int TS_PR = TS_NT * TS_RPT;  // rows (candidates) of one playoff
 constexpr int TS_W = 32;     // widest leaf
 constexpr int TS_RS = 36;    // LDS row stride in floats: 16-byte aligned rows
+constexpr int TS_XS = 64;    // stride of the per-step exchange slots (a shift to address)
 constexpr int TS_FAN = TS_PR / TS_W;  // tree nodes per node of the next level (8)
 constexpr int TS_LMAX = 3;   // tree levels above the leaves: 256 -> 32 -> 4 -> 1
 constexpr int TS_GMAX = 256;  // workgroups (one per 256 rows), all resident
@@ -153,7 +154,7 @@ __device__ __forceinline__ float ts_getf(const float *p) {
 // (one barrier per step: a buffer is rewritten two steps later, after every
 // wavefront has passed the barrier that follows its reads).
 struct __align__(16) TsXchg {
-  float row[2][4][TS_RS];  // per wavefront: its candidate's row (columns J..)
+  float row[2][4][TS_XS];  // per wavefront: its candidate's row (columns J..)
   unsigned key[2][4];      // per wavefront: (rank << 10) | (1023 - slot)
 };
 
@@ -233,23 +234,30 @@ __device__ __forceinline__ void ts_step(TsRows &r, const bool (&qlive)[TS_RPT], 
   }
   __syncthreads();
   const uint4 kq = *reinterpret_cast<const uint4 *>(x.key[par]);
-  const unsigned best =
-      __builtin_amdgcn_readfirstlane(max(max(kq.x, kq.y), max(kq.z, kq.w)));
+  // the best key, the same in every lane; the winner's LDS row address is
+  // formed from it in vector registers (the scalar copy, which takes a
+  // VALU-to-SALU round trip, only decides what is off the LDS chain)
+  const unsigned bestv = max(max(kq.x, kq.y), max(kq.z, kq.w));
+  const unsigned best = __builtin_amdgcn_readfirstlane(bestv);
   const bool found = (best >> 10) != 0;
   const int slot = 1023 - (int)(best & 1023u);
-  const int ww = (slot >> 6) & 3, wqq = slot >> 8, wlw = slot & 63;
+  const int wqq = slot >> 8, wlw = slot & 63;
+  const int ww = (slot >> 6) & 3;
+  const int wwv = (int)((~bestv >> 6) & 3u);  // = ww
   mywin = t == J ? (found ? slot : -1) : mywin;
-  // the winner's row in whole 16-byte chunks (the empty asm marks every
-  // element as used, so each chunk stays one ds_read_b128 even where its
-  // first elements are not needed)
+  // the winner's row in whole 16-byte chunks; every load is issued first and
+  // each chunk is waited for where its columns are used.  The first chunk
+  // holds columns before J when J % 4 != 0: the empty asm marks them as
+  // used, so it stays one ds_read_b128
   float u[TS_W];
-  const ts_f4 *src = reinterpret_cast<const ts_f4 *>(x.row[par][ww]);
+  const ts_f4 *src =
+      reinterpret_cast<const ts_f4 *>(&x.row[par][0][0] + wwv * TS_XS);
   ts_f4 qv[TS_W / 4];
 #pragma unroll
   for (int c4 = J / 4; c4 < TS_W / 4; ++c4) qv[c4] = src[c4];
+  if constexpr (J % 4 != 0) asm volatile("" : "+v"(qv[J / 4]));
 #pragma unroll
   for (int c4 = J / 4; c4 < TS_W / 4; ++c4) {
-    asm volatile("" : "+v"(qv[c4]));
     u[4 * c4] = qv[c4].x;
     u[4 * c4 + 1] = qv[c4].y;
     u[4 * c4 + 2] = qv[c4].z;
@@ -261,7 +269,10 @@ __device__ __forceinline__ void ts_step(TsRows &r, const bool (&qlive)[TS_RPT], 
   const float inv = elim ? ts_recip(pv) : 1.0f;
 #pragma unroll
   for (int q = 0; q < TS_RPT; ++q) {
-    if (!qlive[q]) continue;
+    // a full playoff updates every row (lm = 0 where not live): no branch
+    // between this update and the next step's reduction, so the two share
+    // a basic block and their instructions can interleave
+    if (!FULL && !qlive[q]) continue;
     const bool me = mine && q == wqq;
     r.pick[q] = me ? J : r.pick[q];
     r.live[q] = r.live[q] && !me;
