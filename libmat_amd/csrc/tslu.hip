@@ -92,7 +92,8 @@ struct TsluP {
   int *fin;       // [TS_W] pivot rows (original indices), in order
   uint64_t *done; // tagged word: the final round is written
   unsigned tag;   // distinct per launch, never 0
-  int G;          // workgroups (one per 256 rows)
+  int G;          // workgroups holding rows (one per 256 rows); two more are launched
+                  // for the tail (row moves, inverse)
   int P;          // leaf workgroups of the tree (one per TS_PR rows; = G here)
   int prio;       // wave priority (wave_prio of the launch stream)
   float *linv;    // if not null: the inverse of the leaf's unit lower 32 x 32
@@ -402,6 +403,14 @@ __global__ __launch_bounds__(TS_NT) void tslu_kernel(TsluP p) {
   //    write-back and in LDS (the final pivot rows' place, unused until
   //    then) for the update of the own rows below
   const bool fused = p.prev_c0 >= 0;
+  // the own row and, in a fused launch, its row of the first leaf's L21 are
+  // loaded first, so that their memory round trip overlaps forming U12
+  float x[TS_W], lp[TS_W];
+  if (mine) ts_load_row(x, p.W + (long)myrow * p.ld + p.c0, w, vec);
+  else
+#pragma unroll
+    for (int c = 0; c < TS_W; ++c) x[c] = 0.0f;
+  if (fused && mine) ts_load_row(lp, p.W + (long)myrow * p.ld + p.prev_c0, TS_W, vec);
   float u4[4] = {0.0f, 0.0f, 0.0f, 0.0f};  // U12 row t / 8, columns 4 (t % 8) ..
   float *U12s = orig + TS_NT * TS_RS;
   if (fused) {
@@ -442,14 +451,8 @@ __global__ __launch_bounds__(TS_NT) void tslu_kernel(TsluP p) {
   // 1. own rows, original values, for the substitution (in a fused launch
   //    after A22 -= L21 U12)
   {
-    float x[TS_W];
-    if (mine) ts_load_row(x, p.W + (long)myrow * p.ld + p.c0, w, vec);
-    else
-#pragma unroll
-      for (int c = 0; c < TS_W; ++c) x[c] = 0.0f;
     if (fused && mine) {
-      float lp[TS_W], acc[TS_W];
-      ts_load_row(lp, p.W + (long)myrow * p.ld + p.prev_c0, TS_W, vec);
+      float acc[TS_W];
 #pragma unroll
       for (int c = 0; c < TS_W; ++c) acc[c] = 0.0f;
 #pragma unroll
@@ -630,12 +633,15 @@ __global__ __launch_bounds__(TS_NT) void tslu_kernel(TsluP p) {
       if (cq + cc < w) p.W[(long)(p.prev_c0 + i) * p.ld + p.c0 + cq + cc] = u4[cc];
   }
   // The tail is spread over the workgroups so that no one of them carries
-  // all of it: workgroup 0 writes the window, ipiv and the row pairs; the
-  // last workgroup (mwg) applies the interchanges to the other columns and
-  // perm; the one before it (lwg) forms the inverse of L11.  Each of them
-  // also computes its own rows of L21.  (One workgroup: all of it.)
-  const int mwg = p.G - 1;
-  const int lwg = p.G >= 3 ? p.G - 2 : p.G - 1;
+  // all of it: workgroup 0 writes the window, ipiv and the row pairs (and
+  // its rows of L21); two workgroups launched past the rows' G, which hold
+  // no rows and sit out the tree, take the rest: G applies the interchanges
+  // to the other columns and perm (~10 us at 16384 rows: the rows moved are
+  // scattered over the panel), G + 1 forms the inverse of L11.  The launch
+  // then ends with the longest of these instead of a sum (the row moves
+  // followed by a workgroup's own L21 rows set its end before).
+  const int mwg = p.G;
+  const int lwg = p.G + 1;
   // the final pivot rows (factored) and their indices, once from memory
   // (every load issued before any is used: one memory round trip, where a
   // loop of load-then-store took one per element)
@@ -871,7 +877,7 @@ void tslu_leaf(hipStream_t s, int n, float *W, long ld, int c0, int w, int *ipiv
   p.ch = ch;
   p.prev_c0 = prev_c0;
   KScope ks(s, "panel", 2.0 * m * w * sizeof(float));
-  hipLaunchKernelGGL(tslu_kernel, dim3(p.G), dim3(TS_NT), 0, s, p);
+  hipLaunchKernelGGL(tslu_kernel, dim3(p.G + 2), dim3(TS_NT), 0, s, p);
   MG_HIP(hipGetLastError());
 }
 

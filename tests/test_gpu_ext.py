@@ -117,13 +117,15 @@ def _perm_parity(p):
     return par
 
 
-@pytest.mark.parametrize("n", [32, 100, 300, 1100, 3000, 9000, 16500])
+@pytest.mark.parametrize("n", [32, 100, 300, 1100, 2050, 3000, 9000, 16500])
 def test_getrf_tournament(gpu, n):
     """Tournament pivoting (fp32), one workgroup per 256 rows and eight
     workgroups per node of the reduction tree: one workgroup (n <= 256), one
     level above the leaves (n = 300: 2 workgroups; 1100: 5), two levels
     (3000: 12 workgroups, look-ahead driver; 9000: 36) and three (16500: 65
-    workgroups, a ragged last node).  Checked by what the mathematics fixes, since
+    workgroups, a ragged last node); 2050 has rows that are not 16-byte
+    aligned, so its leaves move rows element by element, the others by
+    16-byte quads.  Checked by what the mathematics fixes, since
     the pivots are not libmat's: perm is a permutation whose parity the
     swap count gives, PA = LU to fp32 rounding (fp64 on the GPU), the
     multipliers stay small (the tournament's growth is close to partial
