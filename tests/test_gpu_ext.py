@@ -124,8 +124,8 @@ def test_getrf_tournament(gpu, n):
     level above the leaves (n = 300: 2 workgroups; 1100: 5), two levels
     (3000: 12 workgroups, look-ahead driver; 9000: 36) and three (16500: 65
     workgroups, a ragged last node); 2050 has rows that are not 16-byte
-    aligned, so its leaves move rows element by element, the others by
-    16-byte quads.  Checked by what the mathematics fixes, since
+    aligned, so its leaves load and store their rows element by element
+    instead of in 16-byte vectors.  Checked by what the mathematics fixes, since
     the pivots are not libmat's: perm is a permutation whose parity the
     swap count gives, PA = LU to fp32 rounding (fp64 on the GPU), the
     multipliers stay small (the tournament's growth is close to partial
