@@ -28,10 +28,12 @@
 //      the elimination: multiply by 1/pivot, then the rank-1 updates, and
 //      no scaling where |pivot| < eps); workgroup 0 writes [L11 \ U11] into
 //      the top w rows and the rows those displaced into the pivots' old
-//      places, turns the pivot order into libmat's sequential interchanges
-//      (ipiv) and the leaf's row pairs, applies them to the other columns of
-//      the block ([cl, ch)) and to perm, and writes the inverse of the unit
-//      lower 32 x 32 block for the block triangular solves (trsm.hip).
+//      places and turns the pivot order into libmat's sequential
+//      interchanges (ipiv) and the leaf's row pairs; two more workgroups,
+//      launched past the ones holding rows, apply the interchanges to the
+//      other columns of the block ([cl, ch)) and to perm, and write the
+//      inverse of the unit lower 32 x 32 block for the block triangular
+//      solves (trsm.hip).
 // The second leaf of a 64-column node also does the node's update first:
 // U12 = L11^-1 A12 from the first leaf's inverse, then every workgroup its
 // rows' A22 -= L21 U12 in LDS before its leaf playoff (the tree nodes get
