@@ -23,8 +23,9 @@
 //  * One persistent kernel per (bin, kind) class, so each class gets the
 //    registers and occupancy its matrices need: its wavefronts pull tasks
 //    (one task = the matrices of one wavefront) from the class's atomic
-//    counter.  The launches (one counter reset, then the class kernels in
-//    order) are captured once in a hipGraph and replayed.
+//    counter.  The launches (a counter reset, then every class kernel
+//    depending on it alone, so the classes run side by side) are built
+//    once into a hipGraph of explicit kernel nodes and replayed.
 //  * Autotuning (coordinate descent, graph replays timed on the actual
 //    batch) picks the bins in use (the order thresholds: a small bin can be
 //    merged into the next), the wavefronts per workgroup of every class
@@ -503,17 +504,89 @@ void launch(hipStream_t s, Plan &pl) {
   for (int i = 0; i < pl.ncls; ++i) launch_one(s, pl, pl.cls_order[i]);
 }
 
+// The graph of a plan, built from explicit kernel nodes: the counters'
+// reset (a kernel, not a memset node), then every class kernel depending on
+// it alone, so the classes run side by side (they share nothing: each pulls
+// its own tasks from its own counter and writes its own matrices).  No
+// capture, so no event or side stream is involved: the fork/join capture
+// this replaces destroyed its events right after instantiation while the
+// graph could still refer to them.
+__global__ void batch_reset_kernel(unsigned *c, int n) {
+  if ((int)threadIdx.x < n) c[threadIdx.x] = 0u;
+}
+
+struct ClassLaunch {
+  const void *fn;
+  int grid, block;
+};
+
+template <int NB, int KIND, int W>
+ClassLaunch class_launch_w(const Plan &pl, int c) {
+  const int t0 = pl.cls_t0[c], t1 = pl.cls_t1[c];
+  // the grid of launch_class_w
+  const int grid = std::max(1, std::min((int)mg::ceil_div(t1 - t0, W),
+                                        mg::num_cus() * pl.wg_per_cu * 4 / W));
+  return {reinterpret_cast<const void *>(&mg::batched_kernel<NB, KIND, W>), grid, 64 * W};
+}
+
+template <int NB, int KIND>
+ClassLaunch class_launch(const Plan &pl, int c) {
+  switch (pl.waves[c]) {
+    case 2: return class_launch_w<NB, KIND, 2>(pl, c);
+    case 8: return class_launch_w<NB, KIND, 8>(pl, c);
+    default: return class_launch_w<NB, KIND, 4>(pl, c);
+  }
+}
+
+ClassLaunch class_launch_of(const Plan &pl, int c) {
+  switch (c) {
+    case mg::class_of(0, 0): return class_launch<8, 0>(pl, c);
+    case mg::class_of(0, 1): return class_launch<8, 1>(pl, c);
+    case mg::class_of(1, 0): return class_launch<16, 0>(pl, c);
+    case mg::class_of(1, 1): return class_launch<16, 1>(pl, c);
+    case mg::class_of(2, 0): return class_launch<32, 0>(pl, c);
+    case mg::class_of(2, 1): return class_launch<32, 1>(pl, c);
+    case mg::class_of(3, 0): return class_launch<64, 0>(pl, c);
+    default: return class_launch<64, 1>(pl, c);
+  }
+}
+
 void capture(hipStream_t s, Plan &pl) {
   if (pl.exec) MG_HIP(hipGraphExecDestroy(pl.exec));
   if (pl.graph) MG_HIP(hipGraphDestroy(pl.graph));
-  hipStream_t cs;
-  MG_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+  pl.exec = nullptr;
+  pl.graph = nullptr;
   MG_HIP(hipStreamSynchronize(s));
-  MG_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-  launch(cs, pl);
-  MG_HIP(hipStreamEndCapture(cs, &pl.graph));
+  MG_HIP(hipGraphCreate(&pl.graph, 0));
+  unsigned *ctr = pl.d_counter;
+  int ncl = mg::NCLASS;
+  void *rargs[] = {&ctr, &ncl};
+  hipKernelNodeParams rp{};
+  rp.func = reinterpret_cast<void *>(&batch_reset_kernel);
+  rp.gridDim = dim3(1);
+  rp.blockDim = dim3(64);
+  rp.sharedMemBytes = 0;
+  rp.kernelParams = rargs;
+  rp.extra = nullptr;
+  hipGraphNode_t reset;
+  MG_HIP(hipGraphAddKernelNode(&reset, pl.graph, nullptr, 0, &rp));
+  for (int i = 0; i < pl.ncls; ++i) {
+    const int c = pl.cls_order[i];
+    const ClassLaunch cl = class_launch_of(pl, c);
+    mg::BatchP bp = pl.p;
+    int t0 = pl.cls_t0[c], t1 = pl.cls_t1[c], cc = c;
+    void *args[] = {&bp, &t0, &t1, &cc};  // copied into the node
+    hipKernelNodeParams kp{};
+    kp.func = const_cast<void *>(cl.fn);
+    kp.gridDim = dim3(cl.grid);
+    kp.blockDim = dim3(cl.block);
+    kp.sharedMemBytes = 0;
+    kp.kernelParams = args;
+    kp.extra = nullptr;
+    hipGraphNode_t node;
+    MG_HIP(hipGraphAddKernelNode(&node, pl.graph, &reset, 1, &kp));
+  }
   MG_HIP(hipGraphInstantiate(&pl.exec, pl.graph, nullptr, nullptr, 0));
-  MG_HIP(hipStreamDestroy(cs));
 }
 
 double time_replays(hipStream_t s, Plan &pl, int reps) {
